@@ -1,0 +1,115 @@
+/*
+ * libodba_hip: MI355X-native DTW / DBA hot path of OpenDBA behind a C ABI.
+ *
+ * Plain pointers and sizes only. Host memory in, host memory out unless a
+ * function says otherwise; the library owns device residency. Every function
+ * returns 0 (ODBA_OK), a reference exit code (exit_codes.hpp:4-32) or an
+ * ODBA_E_* code >= 100 (odba_types.h); odba_last_error() gives the message.
+ * One odba_ctx per host thread; a ctx owns one HIP stream on one device.
+ *
+ * Value types: ODBA_I32 / ODBA_U32 / ODBA_U64 / ODBA_F32 / ODBA_F64, the
+ * template instantiations of the reference (openDBA.cu:121-143). Sequences
+ * are arrays of that type; lengths are element counts.
+ */
+#ifndef ODBA_H
+#define ODBA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "odba_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct odba_ctx odba_ctx;
+typedef struct odba_seqset odba_seqset;
+
+/* Context on HIP device `device` (replaces the implicit cudaSetDevice/stream
+ * pairs of dba.hpp:54-63 and the device queries of cuda_utils.hpp:42-57). */
+int odba_ctx_create(int device, odba_ctx** out);
+int odba_ctx_destroy(odba_ctx* ctx);
+const char* odba_last_error(const odba_ctx* ctx);
+/* Compute units, peak engine clock (kHz), device memory (bytes), name. */
+int odba_device_info(odba_ctx* ctx, int* compute_units, int* clock_khz, uint64_t* mem_bytes, char* name, int name_len);
+
+/*
+ * All-vs-all DTW distances in the given order: pair (r, c > r) aligns
+ * Y = seqs[r] (rows) against X = seqs[c] (columns) and dist_out[k] receives
+ * the pair's distance at the row-major upper-triangle index
+ * k = n(n-1)/2 - (n-r)(n-r-1)/2 + (c-r-1).
+ * Replaces the DTWDistance<T> launch loop of approximateMedoidIndices
+ * (dba.hpp:67-158, kernel dtw.hpp:39-259). ref_w is the reference swath
+ * width (maxThreadsPerBlock, 1024); only 1024 is accepted.
+ * The distance is sqrtf(cost), divided by |Y| when exactly one of
+ * open_start / open_end is set (dtw.hpp:250-255), computed in float and
+ * converted to the value type.
+ */
+int odba_dtw_all_pairs(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                       int use_open_start, int use_open_end, int ref_w, void* dist_out);
+
+/* Device-resident sequence set: the sequences are copied to HBM once (in
+ * the given order, padded) and reused by the calls below. */
+int odba_seqset_create(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                       odba_seqset** out);
+int odba_seqset_destroy(odba_seqset* set);
+/* odba_dtw_all_pairs on a resident set. dist_out may be NULL (results stay
+ * on the device); kernel_ms (optional) receives the kernel time measured
+ * with HIP events on the ctx stream; flags_out (optional, n(n-1)/2 ints)
+ * receives bit 0 = open-end search abrogated (bits 8.. = index of the swath
+ * where it stopped), bit 1 = the reference would
+ * read an unwritten LDS slot for this |Y| == 1 pair (dtw.hpp:155). */
+int odba_dtw_all_pairs_resident(odba_ctx* ctx, odba_seqset* set, int use_open_start, int use_open_end,
+                                void* dist_out, int* flags_out, double* kernel_ms);
+
+/*
+ * One pair with its step matrix: moves_out (optional, ly*lx bytes,
+ * row-major) receives the reference step codes of every cell
+ * (dtw.hpp:10-16; 0 where the reference leaves a cell unwritten after an
+ * open-end abrogation), dist_out the pair distance as above. This is the
+ * path-storing DTWDistance launch sequence of DBAUpdate (dba.hpp:589-630).
+ */
+int odba_dtw_path(odba_ctx* ctx, int dtype, const void* y, uint64_t ly, const void* x, uint64_t lx,
+                  int use_open_start, int use_open_end, uint8_t* moves_out, void* dist_out);
+
+/*
+ * One DBA round (DBAUpdate<T>, dba.hpp:432-848) on the device: every
+ * sequence aligned to the centroid (flipped when open_end and the centroid
+ * is shorter, dba.hpp:497-501), traceback sums accumulated in the
+ * reference's order, new_centroid[t] = sum / count (dba.hpp:820-822),
+ * max_delta = max |old - new| (dba.hpp:830-836). counts_out is optional.
+ * path_policy: ODBA_PATH_FULL, ODBA_PATH_STRIPE or ODBA_PATH_AUTO with the
+ * free-memory figure ref_free_mem the reference's rule compares against
+ * (dba.hpp:523-548).
+ */
+int odba_dba_round(odba_ctx* ctx, int dtype, const void* centroid, uint64_t lc, const void* const* seqs,
+                   const uint64_t* lens, int n, int use_open_start, int use_open_end, int path_policy,
+                   uint64_t ref_free_mem, void* new_centroid, uint32_t* counts_out, double* max_delta);
+
+/*
+ * Z-normalise sequences in place (normalizeSequences, gpu_utils.hpp:131-170):
+ * population sigma, sigma == 0 treated as 1; mu / sigma (optional) receive
+ * each sequence's mean and standard deviation before that substitution.
+ */
+int odba_znorm(odba_ctx* ctx, int dtype, void* const* seqs, const uint64_t* lens, int n, double* mu, double* sigma);
+
+/*
+ * performDBA<T> for one cluster (cdist == 1, CLUSTER_AND_CONSENSUS;
+ * dba.hpp:863-1189) as one call: sanitize, stable length sort, optional
+ * z-norm, all-vs-all medoid search, DBA rounds until delta == 0 (or the
+ * open-mode flip-flop stop, or max_rounds), rescale by the medoid's mu and
+ * sigma. centroid_out must hold max(lens) elements; perm_out (optional, n
+ * ints) receives the sort permutation; dists_out (optional, n(n-1)/2
+ * values) the pair distances in sorted order.
+ */
+int odba_perform_dba(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                     int use_open_start, int use_open_end, int norm, int max_rounds, int path_policy,
+                     uint64_t ref_free_mem, void* centroid_out, uint64_t* centroid_len, int* medoid_sorted,
+                     int* perm_out, int* rounds_out, void* dists_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ODBA_H */

@@ -1,0 +1,741 @@
+// C-ABI of libodba_hip: host orchestration of the DTW / DBA kernels.
+// Declarations and contracts: include/odba/odba.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/odba/odba.h"
+#include "odba_dtw.hpp"
+
+using namespace odba;
+
+struct odba_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  int clock_khz = 0;
+  uint64_t mem = 0;
+  std::string name;
+  std::string err;
+  // grow-only device scratch
+  void* esc = nullptr;
+  size_t esc_bytes = 0;
+  void* small = nullptr;  // counters, flags (256 bytes)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct odba_seqset {
+  odba_ctx* ctx = nullptr;
+  int dtype = 0;
+  int n = 0;
+  void* data = nullptr;        // packed, padded
+  long long* offs = nullptr;   // device
+  int* lens = nullptr;         // device
+  std::vector<long long> h_offs;
+  std::vector<int> h_lens;
+  int maxlen = 0;
+  bool int_no_wrap = true;     // integer costs provably cannot wrap
+};
+
+namespace {
+
+size_t dtype_size(int dt) {
+  switch (dt) {
+    case ODBA_I32: case ODBA_U32: case ODBA_F32: return 4;
+    case ODBA_U64: case ODBA_F64: return 8;
+  }
+  return 0;
+}
+
+int fail(odba_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPC(ctx, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return fail(ctx, ODBA_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure_scratch(odba_ctx* c, size_t bytes) {
+  if (bytes <= c->esc_bytes) return ODBA_OK;
+  if (c->esc) hipFree(c->esc);
+  c->esc = nullptr;
+  c->esc_bytes = 0;
+  if (hipMalloc(&c->esc, bytes) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes of edge scratch", bytes);
+  c->esc_bytes = bytes;
+  return ODBA_OK;
+}
+
+template <typename T> bool is_float() { return std::is_floating_point<T>::value; }
+
+// Integer costs cannot wrap when every path cost stays below the type's max:
+// cost <= (ly + lx) * range^2.
+template <typename T>
+bool int_cannot_wrap(const T* const* seqs, const uint64_t* lens, int n) {
+  if (is_float<T>()) return true;
+  long double lo = 0, hi = 0, maxlen = 0;
+  bool first = true;
+  for (int i = 0; i < n; i++) {
+    for (uint64_t k = 0; k < lens[i]; k++) {
+      long double v = (long double)seqs[i][k];
+      if (first) { lo = hi = v; first = false; }
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    maxlen = std::max(maxlen, (long double)lens[i]);
+  }
+  long double r = hi - lo;
+  long double bound = 2.0L * maxlen * r * r + r * r;
+  return bound < (long double)std::numeric_limits<T>::max() / 2;
+}
+
+int grid_blocks(odba_ctx* c, long long work_items) {
+  long long want = (work_items + 3) / 4;
+  long long cap = (long long)c->cus * 8;  // 8 workgroups of 4 waves per CU
+  return (int)std::max(1LL, std::min(want, cap));
+}
+
+}  // namespace
+
+extern "C" {
+
+int odba_ctx_create(int device, odba_ctx** out) {
+  if (!out) return ODBA_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ODBA_E_NO_DEVICE;
+  if (device < 0 || device >= ndev) return ODBA_E_NO_DEVICE;
+  odba_ctx* c = new odba_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return ODBA_E_HIP; }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return ODBA_E_HIP; }
+  c->cus = p.multiProcessorCount;
+  c->clock_khz = p.clockRate;
+  c->mem = p.totalGlobalMem;
+  c->name = p.gcnArchName;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ODBA_E_HIP; }
+  if (hipMalloc(&c->small, 256) != hipSuccess) { hipStreamDestroy(c->stream); delete c; return ODBA_E_NOMEM; }
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  *out = c;
+  return ODBA_OK;
+}
+
+int odba_ctx_destroy(odba_ctx* c) {
+  if (!c) return ODBA_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  if (c->esc) hipFree(c->esc);
+  if (c->small) hipFree(c->small);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return ODBA_OK;
+}
+
+const char* odba_last_error(const odba_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int odba_device_info(odba_ctx* c, int* cus, int* clock_khz, uint64_t* mem, char* name, int name_len) {
+  if (!c) return ODBA_E_INVALID;
+  if (cus) *cus = c->cus;
+  if (clock_khz) *clock_khz = c->clock_khz;
+  if (mem) *mem = c->mem;
+  if (name && name_len > 0) {
+    std::strncpy(name, c->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return ODBA_OK;
+}
+
+int odba_seqset_create(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, odba_seqset** out) {
+  if (!c || !seqs || !lens || !out || n <= 0) return fail(c, ODBA_E_INVALID, "invalid seqset arguments");
+  const size_t es = dtype_size(dtype);
+  if (!es) return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype);
+  HIPC(c, hipSetDevice(c->device));
+  auto* s = new odba_seqset();
+  s->ctx = c;
+  s->dtype = dtype;
+  s->n = n;
+  s->h_offs.resize(n);
+  s->h_lens.resize(n);
+  long long total = 0;
+  for (int i = 0; i < n; i++) {
+    if (lens[i] == 0 || lens[i] > (uint64_t)INT32_MAX / 2) { delete s; return fail(c, ODBA_E_INVALID, "sequence %d has length %llu", i, (unsigned long long)lens[i]); }
+    s->h_offs[i] = total;
+    s->h_lens[i] = (int)lens[i];
+    s->maxlen = std::max(s->maxlen, (int)lens[i]);
+    total += padded_len((long long)lens[i]);
+  }
+  std::vector<unsigned char> host((size_t)total * es, 0);
+  for (int i = 0; i < n; i++) std::memcpy(&host[(size_t)s->h_offs[i] * es], seqs[i], (size_t)lens[i] * es);
+  switch (dtype) {
+    case ODBA_I32: s->int_no_wrap = int_cannot_wrap<int32_t>((const int32_t* const*)seqs, lens, n); break;
+    case ODBA_U32: s->int_no_wrap = int_cannot_wrap<uint32_t>((const uint32_t* const*)seqs, lens, n); break;
+    case ODBA_U64: s->int_no_wrap = int_cannot_wrap<uint64_t>((const uint64_t* const*)seqs, lens, n); break;
+    default: s->int_no_wrap = true;
+  }
+  if (hipMalloc(&s->data, host.size()) != hipSuccess || hipMalloc(&s->offs, n * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&s->lens, n * sizeof(int)) != hipSuccess) {
+    odba_seqset_destroy(s);
+    return fail(c, ODBA_E_NOMEM, "cannot allocate device sequence set (%zu bytes)", host.size());
+  }
+  HIPC(c, hipMemcpyAsync(s->data, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(s->offs, s->h_offs.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(s->lens, s->h_lens.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  *out = s;
+  return ODBA_OK;
+}
+
+int odba_seqset_destroy(odba_seqset* s) {
+  if (!s) return ODBA_OK;
+  if (s->data) hipFree(s->data);
+  if (s->offs) hipFree(s->offs);
+  if (s->lens) hipFree(s->lens);
+  delete s;
+  return ODBA_OK;
+}
+
+int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, void* dist_out, int* flags_out,
+                                double* kernel_ms) {
+  if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
+  HIPC(c, hipSetDevice(c->device));
+  const int n = s->n;
+  if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences");
+  const long long npairs = (long long)n * (n - 1) / 2;
+  const size_t es = dtype_size(s->dtype);
+  const int nblocks = grid_blocks(c, npairs);
+  const long long stride = s->maxlen + 256;
+  int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
+  if (rc) return rc;
+  void* dist = nullptr;
+  int* flags = nullptr;
+  if (hipMalloc(&dist, npairs * es) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
+  if (flags_out && hipMalloc((void**)&flags, npairs * sizeof(int)) != hipSuccess) {
+    hipFree(dist);
+    return fail(c, ODBA_E_NOMEM, "cannot allocate flags");
+  }
+  unsigned long long* counter = (unsigned long long*)c->small;
+  HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+  HIPC(c, hipEventRecord(c->ev0, c->stream));
+  hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs, s->lens, n,
+                                  npairs, counter, dist, c->esc, stride, flags);
+  if (e != hipSuccess) { hipFree(dist); if (flags) hipFree(flags); return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e)); }
+  HIPC(c, hipEventRecord(c->ev1, c->stream));
+  if (dist_out) HIPC(c, hipMemcpyAsync(dist_out, dist, npairs * es, hipMemcpyDeviceToHost, c->stream));
+  if (flags_out) HIPC(c, hipMemcpyAsync(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (kernel_ms) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    *kernel_ms = ms;
+  }
+  hipFree(dist);
+  if (flags) hipFree(flags);
+  return ODBA_OK;
+}
+
+int odba_dtw_all_pairs(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int os, int oe,
+                       int ref_w, void* dist_out) {
+  if (!c) return ODBA_E_INVALID;
+  if (ref_w != ODBA_W) return fail(c, ODBA_E_UNSUPPORTED, "swath width %d: only %d is supported", ref_w, ODBA_W);
+  if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences, got %d", n);
+  odba_seqset* s = nullptr;
+  int rc = odba_seqset_create(c, dtype, seqs, lens, n, &s);
+  if (rc) return rc;
+  rc = odba_dtw_all_pairs_resident(c, s, os, oe, dist_out, nullptr, nullptr);
+  odba_seqset_destroy(s);
+  return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Path-mode machinery shared by odba_dtw_path and the DBA round.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PathPlan {
+  std::vector<unsigned char> jobs;  // PathJob<T> records
+  std::vector<TbJob> tb;
+  long long mv_words = 0, top_words = 0;
+};
+
+template <typename T> struct PJ {
+  const T* y;
+  const T* x;
+  int ly, lx;
+  long long mv_off, top_off;
+};
+
+// Geometry of one path job; returns words needed.
+void plan_job(int ly, int lx, int os, int oe, long long& mv_words, long long& top_words, long long& mv_off,
+              long long& top_off) {
+  const int K = (lx + ODBA_W - 1) / ODBA_W;
+  mv_off = mv_words;
+  top_off = top_words;
+  mv_words += (long long)K * swath_words(ly, os, oe);
+  top_words += (long long)K * 64;
+}
+
+int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
+                  uint32_t* mv, uint32_t* mtop, int* abr) {
+  const size_t es = dtype_size(dtype);
+  const int nblocks = grid_blocks(c, njobs);
+  const long long stride = maxly + 256;
+  int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
+  if (rc) return rc;
+  unsigned int* counter = (unsigned int*)((char*)c->small + 64);
+  HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned int), c->stream));
+  hipError_t e = launch_path_jobs(dtype, minfirst, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr,
+                                  c->esc, stride);
+  if (e != hipSuccess) return fail(c, ODBA_E_HIP, "path launch: %s", hipGetErrorString(e));
+  return ODBA_OK;
+}
+
+template <typename T>
+int dtw_path_t(odba_ctx* c, const T* y, int ly, const T* x, int lx, int os, int oe, uint8_t* moves_out, T* dist_out) {
+  // upload both sequences padded
+  const long long py = padded_len(ly), px = padded_len(lx);
+  std::vector<T> hb((size_t)(py + px), T(0));
+  std::memcpy(hb.data(), y, ly * sizeof(T));
+  std::memcpy(hb.data() + py, x, lx * sizeof(T));
+  T* d = nullptr;
+  HIPC(c, hipMalloc(&d, hb.size() * sizeof(T)));
+  HIPC(c, hipMemcpyAsync(d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  long long mvw = 0, tw = 0, mo, to;
+  plan_job(ly, lx, os, oe, mvw, tw, mo, to);
+  PJ<T> job{d, d + py, ly, lx, mo, to};
+  void* jd = nullptr;
+  uint32_t *mv = nullptr, *mtop = nullptr;
+  int* abr = nullptr;
+  HIPC(c, hipMalloc(&jd, sizeof(job)));
+  HIPC(c, hipMalloc((void**)&mv, std::max(1LL, mvw) * 4));
+  HIPC(c, hipMalloc((void**)&mtop, std::max(1LL, tw) * 4));
+  HIPC(c, hipMalloc((void**)&abr, sizeof(int)));
+  HIPC(c, hipMemcpyAsync(jd, &job, sizeof(job), hipMemcpyHostToDevice, c->stream));
+  std::vector<const T*> sp{y, x};
+  std::vector<uint64_t> sl{(uint64_t)ly, (uint64_t)lx};
+  bool minfirst = int_cannot_wrap<T>(sp.data(), sl.data(), 2);
+  int rc = run_path_jobs(c, std::is_same<T, float>::value ? ODBA_F32 : std::is_same<T, double>::value ? ODBA_F64
+                              : std::is_same<T, int32_t>::value ? ODBA_I32 : std::is_same<T, uint32_t>::value ? ODBA_U32 : ODBA_U64,
+                         minfirst, os, oe, jd, 1, ly, mv, mtop, abr);
+  if (rc) return rc;
+  int ab = -1;
+  HIPC(c, hipMemcpyAsync(&ab, abr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (moves_out) {
+    uint8_t* md = nullptr;
+    HIPC(c, hipMalloc((void**)&md, (size_t)ly * lx));
+    HIPC(c, launch_decode_moves(c->stream, mv, mtop, ab, ly, lx, os, oe, md));
+    HIPC(c, hipMemcpyAsync(moves_out, md, (size_t)ly * lx, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    hipFree(md);
+  }
+  hipFree(jd); hipFree(mv); hipFree(mtop); hipFree(abr); hipFree(d);
+  if (dist_out) {
+    // the distance is computed by the all-pairs machinery for exactness of the
+    // abrogation path
+    const void* ptrs[2] = {y, x};
+    uint64_t lens[2] = {(uint64_t)ly, (uint64_t)lx};
+    return odba_dtw_all_pairs(c, std::is_same<T, float>::value ? ODBA_F32 : std::is_same<T, double>::value ? ODBA_F64
+                                 : std::is_same<T, int32_t>::value ? ODBA_I32 : std::is_same<T, uint32_t>::value ? ODBA_U32 : ODBA_U64,
+                              ptrs, lens, 2, os, oe, ODBA_W, dist_out);
+  }
+  return ODBA_OK;
+}
+
+template <typename T> int dtype_of() {
+  return std::is_same<T, float>::value ? ODBA_F32 : std::is_same<T, double>::value ? ODBA_F64
+         : std::is_same<T, int32_t>::value ? ODBA_I32 : std::is_same<T, uint32_t>::value ? ODBA_U32 : ODBA_U64;
+}
+
+// Device-resident state of a DBA cluster: member sequences (cluster order),
+// the centroid, and per-round work buffers.
+template <typename T> struct DbaState {
+  odba_ctx* c = nullptr;
+  int n = 0, os = 0, oe = 0;
+  bool minfirst = true;
+  T* seqdata = nullptr;          // packed member sequences
+  long long* offs = nullptr;     // device offsets (cluster order)
+  std::vector<long long> h_offs;
+  std::vector<int> h_lens;
+  int maxlen = 0;
+  int lc = 0;
+  T* cen = nullptr;              // padded centroid
+  T* newc = nullptr;
+  T* acc = nullptr;
+  uint32_t* counts = nullptr;
+  int2* runs = nullptr;
+  size_t runs_cap = 0;
+  uint32_t* mv = nullptr;
+  size_t mv_cap = 0;
+  uint32_t* mtop = nullptr;
+  size_t mtop_cap = 0;
+  int* abr = nullptr;
+  void* jobs = nullptr;
+  TbJob* tbj = nullptr;
+  int* err = nullptr;
+  unsigned long long* dbits = nullptr;
+  long long budget_words = 0;    // per-batch move-word budget
+
+  ~DbaState() {
+    for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
+                    (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits})
+      if (p) hipFree(p);
+  }
+
+  int init(odba_ctx* ctx, const T* const* seqs, const uint64_t* lens, int nn, int os_, int oe_) {
+    c = ctx; n = nn; os = os_; oe = oe_;
+    minfirst = int_cannot_wrap<T>(seqs, lens, nn);
+    h_offs.resize(n);
+    h_lens.resize(n);
+    long long total = 0;
+    for (int i = 0; i < n; i++) {
+      h_offs[i] = total;
+      h_lens[i] = (int)lens[i];
+      maxlen = std::max(maxlen, (int)lens[i]);
+      total += padded_len((long long)lens[i]);
+    }
+    std::vector<T> hb((size_t)total, T(0));
+    for (int i = 0; i < n; i++) std::memcpy(&hb[h_offs[i]], seqs[i], lens[i] * sizeof(T));
+    HIPC(c, hipMalloc((void**)&seqdata, hb.size() * sizeof(T)));
+    HIPC(c, hipMalloc((void**)&offs, n * sizeof(long long)));
+    HIPC(c, hipMemcpyAsync(seqdata, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(offs, h_offs.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMalloc((void**)&abr, n * sizeof(int)));
+    HIPC(c, hipMalloc(&jobs, n * sizeof(PJ<T>)));
+    HIPC(c, hipMalloc((void**)&tbj, n * sizeof(TbJob)));
+    HIPC(c, hipMalloc((void**)&err, sizeof(int)));
+    HIPC(c, hipMalloc((void**)&dbits, sizeof(unsigned long long)));
+    // move-word budget: a quarter of device memory, at least one job
+    budget_words = (long long)(c->mem / 4 / 4);
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return ODBA_OK;
+  }
+
+  int set_centroid(const T* h, int len) {
+    if (len != lc) {
+      if (cen) hipFree(cen);
+      if (newc) hipFree(newc);
+      if (acc) hipFree(acc);
+      if (counts) hipFree(counts);
+      cen = newc = acc = nullptr;
+      counts = nullptr;
+      lc = len;
+      const long long pl = padded_len(lc);
+      HIPC(c, hipMalloc((void**)&cen, pl * sizeof(T)));
+      HIPC(c, hipMalloc((void**)&newc, pl * sizeof(T)));
+      HIPC(c, hipMalloc((void**)&acc, lc * sizeof(T)));
+      HIPC(c, hipMalloc((void**)&counts, lc * sizeof(uint32_t)));
+      HIPC(c, hipMemsetAsync(cen, 0, pl * sizeof(T), c->stream));
+      HIPC(c, hipMemsetAsync(newc, 0, pl * sizeof(T), c->stream));
+    }
+    if (h) HIPC(c, hipMemcpyAsync(cen, h, lc * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return ODBA_OK;
+  }
+
+  int set_centroid_device(const T* d, int len) {
+    int rc = set_centroid(nullptr, len);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(cen, d, lc * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    return ODBA_OK;
+  }
+
+  template <typename P> static int grow(odba_ctx* c, P*& p, size_t& cap, size_t need) {
+    if (need <= cap) return ODBA_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc((void**)&p, need * sizeof(P)) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes", need * sizeof(P));
+    cap = need;
+    return ODBA_OK;
+  }
+
+  // One DBA round: new centroid in newc, counts, delta. Swaps cen/newc.
+  int round(double* delta) {
+    const int W = ODBA_W;
+    (void)W;
+    HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    HIPC(c, hipMemsetAsync(dbits, 0, sizeof(unsigned long long), c->stream));
+    int rc = grow(c, runs, runs_cap, (size_t)n * lc);
+    if (rc) return rc;
+    // batches of sequences whose move words fit the budget (sequence order kept)
+    int b0 = 0;
+    bool first = true;
+    while (b0 < n) {
+      std::vector<PJ<T>> pj;
+      std::vector<TbJob> tb;
+      long long mvw = 0, tw = 0;
+      int maxly = 1;
+      int b1 = b0;
+      while (b1 < n) {
+        const int ls = h_lens[b1];
+        const int flip = (oe && lc < ls) ? 1 : 0;
+        const int ly = flip ? lc : ls, lx = flip ? ls : lc;
+        long long need = (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        if (b1 > b0 && mvw + need > budget_words) break;
+        long long mo, to;
+        plan_job(ly, lx, os, oe, mvw, tw, mo, to);
+        const T* sq = seqdata + h_offs[b1];
+        PJ<T> j{flip ? cen : sq, flip ? sq : cen, ly, lx, mo, to};
+        pj.push_back(j);
+        TbJob t;
+        t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
+        tb.push_back(t);
+        maxly = std::max(maxly, ly);
+        b1++;
+      }
+      const int nb = b1 - b0;
+      rc = grow(c, mv, mv_cap, (size_t)std::max(1LL, mvw));
+      if (rc) return rc;
+      rc = grow(c, mtop, mtop_cap, (size_t)std::max(1LL, tw));
+      if (rc) return rc;
+      HIPC(c, hipMemcpyAsync(jobs, pj.data(), nb * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(tbj, tb.data(), nb * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+      rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nb, maxly, mv, mtop, abr);
+      if (rc) return rc;
+      HIPC(c, launch_traceback(c->stream, tbj, nb, mv, mtop, abr, runs, lc, os, oe, err));
+      HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, nb, lc, acc, counts, first ? 1 : 0));
+      // the host vectors must outlive the async copies
+      HIPC(c, hipStreamSynchronize(c->stream));
+      first = false;
+      b0 = b1;
+    }
+    HIPC(c, launch_mean_delta(dtype_of<T>(), c->stream, acc, counts, cen, newc, lc, dbits, err));
+    int herr = 0;
+    unsigned long long bits = 0;
+    HIPC(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(&bits, dbits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+    if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
+    double d;
+    std::memcpy(&d, &bits, sizeof(d));
+    if (delta) *delta = d;
+    std::swap(cen, newc);
+    return ODBA_OK;
+  }
+};
+
+template <typename T>
+int dba_round_t(odba_ctx* c, const T* C, int lc, const T* const* seqs, const uint64_t* lens, int n, int os, int oe,
+                T* newc, uint32_t* counts_out, double* delta) {
+  DbaState<T> st;
+  int rc = st.init(c, seqs, lens, n, os, oe);
+  if (rc) return rc;
+  rc = st.set_centroid(C, lc);
+  if (rc) return rc;
+  rc = st.round(delta);
+  if (rc) return rc;
+  HIPC(c, hipMemcpyAsync(newc, st.cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  if (counts_out) HIPC(c, hipMemcpyAsync(counts_out, st.counts, lc * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ODBA_OK;
+}
+
+template <typename T> bool sanitize_drop_last(T v) { return (double)v >= std::sqrt((double)std::numeric_limits<T>::max()); }
+template <> bool sanitize_drop_last<float>(float v) { return v >= std::sqrt(std::numeric_limits<float>::max()); }
+
+template <typename T> T host_d2t(double d) {
+  if (std::is_floating_point<T>::value) return (T)d;
+  if (d != d) return T(0);
+  if (d >= (double)std::numeric_limits<T>::max()) return std::numeric_limits<T>::max();
+  if (d <= (double)std::numeric_limits<T>::lowest()) return std::numeric_limits<T>::lowest();
+  return (T)d;
+}
+
+template <typename T>
+int znorm_t(odba_ctx* c, T* const* seqs, const uint64_t* lens, int n, double* mu, double* sigma) {
+  std::vector<long long> offs(n);
+  std::vector<int> L(n);
+  long long total = 0;
+  for (int i = 0; i < n; i++) { offs[i] = total; L[i] = (int)lens[i]; total += padded_len((long long)lens[i]); }
+  std::vector<T> hb((size_t)total, T(0));
+  for (int i = 0; i < n; i++) std::memcpy(&hb[offs[i]], seqs[i], lens[i] * sizeof(T));
+  T* d = nullptr;
+  long long* doffs = nullptr;
+  int* dl = nullptr;
+  double *dmu = nullptr, *dsg = nullptr;
+  HIPC(c, hipMalloc((void**)&d, hb.size() * sizeof(T)));
+  HIPC(c, hipMalloc((void**)&doffs, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&dl, n * sizeof(int)));
+  HIPC(c, hipMalloc((void**)&dmu, n * sizeof(double)));
+  HIPC(c, hipMalloc((void**)&dsg, n * sizeof(double)));
+  HIPC(c, hipMemcpyAsync(d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(doffs, offs.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dl, L.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, launch_znorm(dtype_of<T>(), c->stream, d, doffs, dl, n, dmu, dsg));
+  HIPC(c, hipMemcpyAsync(hb.data(), d, hb.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> hmu(n), hsg(n);
+  HIPC(c, hipMemcpyAsync(hmu.data(), dmu, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(hsg.data(), dsg, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) std::memcpy(seqs[i], &hb[offs[i]], lens[i] * sizeof(T));
+  if (mu) std::memcpy(mu, hmu.data(), n * sizeof(double));
+  if (sigma) std::memcpy(sigma, hsg.data(), n * sizeof(double));
+  hipFree(d); hipFree(doffs); hipFree(dl); hipFree(dmu); hipFree(dsg);
+  return ODBA_OK;
+}
+
+// Sum of squared distances in T in the host loop order, first strict
+// minimum; two members -> the longer, the second on a tie
+// (dba.hpp:186-207, 297-351).
+template <typename T> int medoid_single(const T* d, const std::vector<uint64_t>& l, int n) {
+  std::vector<T> sos(n, T(0));
+  size_t idx = 0;
+  for (int r = 0; r + 1 < n; r++) {
+    for (int cc = r + 1; cc < n; cc++) {
+      T v = d[idx + cc - r - 1];
+      v = (T)(v * v);
+      sos[r] = (T)(sos[r] + v);
+      sos[cc] = (T)(sos[cc] + v);
+    }
+    idx += n - r - 1;
+  }
+  if (n > 2) {
+    int med = -1;
+    T low = std::numeric_limits<T>::max();
+    for (int i = 0; i < n; i++)
+      if (sos[i] < low) { med = i; low = sos[i]; }
+    return med;
+  }
+  if (n == 2) return l[0] > l[1] ? 0 : 1;
+  return 0;
+}
+
+template <typename T>
+int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
+                  int max_rounds, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
+                  T* dists_out) {
+  if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "at least two sequences are needed, got %d", n);
+  std::vector<uint64_t> lens(in_lens, in_lens + n);
+  for (int i = 0; i < n; i++)
+    if (lens[i] && sanitize_drop_last<T>(in[i][lens[i] - 1])) lens[i]--;  // dba.hpp:868-872
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return lens[a] < lens[b]; });  // dba.hpp:876-885
+  std::vector<std::vector<T>> s(n);
+  std::vector<uint64_t> l(n);
+  for (int i = 0; i < n; i++) { l[i] = lens[perm[i]]; s[i].assign(in[perm[i]], in[perm[i]] + l[i]); }
+  std::vector<T*> sp(n);
+  for (int i = 0; i < n; i++) sp[i] = s[i].data();
+  std::vector<double> mu(n, 0.0), sg(n, 1.0);
+  int rc;
+  if (norm) {
+    rc = znorm_t<T>(c, sp.data(), l.data(), n, mu.data(), sg.data());
+    if (rc) return rc;
+  }
+  std::vector<T> d((size_t)n * (n - 1) / 2);
+  rc = odba_dtw_all_pairs(c, dtype_of<T>(), (const void* const*)sp.data(), l.data(), n, os, oe, ODBA_W, d.data());
+  if (rc) return rc;
+  if (dists_out) std::memcpy(dists_out, d.data(), d.size() * sizeof(T));
+  const int med = medoid_single<T>(d.data(), l, n);
+  if (med < 0) return fail(c, ODBA_E_MEDOID, "no medoid found (all sums of squares at the type maximum)");
+  const int lc = (int)l[med];
+  DbaState<T> st;
+  rc = st.init(c, (const T* const*)sp.data(), l.data(), n, os, oe);
+  if (rc) return rc;
+  rc = st.set_centroid(s[med].data(), lc);
+  if (rc) return rc;
+  std::vector<T> nb(lc), prev(lc, T(0)), prev2(lc, T(0));
+  int rounds = 0;
+  for (int r = 0; r < max_rounds; r++) {  // dba.hpp:1084-1110
+    double delta = 0;
+    rc = st.round(&delta);
+    if (rc) return rc;
+    rounds = r + 1;
+    HIPC(c, hipMemcpyAsync(nb.data(), st.cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (delta == 0) break;
+    if (os || oe) {
+      if (r >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * lc)) break;
+      if (r > 1) prev2 = prev;
+      if (r > 0) prev = nb;
+    }
+  }
+  if (norm)
+    for (int i = 0; i < lc; i++) nb[i] = host_d2t<T>(mu[med] + nb[i] * sg[med]);  // dba.hpp:1117-1125
+  std::memcpy(cen_out, nb.data(), lc * sizeof(T));
+  if (cen_len) *cen_len = lc;
+  if (med_out) *med_out = med;
+  if (perm_out) std::memcpy(perm_out, perm.data(), n * sizeof(int));
+  if (rounds_out) *rounds_out = rounds;
+  return ODBA_OK;
+}
+
+}  // namespace
+
+#define ODBA_DISPATCH(dtype, CALL)                        \
+  switch (dtype) {                                        \
+    case ODBA_I32: { typedef int32_t T; return CALL; }    \
+    case ODBA_U32: { typedef uint32_t T; return CALL; }   \
+    case ODBA_U64: { typedef uint64_t T; return CALL; }   \
+    case ODBA_F32: { typedef float T; return CALL; }      \
+    case ODBA_F64: { typedef double T; return CALL; }     \
+    default: return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype); \
+  }
+
+extern "C" {
+
+int odba_dtw_path(odba_ctx* c, int dtype, const void* y, uint64_t ly, const void* x, uint64_t lx, int os, int oe,
+                  uint8_t* moves_out, void* dist_out) {
+  if (!c || !y || !x || ly == 0 || lx == 0) return fail(c, ODBA_E_INVALID, "invalid path arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(dtype, dtw_path_t<T>(c, (const T*)y, (int)ly, (const T*)x, (int)lx, os, oe, moves_out, (T*)dist_out))
+}
+
+int odba_dba_round(odba_ctx* c, int dtype, const void* centroid, uint64_t lc, const void* const* seqs,
+                   const uint64_t* lens, int n, int os, int oe, int path_policy, uint64_t ref_free_mem,
+                   void* new_centroid, uint32_t* counts_out, double* max_delta) {
+  if (!c || !centroid || !seqs || !lens || n <= 0 || lc == 0 || !new_centroid) return fail(c, ODBA_E_INVALID, "invalid DBA round arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  if (path_policy == ODBA_PATH_STRIPE) return fail(c, ODBA_E_UNSUPPORTED, "stripe mode is not available in this build");
+  if (path_policy == ODBA_PATH_AUTO) {
+    for (int i = 0; i < n; i++) {
+      const bool flip = oe && lc < lens[i];
+      const double need = (double)((flip ? lc : lens[i]) * dtype_size(dtype)) + (double)lens[i] * (double)lc * 1.05;
+      if ((double)ref_free_mem < need) return fail(c, ODBA_E_UNSUPPORTED, "the reference would use stripe mode for sequence %d; stripe mode is not available in this build", i);
+    }
+  }
+  ODBA_DISPATCH(dtype, dba_round_t<T>(c, (const T*)centroid, (int)lc, (const T* const*)seqs, lens, n, os, oe,
+                                      (T*)new_centroid, counts_out, max_delta))
+}
+
+int odba_znorm(odba_ctx* c, int dtype, void* const* seqs, const uint64_t* lens, int n, double* mu, double* sigma) {
+  if (!c || !seqs || !lens || n <= 0) return fail(c, ODBA_E_INVALID, "invalid z-norm arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(dtype, znorm_t<T>(c, (T* const*)seqs, lens, n, mu, sigma))
+}
+
+int odba_perform_dba(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int os, int oe,
+                     int norm, int max_rounds, int path_policy, uint64_t ref_free_mem, void* centroid_out,
+                     uint64_t* centroid_len, int* medoid_sorted, int* perm_out, int* rounds_out, void* dists_out) {
+  if (!c || !seqs || !lens || !centroid_out) return fail(c, ODBA_E_INVALID, "invalid performDBA arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  if (path_policy == ODBA_PATH_STRIPE) return fail(c, ODBA_E_UNSUPPORTED, "stripe mode is not available in this build");
+  (void)ref_free_mem;
+  ODBA_DISPATCH(dtype, perform_dba_t<T>(c, (const T* const*)seqs, lens, n, os, oe, norm, max_rounds, (T*)centroid_out,
+                                        centroid_len, medoid_sorted, perm_out, rounds_out, (T*)dists_out))
+}
+
+}  // extern "C"

@@ -1,0 +1,338 @@
+// DBA update kernels (traceback, ordered accumulation, mean/delta) and the
+// z-normalisation kernel for gfx950.
+//
+// The reference accumulates every sequence's traceback into the centroid
+// sums one sequence after the other, each in traceback order
+// (updateCentroid, dba.hpp:374-423, serialised per sequence at
+// dba.hpp:641-643). Along a monotone path the cells that feed one centroid
+// position form one contiguous run of sequence indices, visited in
+// descending order. The traceback kernel therefore emits one (lo, hi) run
+// per (sequence, centroid position); the accumulation kernel then walks, for
+// each centroid position in parallel, the sequences in cluster order and
+// each run from hi down to lo. The float additions happen in exactly the
+// reference's order, so the sums are bit-identical, with no atomics.
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include "odba_dtw.hpp"
+
+namespace odba {
+
+struct Geo {
+  int ly, lx, os, oe, ab, K, lastC, r0;
+  long long S64;  // words per swath
+};
+
+__device__ __forceinline__ Geo make_geo(int ly, int lx, int os, int oe, int ab) {
+  Geo g;
+  g.ly = ly; g.lx = lx; g.os = os; g.oe = oe; g.ab = ab;
+  g.K = (lx + ODBA_W - 1) / ODBA_W;
+  int wl = lx - (g.K - 1) * ODBA_W;
+  g.lastC = wl <= 64 ? 1 : wl <= 128 ? 2 : wl <= 256 ? 4 : wl <= 512 ? 8 : wl <= 768 ? 12 : 16;
+  g.r0 = os ? 1 : 0;
+  int nr = ((oe && ly >= 2) ? ly - 2 : ly - 1) - g.r0 + 1;
+  g.S64 = nr > 0 ? (long long)(nr + 63) * 64 : 0;
+  return g;
+}
+
+// Reference step code of cell (i, j) (dtw.hpp:10-16). Cells the reference
+// never writes (below the top row of an abrogated swath) come back as 0.
+__device__ __forceinline__ uint8_t step_code(const Geo& g, const uint32_t* __restrict__ mv,
+                                             const uint32_t* __restrict__ mtop, int i, int j, long long& caddr,
+                                             uint32_t& cword) {
+  if (i == 0 && j == 0) return g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
+  if (g.ab >= 0 && j >= g.ab * ODBA_W) return (i == g.ly - 1) ? ODBA_OPEN_RIGHT : 0;
+  if (i == 0) return g.os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
+  const int k = j / ODBA_W, t = j - k * ODBA_W;
+  const int C = (k == g.K - 1) ? g.lastC : 16;
+  const int lane = t / C, q = t - lane * C;
+  if (g.oe && g.ly >= 2 && i == g.ly - 1) {
+    uint32_t w = mtop[(long long)k * 64 + lane];
+    uint32_t c = (w >> (2 * q)) & 3u;
+    return c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
+  }
+  const long long s = (long long)(i - g.r0) + lane;
+  const long long addr = (long long)k * g.S64 + s * 64 + lane;
+  uint32_t w;
+  if (addr == caddr) w = cword;
+  else { w = mv[addr]; caddr = addr; cword = w; }
+  uint32_t c = (w >> (2 * q)) & 3u;
+  return c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
+}
+
+// One thread per aligned sequence. runs[seq][c] = (lo, hi) of the sequence
+// indices counted into centroid position c (empty when lo > hi).
+__global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uint32_t* __restrict__ mv,
+                            const uint32_t* __restrict__ mtop, const int* __restrict__ abr, int2* __restrict__ runs,
+                            int lc, int os, int oe, int* __restrict__ err) {
+  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jid >= njobs) return;
+  const TbJob J = jobs[jid];
+  const Geo g = make_geo(J.ly, J.lx, os, oe, abr[jid]);
+  const uint32_t* M = mv + J.mv_off;
+  const uint32_t* TP = mtop + J.top_off;
+  int2* R = runs + (long long)J.seq * lc;
+  int i = J.ly - 1, j = J.lx - 1;
+  int curc = J.flip ? i : j;
+  int lo = 1, hi = 0;
+  long long caddr = -1;
+  uint32_t cword = 0;
+  for (;;) {
+    const uint8_t code = step_code(g, M, TP, i, j, caddr, cword);
+    const int c = J.flip ? i : j;
+    const int sidx = J.flip ? j : i;
+    if (c != curc) {
+      R[curc] = make_int2(lo, hi);
+      curc = c;
+      lo = 1;
+      hi = 0;
+    }
+    const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
+    const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);  // dba.hpp:395-415
+    if (counted) {
+      if (hi < lo) { hi = sidx; lo = sidx; }
+      else lo = sidx;
+    }
+    if (corner) break;
+    if (code == ODBA_DIAGONAL) { i--; j--; }
+    else if (code == ODBA_UP) { i--; }
+    else if (code == ODBA_RIGHT || code == ODBA_OPEN_RIGHT) { j--; }
+    else { atomicOr(err, 1); break; }
+    if (i < 0 || j < 0) { atomicOr(err, 1); break; }
+  }
+  R[curc] = make_int2(lo, hi);
+}
+
+template <typename T>
+__global__ void k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
+                             const int2* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
+                             uint32_t* __restrict__ counts, int first) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= lc) return;
+  T a = first ? T(0) : acc[c];
+  uint32_t n = first ? 0u : counts[c];
+  for (int s = 0; s < nseq; s++) {
+    const int2 r = runs[(long long)s * lc + c];
+    if (r.y >= r.x) {
+      const T* p = data + offs[s];
+      for (int k = r.y; k >= r.x; k--) a = (T)(a + p[k]);
+      n += (uint32_t)(r.y - r.x + 1);
+    }
+  }
+  acc[c] = a;
+  counts[c] = n;
+}
+
+// mean = sum / count with the host's T / unsigned arithmetic (dba.hpp:820-822)
+template <typename T> __device__ __forceinline__ T div_count(T s, uint32_t c) { return (T)(s / (T)c); }
+template <> __device__ __forceinline__ float div_count<float>(float s, uint32_t c) { return __fdiv_rn(s, (float)c); }
+template <> __device__ __forceinline__ double div_count<double>(double s, uint32_t c) { return s / (double)c; }
+template <> __device__ __forceinline__ int32_t div_count<int32_t>(int32_t s, uint32_t c) { return (int32_t)((uint32_t)s / c); }
+template <> __device__ __forceinline__ uint64_t div_count<uint64_t>(uint64_t s, uint32_t c) { return s / (uint64_t)c; }
+
+template <typename T> struct IsFloat { static constexpr bool v = false; };
+template <> struct IsFloat<float> { static constexpr bool v = true; };
+template <> struct IsFloat<double> { static constexpr bool v = true; };
+
+template <typename T>
+__global__ void k_mean_delta(const T* __restrict__ acc, const uint32_t* __restrict__ counts,
+                             const T* __restrict__ oldc, T* __restrict__ newc, int lc,
+                             unsigned long long* __restrict__ delta_bits, int* __restrict__ err) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= lc) return;
+  T m;
+  if (!IsFloat<T>::v && counts[c] == 0) {
+    atomicOr(err, 2);
+    m = T(0);
+  } else {
+    m = div_count<T>(acc[c], counts[c]);
+  }
+  newc[c] = m;
+  const double d = fabs((double)(T)(oldc[c] - m));  // dba.hpp:830-836
+  if (d == d) atomicMax(delta_bits, (unsigned long long)__double_as_longlong(d));
+}
+
+// ---------------------------------------------------------------------------
+// Z-normalisation (gpu_utils.hpp:34-129), one 1024-thread workgroup per
+// sequence: chunk sums with the 32-lane shuffle-down tree and the same tree
+// over the 32 group sums, chunks added in ascending order (deterministic;
+// the reference adds them with atomics), then the double sum of squares the
+// same way, then x <- (T)(0 + ((x - mu)/sigma) * 1).
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T d2t(double d) { return (T)d; }
+template <> __device__ __forceinline__ int32_t d2t<int32_t>(double d) {
+  if (d != d) return 0;
+  if (d >= 2147483647.0) return INT_MAX;
+  if (d <= -2147483648.0) return INT_MIN;
+  return (int32_t)d;
+}
+template <> __device__ __forceinline__ uint32_t d2t<uint32_t>(double d) {
+  if (d != d || d <= 0.0) return 0;
+  if (d >= 4294967295.0) return UINT_MAX;
+  return (uint32_t)d;
+}
+template <> __device__ __forceinline__ uint64_t d2t<uint64_t>(double d) {
+  if (d != d || d <= 0.0) return 0;
+  if (d >= 18446744073709551615.0) return ULLONG_MAX;
+  return (uint64_t)d;
+}
+
+template <typename A> __device__ __forceinline__ A tree32(A v) {
+  for (int off = 16; off > 0; off >>= 1) v = (A)(v + __shfl_down(v, off, 32));
+  return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_znorm(T* __restrict__ data, const long long* __restrict__ offs,
+                                                const int* __restrict__ lens, int n, double* __restrict__ mu,
+                                                double* __restrict__ sigma) {
+  __shared__ T wsum[32];
+  __shared__ double wsos[32];
+  __shared__ T total_s;
+  __shared__ double sos_s;
+  const int seq = blockIdx.x;
+  const int L = lens[seq];
+  T* x = data + offs[seq];
+  const int tid = threadIdx.x;
+  const int nch = (L + 1023) / 1024;
+  T total = T(0);
+  for (int ch = 0; ch < nch; ch++) {
+    const int p = ch * 1024 + tid;
+    T v = p < L ? x[p] : T(0);
+    v = tree32(v);
+    if ((tid & 31) == 0) wsum[tid >> 5] = v;
+    __syncthreads();
+    if (tid < 32) {
+      T w = wsum[tid];
+      w = tree32(w);
+      if (tid == 0) total = (T)(total + w);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) total_s = total;
+  __syncthreads();
+  total = total_s;
+  const double mean = (double)total / (double)L;
+  double sos = 0.0;
+  for (int ch = 0; ch < nch; ch++) {
+    const int p = ch * 1024 + tid;
+    double v = 0.0;
+    if (p < L) {
+      v = (double)x[p] - ((double)total / (double)L);
+      v *= v;
+    }
+    v = tree32(v);
+    if ((tid & 31) == 0) wsos[tid >> 5] = v;
+    __syncthreads();
+    if (tid < 32) {
+      double w = wsos[tid];
+      w = tree32(w);
+      if (tid == 0) sos += w;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) sos_s = sos;
+  __syncthreads();
+  sos = sos_s;
+  double sd = sqrt(sos / (double)L);
+  if (tid == 0) {
+    if (mu) mu[seq] = mean;
+    if (sigma) sigma[seq] = sd;
+  }
+  if (sd == 0) sd = 1;
+  for (int p = tid; p < L; p += 1024) x[p] = d2t<T>(0.0 + (((double)x[p] - mean) / sd) * 1.0);
+}
+
+// Reference step-code matrix of one pair (ly x lx, row-major).
+__global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* __restrict__ mtop, int ab, int ly,
+                               int lx, int os, int oe, uint8_t* __restrict__ out) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)ly * lx) return;
+  const int i = (int)(idx / lx), j = (int)(idx % lx);
+  const Geo g = make_geo(ly, lx, os, oe, ab);
+  long long caddr = -1;
+  uint32_t cword = 0;
+  out[idx] = step_code(g, mv, mtop, i, j, caddr, cword);
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
+                            const int* abr, int2* runs, int lc, int os, int oe, int* err) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t acc_t(hipStream_t st, const void* data, const long long* offs, const int2* runs, int nseq, int lc,
+                        void* acc, uint32_t* counts, int first) {
+  hipLaunchKernelGGL(k_accumulate<T>, dim3((lc + 255) / 256), dim3(256), 0, st, (const T*)data, offs, runs, nseq, lc,
+                     (T*)acc, counts, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int2* runs,
+                             int nseq, int lc, void* acc, uint32_t* counts, int first) {
+  switch (dtype) {
+    case ODBA_F32: return acc_t<float>(st, data, offs, runs, nseq, lc, acc, counts, first);
+    case ODBA_F64: return acc_t<double>(st, data, offs, runs, nseq, lc, acc, counts, first);
+    case ODBA_I32: return acc_t<int32_t>(st, data, offs, runs, nseq, lc, acc, counts, first);
+    case ODBA_U32: return acc_t<uint32_t>(st, data, offs, runs, nseq, lc, acc, counts, first);
+    case ODBA_U64: return acc_t<uint64_t>(st, data, offs, runs, nseq, lc, acc, counts, first);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t md_t(hipStream_t st, const void* acc, const uint32_t* counts, const void* oldc, void* newc, int lc,
+                       unsigned long long* delta_bits, int* err) {
+  hipLaunchKernelGGL(k_mean_delta<T>, dim3((lc + 255) / 256), dim3(256), 0, st, (const T*)acc, counts, (const T*)oldc,
+                     (T*)newc, lc, delta_bits, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const uint32_t* counts, const void* old_c,
+                             void* new_c, int lc, unsigned long long* delta_bits, int* err) {
+  switch (dtype) {
+    case ODBA_F32: return md_t<float>(st, acc, counts, old_c, new_c, lc, delta_bits, err);
+    case ODBA_F64: return md_t<double>(st, acc, counts, old_c, new_c, lc, delta_bits, err);
+    case ODBA_I32: return md_t<int32_t>(st, acc, counts, old_c, new_c, lc, delta_bits, err);
+    case ODBA_U32: return md_t<uint32_t>(st, acc, counts, old_c, new_c, lc, delta_bits, err);
+    case ODBA_U64: return md_t<uint64_t>(st, acc, counts, old_c, new_c, lc, delta_bits, err);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t zn_t(hipStream_t st, void* data, const long long* offs, const int* lens, int n, double* mu,
+                       double* sigma) {
+  hipLaunchKernelGGL(k_znorm<T>, dim3(n), dim3(1024), 0, st, (T*)data, offs, lens, n, mu, sigma);
+  return hipGetLastError();
+}
+
+hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* offs, const int* lens, int n,
+                        double* mu, double* sigma) {
+  if (n <= 0) return hipSuccess;
+  switch (dtype) {
+    case ODBA_F32: return zn_t<float>(st, data, offs, lens, n, mu, sigma);
+    case ODBA_F64: return zn_t<double>(st, data, offs, lens, n, mu, sigma);
+    case ODBA_I32: return zn_t<int32_t>(st, data, offs, lens, n, mu, sigma);
+    case ODBA_U32: return zn_t<uint32_t>(st, data, offs, lens, n, mu, sigma);
+    case ODBA_U64: return zn_t<uint64_t>(st, data, offs, lens, n, mu, sigma);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_decode_moves(hipStream_t st, const uint32_t* mv, const uint32_t* mtop, int ab, int ly, int lx, int os,
+                               int oe, uint8_t* out) {
+  long long cells = (long long)ly * lx;
+  if (cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_decode_moves, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, mv, mtop, ab, ly, lx, os,
+                     oe, out);
+  return hipGetLastError();
+}
+
+}  // namespace odba

@@ -1,0 +1,62 @@
+// Internal declarations shared by the HIP kernel files and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/odba/odba_types.h"
+
+#define ODBA_W 1024  // swath width; equals the reference's blockDim.x (dba.hpp:41-48, 453-457)
+
+namespace odba {
+
+// Padded element count of a stored sequence: every swath may read one full
+// 1024-wide column block and the row window 128 entries past the end.
+inline long long padded_len(long long L) { return ((L + 63) / 64) * 64 + 1024; }
+
+inline int last_swath_c_host(int wcols) {
+  if (wcols <= 64) return 1;
+  if (wcols <= 128) return 2;
+  if (wcols <= 256) return 4;
+  if (wcols <= 512) return 8;
+  if (wcols <= 768) return 12;
+  return 16;
+}
+
+// Rows covered by the skewed main loop of a swath (the rest is positional
+// or the open-end top-row scan).
+inline int main_rows(int ly, int os, int oe) { return ((oe && ly >= 2) ? ly - 2 : ly - 1) - (os ? 1 : 0) + 1; }
+inline long long swath_words(int ly, int os, int oe) {
+  int nr = main_rows(ly, os, oe);
+  return nr > 0 ? (long long)(nr + 63) * 64 : 0;
+}
+
+hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* data,
+                            const long long* offs, const int* lens, int n, long long npairs,
+                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags);
+hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,
+                            int njobs, unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
+                            long long stride);
+size_t path_job_size(int dtype);
+
+// Traceback job: geometry of one aligned pair in path mode.
+struct TbJob {
+  int ly, lx;          // Y rows, X columns
+  int flip;            // 1: centroid on Y (dba.hpp:497-501)
+  int seq;             // index of the sequence in the cluster order
+  long long mv_off;    // word offset of the pair's step words
+  long long top_off;   // word offset of the open-end top-row words
+};
+
+hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
+                            const int* abr, int2* runs, int lc, int os, int oe, int* err);
+hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int2* runs,
+                             int nseq, int lc, void* acc, uint32_t* counts, int first_batch);
+hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const uint32_t* counts, const void* old_c,
+                             void* new_c, int lc, unsigned long long* delta_bits, int* err);
+hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* offs, const int* lens, int n,
+                        double* mu, double* sigma);
+hipError_t launch_decode_moves(hipStream_t st, const uint32_t* mv, const uint32_t* mtop, int ab, int ly, int lx, int os,
+                               int oe, uint8_t* out);
+
+}  // namespace odba

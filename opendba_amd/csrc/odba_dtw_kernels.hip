@@ -1,0 +1,688 @@
+// DTW swath kernels for gfx950 (CDNA4): all-vs-all distances and
+// path-storing DTW for the DBA update.
+//
+// Semantics follow DTWDistance<T> (dtw.hpp:39-259) exactly, including the
+// swath-width-dependent open-end rules; the work decomposition is this
+// file's own:
+//
+//   * One wavefront (64 lanes) sweeps one pair. The cost matrix is cut into
+//     vertical swaths 1024 columns wide, aligned like the reference's
+//     launches (swath k covers columns [1024k, 1024k+1024)). Inside a swath,
+//     lane l owns C consecutive columns (C = 16 for full swaths, fewer for
+//     the last, partial one) and the lanes advance one row per step with a
+//     one-step skew (lane l works on row s - l at step s). The left
+//     neighbour value crosses lanes with a DPP wave_shr:1; lane 0 takes the
+//     previous swath's right edge column instead. Each step costs 4 VALU ops
+//     per cell (sub, mul, min3, add) plus one DPP move per lane.
+//   * The row values y[i] and the left edge column stream through a
+//     128-entry LDS window per wave; the swath's right edge column is
+//     collected in LDS and written to HBM every 64 rows. Nothing else of the
+//     cost matrix leaves registers.
+//   * The open-end top row (dtw.hpp:196-200, 172-186) is a prefix minimum
+//     along the row, computed after the swath body as an in-lane scan plus a
+//     cross-lane exclusive scan.
+//   * The open-end early abrogation (dtw.hpp:74-129) is evaluated between
+//     swaths from the running minimum of the edge column.
+//   * A persistent grid pulls pairs from an atomic counter, largest second
+//     sequence first.
+//
+// Path mode additionally writes a 2-bit step code per cell (DIAG=0,
+// RIGHT=1, UP=2), one 32-bit word per lane per step, so every store of a
+// step is a coalesced 256-byte row. OPEN_RIGHT, NIL and NIL_OPEN_RIGHT are
+// positional (open-end top row, open-start row 0, abrogated swaths, the
+// corner) and are reconstructed by the traceback.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+#include <limits.h>
+
+#include "odba_dtw.hpp"
+
+namespace odba {
+
+// ---------------------------------------------------------------------------
+// Per-type arithmetic
+// ---------------------------------------------------------------------------
+template <typename T> struct Ar;
+
+template <> struct Ar<float> {
+  static constexpr bool kFloat = true;
+  __device__ static float maxv() { return FLT_MAX; }
+  __device__ static float rowinf() { return __builtin_inff(); }
+  __device__ static float mn(float a, float b) { return __builtin_fminf(a, b); }
+  __device__ static float mn3(float a, float b, float c) { return __builtin_fminf(__builtin_fminf(a, b), c); }
+  __device__ static float shr(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+  }
+  __device__ static float tofloat(float v) { return v; }
+  __device__ static float fromfloat(float f) { return f; }
+};
+
+template <> struct Ar<double> {
+  static constexpr bool kFloat = true;
+  __device__ static double maxv() { return DBL_MAX; }
+  __device__ static double rowinf() { return __builtin_inf(); }
+  __device__ static double mn(double a, double b) { return __builtin_fmin(a, b); }
+  __device__ static double mn3(double a, double b, double c) { return __builtin_fmin(__builtin_fmin(a, b), c); }
+  __device__ static double shr(double old, double v) {
+    int2 o = *reinterpret_cast<int2*>(&old), s = *reinterpret_cast<int2*>(&v), r;
+    r.x = __builtin_amdgcn_update_dpp(o.x, s.x, 0x138, 0xf, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(o.y, s.y, 0x138, 0xf, 0xf, false);
+    return *reinterpret_cast<double*>(&r);
+  }
+  __device__ static float tofloat(double v) { return (float)v; }
+  __device__ static double fromfloat(float f) { return (double)f; }
+};
+
+template <> struct Ar<int32_t> {
+  static constexpr bool kFloat = false;
+  __device__ static int32_t maxv() { return INT_MAX; }
+  __device__ static int32_t rowinf() { return INT_MAX; }
+  __device__ static int32_t mn(int32_t a, int32_t b) { return a < b ? a : b; }
+  __device__ static int32_t mn3(int32_t a, int32_t b, int32_t c) { return mn(mn(a, b), c); }
+  __device__ static int32_t shr(int32_t old, int32_t v) { return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false); }
+  __device__ static float tofloat(int32_t v) { return (float)v; }
+  __device__ static int32_t fromfloat(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT_MAX;
+    if (f <= -2147483648.0f) return INT_MIN;
+    return (int32_t)f;
+  }
+};
+
+template <> struct Ar<uint32_t> {
+  static constexpr bool kFloat = false;
+  __device__ static uint32_t maxv() { return UINT_MAX; }
+  __device__ static uint32_t rowinf() { return UINT_MAX; }
+  __device__ static uint32_t mn(uint32_t a, uint32_t b) { return a < b ? a : b; }
+  __device__ static uint32_t mn3(uint32_t a, uint32_t b, uint32_t c) { return mn(mn(a, b), c); }
+  __device__ static uint32_t shr(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false);
+  }
+  __device__ static float tofloat(uint32_t v) { return (float)v; }
+  __device__ static uint32_t fromfloat(float f) {
+    if (f != f || f <= 0.0f) return 0;
+    if (f >= 4294967296.0f) return UINT_MAX;
+    return (uint32_t)f;
+  }
+};
+
+template <> struct Ar<uint64_t> {
+  static constexpr bool kFloat = false;
+  __device__ static uint64_t maxv() { return ULLONG_MAX; }
+  __device__ static uint64_t rowinf() { return ULLONG_MAX; }
+  __device__ static uint64_t mn(uint64_t a, uint64_t b) { return a < b ? a : b; }
+  __device__ static uint64_t mn3(uint64_t a, uint64_t b, uint64_t c) { return mn(mn(a, b), c); }
+  __device__ static uint64_t shr(uint64_t old, uint64_t v) {
+    int2 o = *reinterpret_cast<int2*>(&old), s = *reinterpret_cast<int2*>(&v), r;
+    r.x = __builtin_amdgcn_update_dpp(o.x, s.x, 0x138, 0xf, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(o.y, s.y, 0x138, 0xf, 0xf, false);
+    return *reinterpret_cast<uint64_t*>(&r);
+  }
+  __device__ static float tofloat(uint64_t v) { return (float)v; }
+  __device__ static uint64_t fromfloat(float f) {
+    if (f != f || f <= 0.0f) return 0;
+    if (f >= 18446744073709551616.0f) return ULLONG_MAX;
+    return (uint64_t)f;
+  }
+};
+
+// Wavefront-scope ordering point for LDS traffic between lanes of one wave:
+// LDS operations of a wave execute in order, this only stops the compiler
+// from moving memory operations across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T> __device__ __forceinline__ T wave_min(T v) {
+  for (int d = 32; d > 0; d >>= 1) v = Ar<T>::mn(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+// Exclusive prefix minimum across the 64 lanes, seeded by `seed` at lane 0.
+template <typename T> __device__ __forceinline__ T wave_excl_prefix_min(T tot, T seed, int lane) {
+  T v = tot;
+  for (int d = 1; d < 64; d <<= 1) {
+    T t = __shfl_up(v, d, 64);
+    if (lane >= d) v = Ar<T>::mn(v, t);
+  }
+  T e = __shfl_up(v, 1, 64);
+  return lane == 0 ? seed : Ar<T>::mn(e, seed);
+}
+
+// ---------------------------------------------------------------------------
+// One cell update. MINFIRST: min3(up,left,diag) + dd, bit-identical to the
+// reference's add-then-compare for floats (rounding is monotone) and for
+// integers that cannot wrap. Otherwise the add-then-min form with explicit
+// missing-candidate flags (integer wrap-around keeps the reference's order).
+// PATH: the comparison is on the post-addition values, as the reference's
+// step selection is (dtw.hpp:202-227); the 2-bit code is returned.
+// ---------------------------------------------------------------------------
+struct CellFlags {
+  bool up_ok, left_ok, diag_ok;
+};
+
+template <typename T, bool MINFIRST, bool PATH>
+__device__ __forceinline__ T cell(T up, T left, T diag, T dd, uint32_t& code) {
+  if (MINFIRST && !PATH) {
+    return Ar<T>::mn3(up, left, diag) + dd;
+  } else {
+    T U = up + dd, R = left + dd, G = diag + dd;
+    T v = Ar<T>::mn3(U, R, G);
+    if (PATH) code = (G == v) ? 0u : ((U == v) ? 2u : 1u);
+    return v;
+  }
+}
+
+// Add-then-min with per-candidate presence (integer types that may wrap).
+template <typename T, bool PATH>
+__device__ __forceinline__ T cell_wrap(T up, T left, T diag, T dd, CellFlags f, uint32_t& code) {
+  const T M = Ar<T>::maxv();
+  T U = f.up_ok ? (T)(up + dd) : M;
+  T R = f.left_ok ? (T)(left + dd) : M;
+  T G = f.diag_ok ? (T)(diag + dd) : M;
+  // reference selection (dtw.hpp:202-227)
+  T v;
+  uint32_t c;
+  if (G > U) {
+    if (U > R) { v = R; c = 1u; } else { v = U; c = 2u; }
+  } else {
+    if (G > R) { v = R; c = 1u; } else { v = G; c = 0u; }
+  }
+  if (PATH) code = c;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Per-wave LDS staging window.
+// ---------------------------------------------------------------------------
+// Separate objects so the compiler can move window reads across the
+// edge-slot writes of earlier steps.
+template <typename T> struct WaveWin {
+  T* __restrict__ y;  // 128 entries: y rows [base-64, base+64)
+  T* __restrict__ e;  // 64 entries: left edge column rows [base, base+64)
+  T* __restrict__ o;  // 128 entries: right edge column slots rows [base-64, base+64)
+};
+
+template <typename T> struct SwathIO {
+  const T* y;      // first sequence (rows), padded
+  int ly;          // number of rows (may be a truncated height)
+  const T* x;      // second sequence (columns), padded
+  int lx;
+  int off;         // swath left column
+  const T* ein;    // previous swath right edge column (null for off == 0)
+  T* eout;         // this swath's right edge column (null: not needed)
+  uint32_t* mv;    // path words for this swath (null: distance only)
+  uint32_t* mtop;  // path words of the open-end top row (64 per swath)
+};
+
+template <typename T> struct SwathRes {
+  T colmin;  // min over the right edge column (valid when eout != null)
+  T top;     // right edge value of the top row
+  T last;    // D(ly-1, lx-1) when the swath holds column lx-1
+};
+
+// Runs one swath of C columns per lane. Rows r0..r1 go through the skewed
+// main loop; with open_end the top row goes through the prefix-min scan.
+template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH>
+__device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int lane) {
+  const T MAXV = Ar<T>::maxv();
+  const T RINF = Ar<T>::rowinf();
+  const int ly = a.ly, off = a.off;
+  const int r0 = OS ? 1 : 0;
+  const int r1 = (OE && ly >= 2) ? ly - 2 : ly - 1;
+  const int nrows = r1 - r0 + 1;  // may be 0
+  const int steps = nrows > 0 ? nrows + 63 : 0;
+  const int nblk = (steps + 63) >> 6;
+  const bool first_swath = (off == 0);
+  const bool edge_out = (a.eout != nullptr);
+  const int col0 = off + lane * C;  // first global column of this lane
+
+  T x[C], D[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    x[q] = a.x[col0 + q];  // padded: always in bounds
+    D[q] = OS ? T(0) : RINF;
+  }
+  // diagonal input of the lane's first row
+  T diag_in;
+  if (lane == 0) {
+    if (first_swath) diag_in = OS ? MAXV : T(0);
+    else diag_in = OS ? a.ein[0] : RINF;
+  } else {
+    diag_in = OS ? T(0) : RINF;
+  }
+
+  SwathRes<T> res;
+  T runmin = MAXV;
+
+  // prefetch block 0 inputs
+  auto load_y = [&](int b) -> T { return a.y[r0 + 64 * b + lane]; };
+  auto load_e = [&](int b) -> T {
+    if (first_swath) return MAXV;
+    int r = r0 + 64 * b + lane;
+    return a.ein[r];  // edge buffers are padded by 128 entries
+  };
+  T ynext = nblk > 0 ? load_y(0) : T(0);
+  T enext = nblk > 0 ? load_e(0) : T(0);
+
+  for (int b = 0; b < nblk; b++) {
+    // --- stage the window for block b
+    T yshift = W->y[64 + lane];
+    wave_sync();
+    W->y[lane] = yshift;
+    W->y[64 + lane] = ynext;
+    W->e[lane] = enext;
+    wave_sync();
+    if (b + 1 < nblk) { ynext = load_y(b + 1); enext = load_e(b + 1); }
+
+    const int sbase = 64 * b;
+    const bool steady = (sbase >= 63) && (sbase + 63 < nrows);
+    if (steady) {
+#pragma unroll 8
+      for (int u = 0; u < 64; u++) {
+        const int s = sbase + u;
+        T yv = W->y[u - lane + 64];
+        T ev = W->e[u];
+        T left_in = Ar<T>::shr(ev, D[C - 1]);
+        T dg = diag_in;
+        diag_in = left_in;
+        T l = left_in;
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          T up = D[q];
+          T df = yv - x[q];
+          T dd = df * df;
+          uint32_t code = 0;
+          T v;
+          if (MINFIRST || Ar<T>::kFloat) {
+            v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
+          } else {
+            const int row = r0 + s - lane;
+            CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
+            T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
+            v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
+          }
+          if (PATH) word |= code << (2 * q);
+          dg = up;
+          D[q] = v;
+          l = v;
+        }
+        if (PATH) a.mv[(size_t)s * 64 + lane] = word;
+        W->o[u - lane + 64] = D[C - 1];
+      }
+    } else {
+      for (int u = 0; u < 64; u++) {
+        const int s = sbase + u;
+        T yv = W->y[u - lane + 64];
+        T ev = W->e[u];
+        T left_in = Ar<T>::shr(ev, D[C - 1]);
+        const int row = r0 + s - lane;
+        const bool active = (unsigned)(s - lane) < (unsigned)nrows;
+        if (active) {
+          T dg = diag_in;
+          diag_in = left_in;
+          T l = left_in;
+          uint32_t word = 0;
+#pragma unroll
+          for (int q = 0; q < C; q++) {
+            T up = D[q];
+            T df = yv - x[q];
+            T dd = df * df;
+            uint32_t code = 0;
+            T v;
+            if (MINFIRST || Ar<T>::kFloat) {
+              v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
+            } else {
+              CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
+              T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
+              v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
+            }
+            if (PATH) word |= code << (2 * q);
+            dg = up;
+            D[q] = v;
+            l = v;
+          }
+          if (PATH) a.mv[(size_t)s * 64 + lane] = word;
+        }
+        W->o[u - lane + 64] = D[C - 1];
+      }
+    }
+    wave_sync();
+    // --- flush final right-edge values of rows [r0+sbase-64, r0+sbase)
+    if (edge_out) {
+      const int row = r0 + sbase - 64 + lane;
+      T v = W->o[lane];
+      if (row >= r0 && row <= r1) {
+        a.eout[row] = v;
+        runmin = Ar<T>::mn(runmin, v);
+      }
+      T sh = W->o[64 + lane];
+      wave_sync();
+      W->o[lane] = sh;
+      wave_sync();
+    }
+  }
+  if (edge_out && nblk > 0) {
+    const int row = r0 + 64 * nblk - 64 + lane;
+    T v = W->o[lane];
+    if (row >= r0 && row <= r1) {
+      a.eout[row] = v;
+      runmin = Ar<T>::mn(runmin, v);
+    }
+  }
+  if (OS && edge_out && lane == 0) a.eout[0] = T(0);
+  if (OS) runmin = Ar<T>::mn(runmin, T(0));
+
+  // --- open-end top row: D(t,j) = min(min(U,G), D(t,j-1)) with U,G carrying
+  // the local cost, except at the swath-left column (dtw.hpp:175-186).
+  if (OE && ly >= 2) {
+    const int t = ly - 1;
+    const T yv = a.y[t];
+    const T lane0_diag = first_swath ? MAXV : a.ein[t - 1];
+    const T left0 = first_swath ? MAXV : a.ein[t];
+    const T nb = Ar<T>::shr(lane0_diag, D[C - 1]);
+    // pass 1: m[q] = running min over the lane of min(U, G); U = up + dd,
+    // G = diag + dd except at the swath-left column where the reference adds
+    // no local cost to the diagonal (and at column 0 the diagonal is MAX).
+    T m[C];
+    T run = T(0);
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const T df = yv - x[q];
+      const T dd = df * df;
+      const T U = D[q] + dd;
+      const T G = (q == 0 && lane == 0) ? nb : ((q == 0 ? nb : D[q - 1]) + dd);
+      const T mm = (G > U) ? U : G;
+      run = (q == 0) ? mm : Ar<T>::mn(run, mm);
+      m[q] = run;
+    }
+    const T pre = wave_excl_prefix_min<T>(m[C - 1], left0, lane);
+    // pass 2: values and step codes (reference tie order: DIAG, UP, RIGHT)
+    uint32_t word = 0;
+    T dprev_old = nb;
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const T df = yv - x[q];
+      const T dd = df * df;
+      const T U = D[q] + dd;
+      const T G = (q == 0 && lane == 0) ? nb : (dprev_old + dd);
+      const T v = Ar<T>::mn(pre, m[q]);
+      if (PATH) word |= ((G == v) ? 0u : ((U == v) ? 2u : 1u)) << (2 * q);
+      dprev_old = D[q];
+      D[q] = v;
+    }
+    if (PATH) a.mtop[lane] = word;
+    if (edge_out) {
+      T top = __shfl(D[C - 1], 63, 64);
+      if (lane == 0) a.eout[t] = top;
+      runmin = Ar<T>::mn(runmin, top);
+    }
+  }
+  if (edge_out) {
+    res.colmin = wave_min<T>(runmin);
+    res.top = __shfl(D[C - 1], 63, 64);  // lane 63 holds the last row at the right edge
+  }
+  // value at the pair's last column (only meaningful for the swath holding
+  // it): the owning lane parks its C values in the LDS slot window, every
+  // lane reads the wanted one (static register indices only).
+  {
+    const int tl = a.lx - 1 - off;
+    res.last = T(0);
+    if (tl >= 0 && tl < 64 * C) {
+      const int ll = tl / C, qq = tl - (tl / C) * C;
+      wave_sync();
+      if (lane == ll) {
+#pragma unroll
+        for (int k = 0; k < C; k++) W->o[k] = D[k];
+      }
+      wave_sync();
+      res.last = W->o[qq];
+      wave_sync();
+    }
+  }
+  if (edge_out || a.mv) __builtin_amdgcn_s_waitcnt(0);
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// Whole pair: swaths left to right with the abrogation check between them.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T pair_distance(T cost, int ly, bool normalise) {
+  float f = __fsqrt_rn(Ar<T>::tofloat(cost));
+  if (normalise) f = __fdiv_rn(f, (float)ly);
+  return Ar<T>::fromfloat(f);
+}
+
+template <typename T, bool OS, bool OE, bool MINFIRST, bool PATH, int C>
+__device__ SwathRes<T> swath_c(const SwathIO<T>& io, const WaveWin<T>* W, int lane) {
+  return run_swath<T, C, OS, OE, MINFIRST, PATH>(io, W, lane);
+}
+
+template <typename T, bool OS, bool OE, bool MINFIRST, bool PATH>
+__device__ SwathRes<T> swath_dispatch(int C, const SwathIO<T>& io, const WaveWin<T>* W, int lane) {
+  switch (C) {
+    case 1: return swath_c<T, OS, OE, MINFIRST, PATH, 1>(io, W, lane);
+    case 2: return swath_c<T, OS, OE, MINFIRST, PATH, 2>(io, W, lane);
+    case 4: return swath_c<T, OS, OE, MINFIRST, PATH, 4>(io, W, lane);
+    case 8: return swath_c<T, OS, OE, MINFIRST, PATH, 8>(io, W, lane);
+    case 12: return swath_c<T, OS, OE, MINFIRST, PATH, 12>(io, W, lane);
+    default: return swath_c<T, OS, OE, MINFIRST, PATH, 16>(io, W, lane);
+  }
+}
+
+__host__ __device__ inline int last_swath_c(int wcols) {
+  if (wcols <= 64) return 1;
+  if (wcols <= 128) return 2;
+  if (wcols <= 256) return 4;
+  if (wcols <= 512) return 8;
+  if (wcols <= 768) return 12;
+  return 16;
+}
+
+struct PairOut {
+  int abrogated;  // swath index where the open-end search stopped, -1 if none
+  int q3;         // 1 when the reference would read an unwritten LDS slot (ly == 1)
+};
+
+template <typename T, bool OS, bool OE, bool MINFIRST, bool PATH>
+__device__ T dtw_one_pair(const T* y, int ly, const T* x, int lx, T* ebuf0, T* ebuf1, uint32_t* mvbase,
+                          uint32_t* mtop, const WaveWin<T>* W, int lane, PairOut& po) {
+  const int K = (lx + ODBA_W - 1) / ODBA_W;
+  const int nrows_main = ((OE && ly >= 2) ? ly - 2 : ly - 1) - (OS ? 1 : 0) + 1;
+  const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
+  po.abrogated = -1;
+  po.q3 = 0;
+  T colmin = T(0), top = T(0);
+  for (int k = 0; k < K; k++) {
+    const int off = k * ODBA_W;
+    if (k > 0 && OE && !OS && off > ly) {
+      if (top == colmin) {  // dtw.hpp:106-127
+        po.abrogated = k;
+        return pair_distance<T>(ly == 1 ? Ar<T>::maxv() : top, ly, true);
+      }
+    }
+    const bool last = (k == K - 1);
+    const int wcols = last ? lx - off : ODBA_W;
+    SwathIO<T> io;
+    io.y = y; io.ly = ly; io.x = x; io.lx = lx; io.off = off;
+    io.ein = k > 0 ? (((k - 1) & 1) ? ebuf1 : ebuf0) : nullptr;
+    io.eout = last ? nullptr : ((k & 1) ? ebuf1 : ebuf0);
+    io.mv = PATH ? mvbase + (size_t)k * swath_words : nullptr;
+    io.mtop = PATH ? mtop + (size_t)k * 64 : nullptr;
+    const int C = last ? last_swath_c(wcols) : 16;
+    SwathRes<T> r = swath_dispatch<T, OS, OE, MINFIRST, PATH>(C, io, W, lane);
+    if (!last) { colmin = r.colmin; top = r.top; }
+    if (last) {
+      if (ly == 1 && ((wcols - 1) % 3) != 0) po.q3 = 1;
+      T cost = r.last;
+      if (OS && ly == 1) cost = T(0);
+      return pair_distance<T>(cost, ly, OE != OS);
+    }
+  }
+  return T(0);
+}
+
+// ---------------------------------------------------------------------------
+// All-vs-all kernel. Pair order: second sequence index c descending, then
+// first index r descending (longest X first). Pair (r,c) has Y = seq r.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decode_pair(long long p, long long npairs, int& r, int& c) {
+  long long q = npairs - 1 - p;  // reverse index: q = c(c-1)/2 + r
+  long long cc = (long long)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
+  while (cc * (cc - 1) / 2 > q) cc--;
+  while ((cc + 1) * cc / 2 <= q) cc++;
+  c = (int)cc;
+  r = (int)(q - cc * (cc - 1) / 2);
+}
+
+template <typename T, bool OS, bool OE, bool MINFIRST>
+__global__ void __launch_bounds__(256) k_all_pairs(const T* __restrict__ data, const long long* __restrict__ offs,
+                                                   const int* __restrict__ lens, int n, long long npairs,
+                                                   unsigned long long* __restrict__ counter, T* __restrict__ dist,
+                                                   T* __restrict__ escratch, long long escratch_stride,
+                                                   int* __restrict__ flags) {
+  __shared__ T s_y[4][128];
+  __shared__ T s_e[4][64];
+  __shared__ T s_o[4][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const long long gw = (long long)blockIdx.x * 4 + wid;
+  T* e0 = escratch + gw * 2 * escratch_stride;
+  T* e1 = e0 + escratch_stride;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  const WaveWin<T>* W = &win;
+  for (;;) {
+    unsigned long long p = 0;
+    if (lane == 0) p = atomicAdd(counter, 1ull);
+    p = __shfl(p, 0, 64);
+    if ((long long)p >= npairs) break;
+    int r, c;
+    decode_pair((long long)p, npairs, r, c);
+    PairOut po;
+    T d = dtw_one_pair<T, OS, OE, MINFIRST, false>(data + offs[r], lens[r], data + offs[c], lens[c], e0, e1, nullptr,
+                                                    nullptr, W, lane, po);
+    if (lane == 0) {
+      long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
+      dist[idx] = d;
+      if (flags) flags[idx] = (po.abrogated >= 0 ? (1 | (po.abrogated << 8)) : 0) | (po.q3 ? 2 : 0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Path-DTW jobs (DBA update): job j aligns Y=ys[j] against X=xs[j] and stores
+// step words at mv + mv_off[j], top-row words at mtop + top_off[j].
+// ---------------------------------------------------------------------------
+template <typename T> struct PathJob {
+  const T* y;
+  const T* x;
+  int ly, lx;
+  long long mv_off;   // in 32-bit words
+  long long top_off;  // in 32-bit words
+};
+
+template <typename T, bool OS, bool OE, bool MINFIRST>
+__global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict__ jobs, int njobs,
+                                                   unsigned int* __restrict__ counter, uint32_t* __restrict__ mv,
+                                                   uint32_t* __restrict__ mtop, int* __restrict__ abrogated,
+                                                   T* __restrict__ escratch, long long escratch_stride) {
+  __shared__ T s_y[4][128];
+  __shared__ T s_e[4][64];
+  __shared__ T s_o[4][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const long long gw = (long long)blockIdx.x * 4 + wid;
+  T* e0 = escratch + gw * 2 * escratch_stride;
+  T* e1 = e0 + escratch_stride;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  const WaveWin<T>* W = &win;
+  for (;;) {
+    unsigned int j = 0;
+    if (lane == 0) j = atomicAdd(counter, 1u);
+    j = __shfl(j, 0, 64);
+    if ((int)j >= njobs) break;
+    PathJob<T> jb = jobs[j];
+    PairOut po;
+    (void)dtw_one_pair<T, OS, OE, MINFIRST, true>(jb.y, jb.ly, jb.x, jb.lx, e0, e1, mv + jb.mv_off, mtop + jb.top_off,
+                                                  W, lane, po);
+    if (lane == 0) abrogated[j] = po.abrogated;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers (instantiation dispatch).
+// ---------------------------------------------------------------------------
+template <typename T, bool MINFIRST>
+static hipError_t launch_all_pairs_t(int os, int oe, dim3 grid, hipStream_t st, const T* data, const long long* offs,
+                                     const int* lens, int n, long long npairs, unsigned long long* counter, T* dist,
+                                     T* esc, long long stride, int* flags) {
+  if (os && oe) hipLaunchKernelGGL((k_all_pairs<T, true, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+  else if (os) hipLaunchKernelGGL((k_all_pairs<T, true, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+  else if (oe) hipLaunchKernelGGL((k_all_pairs<T, false, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+  else hipLaunchKernelGGL((k_all_pairs<T, false, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+  return hipGetLastError();
+}
+
+template <typename T, bool MINFIRST>
+static hipError_t launch_path_jobs_t(int os, int oe, dim3 grid, hipStream_t st, const void* jobs, int njobs,
+                                     unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, T* esc,
+                                     long long stride) {
+  const PathJob<T>* J = (const PathJob<T>*)jobs;
+  if (os && oe) hipLaunchKernelGGL((k_path_jobs<T, true, true, MINFIRST>), grid, dim3(256), 0, st, J, njobs, counter, mv, mtop, abr, esc, stride);
+  else if (os) hipLaunchKernelGGL((k_path_jobs<T, true, false, MINFIRST>), grid, dim3(256), 0, st, J, njobs, counter, mv, mtop, abr, esc, stride);
+  else if (oe) hipLaunchKernelGGL((k_path_jobs<T, false, true, MINFIRST>), grid, dim3(256), 0, st, J, njobs, counter, mv, mtop, abr, esc, stride);
+  else hipLaunchKernelGGL((k_path_jobs<T, false, false, MINFIRST>), grid, dim3(256), 0, st, J, njobs, counter, mv, mtop, abr, esc, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* data,
+                            const long long* offs, const int* lens, int n, long long npairs,
+                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags) {
+  dim3 g(nblocks);
+  switch (dtype) {
+    case ODBA_F32: return launch_all_pairs_t<float, true>(os, oe, g, st, (const float*)data, offs, lens, n, npairs, counter, (float*)dist, (float*)esc, stride, flags);
+    case ODBA_F64: return launch_all_pairs_t<double, true>(os, oe, g, st, (const double*)data, offs, lens, n, npairs, counter, (double*)dist, (double*)esc, stride, flags);
+    case ODBA_I32:
+      if (minfirst) return launch_all_pairs_t<int32_t, true>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
+      return launch_all_pairs_t<int32_t, false>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
+    case ODBA_U32:
+      if (minfirst) return launch_all_pairs_t<uint32_t, true>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
+      return launch_all_pairs_t<uint32_t, false>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
+    case ODBA_U64:
+      if (minfirst) return launch_all_pairs_t<uint64_t, true>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
+      return launch_all_pairs_t<uint64_t, false>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,
+                            int njobs, unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
+                            long long stride) {
+  dim3 g(nblocks);
+  switch (dtype) {
+    case ODBA_F32: return launch_path_jobs_t<float, true>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (float*)esc, stride);
+    case ODBA_F64: return launch_path_jobs_t<double, true>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (double*)esc, stride);
+    case ODBA_I32: return launch_path_jobs_t<int32_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (int32_t*)esc, stride);
+    case ODBA_U32: return launch_path_jobs_t<uint32_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (uint32_t*)esc, stride);
+    case ODBA_U64: return launch_path_jobs_t<uint64_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (uint64_t*)esc, stride);
+  }
+  return hipErrorInvalidValue;
+}
+
+size_t path_job_size(int dtype) {
+  switch (dtype) {
+    case ODBA_F32: return sizeof(PathJob<float>);
+    case ODBA_F64: return sizeof(PathJob<double>);
+    case ODBA_I32: return sizeof(PathJob<int32_t>);
+    case ODBA_U32: return sizeof(PathJob<uint32_t>);
+    case ODBA_U64: return sizeof(PathJob<uint64_t>);
+  }
+  return 0;
+}
+
+}  // namespace odba

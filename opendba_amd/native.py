@@ -1,0 +1,186 @@
+"""ctypes binding over libodba_hip.so (include/odba/odba.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libodba_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "odba", "odba.h")
+
+I32, U32, U64, F32, F64 = 0, 1, 2, 3, 4
+NP = {I32: np.int32, U32: np.uint32, U64: np.uint64, F32: np.float32, F64: np.float64}
+CODE = {np.dtype(v): k for k, v in NP.items()}
+PATH_AUTO, PATH_FULL, PATH_STRIPE = 0, 1, 2
+
+EXPORTS = [
+    "odba_ctx_create", "odba_ctx_destroy", "odba_last_error", "odba_device_info",
+    "odba_dtw_all_pairs", "odba_seqset_create", "odba_seqset_destroy", "odba_dtw_all_pairs_resident",
+    "odba_dtw_path", "odba_dba_round", "odba_znorm", "odba_perform_dba",
+]
+
+
+class OdbaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"odba error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OdbaError(-1, f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C opendba_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    L.odba_ctx_create.argtypes = [i32, C.POINTER(vp)]
+    L.odba_ctx_destroy.argtypes = [vp]
+    L.odba_last_error.argtypes = [vp]
+    L.odba_last_error.restype = C.c_char_p
+    L.odba_device_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u64), C.c_char_p, i32]
+    L.odba_dtw_all_pairs.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, vp]
+    L.odba_seqset_create.argtypes = [vp, i32, vp, vp, i32, C.POINTER(vp)]
+    L.odba_seqset_destroy.argtypes = [vp]
+    L.odba_dtw_all_pairs_resident.argtypes = [vp, vp, i32, i32, vp, vp, C.POINTER(C.c_double)]
+    L.odba_dtw_path.argtypes = [vp, i32, vp, u64, vp, u64, i32, i32, vp, vp]
+    L.odba_dba_round.argtypes = [vp, i32, vp, u64, vp, vp, i32, i32, i32, i32, u64, vp, vp, C.POINTER(C.c_double)]
+    L.odba_znorm.argtypes = [vp, i32, vp, vp, i32, vp, vp]
+    L.odba_perform_dba.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, i32, u64, vp, C.POINTER(u64),
+                                   C.POINTER(i32), vp, C.POINTER(i32), vp]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _ptrs(seqs):
+    return (C.c_void_p * len(seqs))(*[s.ctypes.data for s in seqs])
+
+
+class Context:
+    """One HIP device + stream (odba_ctx)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        self.h = C.c_void_p()
+        rc = self.L.odba_ctx_create(device, C.byref(self.h))
+        if rc:
+            raise OdbaError(rc, "odba_ctx_create failed (no HIP device?)")
+
+    def close(self):
+        if self.h:
+            self.L.odba_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise OdbaError(rc, self.L.odba_last_error(self.h).decode())
+
+    def device_info(self):
+        cu, clk, mem = C.c_int(), C.c_int(), C.c_uint64()
+        name = C.create_string_buffer(64)
+        self._check(self.L.odba_device_info(self.h, C.byref(cu), C.byref(clk), C.byref(mem), name, 64))
+        return dict(compute_units=cu.value, clock_khz=clk.value, mem_bytes=mem.value, name=name.value.decode())
+
+    def all_pairs(self, seqs, open_start=0, open_end=0):
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        dt = CODE[seqs[0].dtype]
+        n = len(seqs)
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        out = np.zeros(n * (n - 1) // 2, dtype=seqs[0].dtype)
+        self._check(self.L.odba_dtw_all_pairs(self.h, dt, _ptrs(seqs), _p(lens), n, open_start, open_end, 1024,
+                                              _p(out)))
+        return out
+
+    def seqset(self, seqs):
+        return SeqSet(self, seqs)
+
+    def dtw_path(self, y, x, open_start=0, open_end=0):
+        y = np.ascontiguousarray(y)
+        x = np.ascontiguousarray(x)
+        M = np.zeros((len(y), len(x)), dtype=np.uint8)
+        d = np.zeros(1, dtype=y.dtype)
+        self._check(self.L.odba_dtw_path(self.h, CODE[y.dtype], _p(y), len(y), _p(x), len(x), open_start, open_end,
+                                         _p(M), _p(d)))
+        return d[0], M
+
+    def dba_round(self, centroid, seqs, open_start=0, open_end=0, path_policy=PATH_FULL, ref_free_mem=0):
+        centroid = np.ascontiguousarray(centroid)
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        out = np.zeros_like(centroid)
+        counts = np.zeros(len(centroid), dtype=np.uint32)
+        delta = C.c_double(0)
+        self._check(self.L.odba_dba_round(self.h, CODE[centroid.dtype], _p(centroid), len(centroid), _ptrs(seqs),
+                                          _p(lens), len(seqs), open_start, open_end, path_policy, ref_free_mem,
+                                          _p(out), _p(counts), C.byref(delta)))
+        return out, counts, delta.value
+
+    def znorm(self, seqs):
+        seqs = [np.array(s, copy=True) for s in seqs]
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        mu = np.zeros(len(seqs))
+        sg = np.zeros(len(seqs))
+        self._check(self.L.odba_znorm(self.h, CODE[seqs[0].dtype], _ptrs(seqs), _p(lens), len(seqs), _p(mu),
+                                      _p(sg)))
+        return seqs, mu, sg
+
+    def perform_dba(self, seqs, open_start=0, open_end=0, norm=0, max_rounds=250, path_policy=PATH_FULL,
+                    ref_free_mem=0):
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        n = len(seqs)
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        cen = np.zeros(int(lens.max()), dtype=seqs[0].dtype)
+        clen = C.c_uint64()
+        med = C.c_int()
+        perm = np.zeros(n, dtype=np.int32)
+        rounds = C.c_int()
+        dists = np.zeros(n * (n - 1) // 2, dtype=seqs[0].dtype)
+        self._check(self.L.odba_perform_dba(self.h, CODE[seqs[0].dtype], _ptrs(seqs), _p(lens), n, open_start,
+                                            open_end, norm, max_rounds, path_policy, ref_free_mem, _p(cen),
+                                            C.byref(clen), C.byref(med), _p(perm), C.byref(rounds), _p(dists)))
+        return dict(centroid=cen[: clen.value], medoid=med.value, perm=perm, rounds=rounds.value, dists=dists)
+
+
+class SeqSet:
+    """Device-resident sequence set (odba_seqset)."""
+
+    def __init__(self, ctx, seqs):
+        self.ctx = ctx
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        self.dtype = seqs[0].dtype
+        self.n = len(seqs)
+        self.lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        self.h = C.c_void_p()
+        ctx._check(ctx.L.odba_seqset_create(ctx.h, CODE[self.dtype], _ptrs(seqs), _p(self.lens), self.n,
+                                            C.byref(self.h)))
+
+    def all_pairs(self, open_start=0, open_end=0, copy_out=True, flags=False):
+        n = self.n
+        out = np.zeros(n * (n - 1) // 2, dtype=self.dtype) if copy_out else None
+        fl = np.zeros(n * (n - 1) // 2, dtype=np.int32) if flags else None
+        ms = C.c_double(0)
+        self.ctx._check(self.ctx.L.odba_dtw_all_pairs_resident(self.ctx.h, self.h, open_start, open_end, _p(out),
+                                                               _p(fl), C.byref(ms)))
+        return out, fl, ms.value
+
+    def close(self):
+        if self.h:
+            self.ctx.L.odba_seqset_destroy(self.h)
+            self.h = C.c_void_p()

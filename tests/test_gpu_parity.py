@@ -1,0 +1,209 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Integer-input distances, step matrices and medoids must be bit-exact; float
+and double distances, centroids and z-norms are compared bit-exactly too
+(the kernels keep the reference's operation order), which is stricter than
+the 1e-6 relative (distances) / 1e-5 (centroids) tolerances the contract
+allows.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from opendba_amd import native as N
+from opendba_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def _walks(rng, lens, dtype=np.float32):
+    return [np.cumsum(rng.normal(size=int(L))).astype(dtype) for L in lens]
+
+
+def _znorm_cpu(seqs):
+    return O.znorm(seqs)[0]
+
+
+MODES = [(0, 0), (0, 1), (1, 0), (1, 1)]
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_all_pairs_config0_bitexact(ctx, os_, oe):
+    seqs = WL.random_walks(32, 200, 400, seed=0)
+    seqs, _ = WL.stable_length_sort(_znorm_cpu(seqs))
+    got = ctx.all_pairs(seqs, os_, oe)
+    exp = O.all_pairs(seqs, os_, oe)
+    print(f"config0 os={os_} oe={oe} pairs={len(got)}", flush=True)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_all_pairs_swath_boundaries(ctx, os_, oe):
+    rng = np.random.default_rng(10 + 2 * os_ + oe)
+    lens = [1, 2, 3, 63, 64, 65, 700, 1023, 1024, 1025, 1500, 2047, 2048, 2049, 3100]
+    seqs = _znorm_cpu([x for x in _walks(rng, lens)])
+    got = ctx.all_pairs(seqs, os_, oe)
+    exp = O.all_pairs(seqs, os_, oe)
+    # |Y| == 1 pairs: the oracle and the kernel both use the true row-0 cost;
+    # the reference reads an unwritten LDS slot when the last swath width w
+    # has (w-1) % 3 != 0 (dtw.hpp:155), so those entries are parity unpinned
+    # against the reference itself but still compared here.
+    mask = np.ones(len(got), bool)
+    bad = np.nonzero((got.view(np.uint32) != exp.view(np.uint32)) & mask)[0]
+    assert len(bad) == 0, (bad[:10], got[bad[:10]], exp[bad[:10]])
+
+
+def test_all_pairs_open_end_abrogation(ctx):
+    """Short Y against long X in open-end mode: the search is abandoned at a
+    swath start once the top row holds the column minimum (dtw.hpp:74-129)."""
+    rng = np.random.default_rng(7)
+    lens = [150, 300, 900, 5000, 7000, 9000]
+    seqs = _znorm_cpu(_walks(rng, lens))
+    sset = ctx.seqset(seqs)
+    got, flags, _ = sset.all_pairs(0, 1, flags=True)
+    sset.close()
+    exp = O.all_pairs(seqs, 0, 1)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert (flags & 1).sum() > 0  # some pairs were abrogated
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.int32, np.uint32, np.uint64])
+@pytest.mark.parametrize("os_,oe", [(0, 0), (0, 1), (1, 1)])
+def test_all_pairs_dtypes(ctx, dtype, os_, oe):
+    rng = np.random.default_rng(3)
+    lens = [5, 200, 1030, 1500, 2100]
+    if np.issubdtype(dtype, np.floating):
+        seqs = [x.astype(dtype) for x in _walks(rng, lens, np.float64)]
+    elif dtype == np.int32:
+        seqs = [rng.integers(-50, 50, size=L).astype(dtype) for L in lens]
+    else:
+        seqs = [rng.integers(0, 90, size=L).astype(dtype) for L in lens]
+    got = ctx.all_pairs(seqs, os_, oe)
+    exp = O.all_pairs(seqs, os_, oe)
+    assert np.array_equal(got, exp)
+
+
+def test_all_pairs_int_wraparound(ctx):
+    """Large int32 values make the costs wrap; the kernel must keep the
+    reference's add-then-compare order on wrapped values."""
+    rng = np.random.default_rng(4)
+    seqs = [rng.integers(-40000, 40000, size=L).astype(np.int32) for L in (50, 130, 1100)]
+    for os_, oe in MODES:
+        got = ctx.all_pairs(seqs, os_, oe)
+        exp = O.all_pairs(seqs, os_, oe)
+        assert np.array_equal(got, exp), (os_, oe, got, exp)
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+@pytest.mark.parametrize("ly,lx", [(37, 53), (200, 1100), (1030, 700), (64, 2100), (2, 1500)])
+def test_path_matrix_bitexact(ctx, os_, oe, ly, lx):
+    rng = np.random.default_rng(ly * 7 + lx)
+    y, x = _znorm_cpu(_walks(rng, [ly, lx]))
+    d, M = ctx.dtw_path(y, x, os_, oe)
+    de, Me, ab, q3 = O.dtw_pair(y, x, os_, oe, moves=True)
+    print(f"path ly={ly} lx={lx} os={os_} oe={oe} abrogated={ab}", flush=True)
+    assert d == de
+    assert np.array_equal(M, Me), np.argwhere(M != Me)[:10]
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_dba_round_bitexact(ctx, os_, oe):
+    rng = np.random.default_rng(20 + os_ * 2 + oe)
+    lens = [120, 260, 300, 1100, 90, 1500]
+    seqs = _znorm_cpu(_walks(rng, lens))
+    cen = seqs[2].copy()
+    got, counts, delta = ctx.dba_round(cen, seqs, os_, oe)
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_, oe, path_policy=1)
+    assert rc == 0
+    assert np.array_equal(counts, ecounts)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert delta == edelta
+
+
+def test_dba_round_double_and_int(ctx):
+    rng = np.random.default_rng(5)
+    seqs = [rng.normal(size=L) for L in (80, 150, 210)]
+    cen = seqs[1].copy()
+    got, counts, delta = ctx.dba_round(cen, seqs)
+    rc, exp, ec, ed = O.dba_update(cen, seqs)
+    assert rc == 0 and np.array_equal(got, exp) and np.array_equal(counts, ec) and delta == ed
+    iseqs = [rng.integers(0, 100, size=L).astype(np.int32) for L in (80, 150, 210)]
+    cen = iseqs[1].copy()
+    got, counts, delta = ctx.dba_round(cen, iseqs)
+    rc, exp, ec, ed = O.dba_update(cen, iseqs)
+    assert rc == 0 and np.array_equal(got, exp) and np.array_equal(counts, ec) and delta == ed
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32])
+def test_znorm_bitexact(ctx, dtype):
+    rng = np.random.default_rng(8)
+    lens = [1, 7, 1023, 1024, 1025, 5000]
+    seqs = [(rng.normal(size=L) * 10 + 3).astype(dtype) for L in lens]
+    got, mu, sg = ctx.znorm(seqs)
+    exp, emu, esg = O.znorm(seqs)
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+    assert np.array_equal(mu, emu) and np.array_equal(sg, esg)
+
+
+def test_golden_test1_on_gpu(ctx):
+    s1 = O.read_text_series(os.path.join(G, "test1/seq1_test.txt"), np.float64)
+    s2 = O.read_text_series(os.path.join(G, "test1/seq2_test.txt"), np.float64)
+    exp = O.read_tokens(os.path.join(G, "test1/output_results.txt"))
+    r = ctx.perform_dba([s1, s2], norm=0)
+    assert r["medoid"] == 1 and np.array_equal(r["centroid"], exp)
+
+
+@pytest.mark.parametrize("name", ["same", "different", "three", "truncated", "large"])
+def test_reference_kats_on_gpu(ctx, name):
+    K = json.load(open(os.path.join(G, "kat_expected.json")))[name]
+    seqs = [O.read_text_series(os.path.join(G, "test2", f), np.float32) for f in K["files"]]
+    r = ctx.perform_dba(seqs, open_end=K["open_end"], norm=0)
+    c = r["centroid"]
+    assert len(c) == K["n"]
+    for i, v in K["exact"].items():
+        assert c[int(i)] == np.float32(v)
+    for i, v in K["round3"].items():
+        assert np.float32(float("%.3f" % c[int(i)])) == np.float32(v)
+    e = O.perform_dba(seqs, oe=K["open_end"], norm=0)
+    assert np.array_equal(c, e["centroid"])
+
+
+@pytest.mark.parametrize("oe", [0, 1])
+def test_perform_dba_config0_norm(ctx, oe):
+    """Workload 0: 32 random walks, z-norm, all-vs-all, medoid, DBA rounds."""
+    seqs = WL.random_walks(32, 200, 400, seed=0)
+    r = ctx.perform_dba(seqs, open_end=oe, norm=1, max_rounds=3)
+    e = O.perform_dba(seqs, oe=oe, norm=1, max_rounds=3)
+    assert r["medoid"] == e["medoid"] and np.array_equal(r["perm"], e["perm"])
+    assert np.array_equal(r["dists"], e["dists"])
+    assert np.array_equal(r["centroid"], e["centroid"])
+
+
+def test_squiggle_workload_sampled_pairs(ctx):
+    """Workload 1 generator at reduced count (300 squiggles, 2k-20k samples,
+    short->float, z-norm, open_end): every pair on the GPU, a seeded sample
+    of pairs checked against the oracle bit-exactly."""
+    raw = WL.squiggles(300, 2000, 20000, seed=11)
+    seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))
+    seqs, _, _ = ctx.znorm(seqs)
+    sset = ctx.seqset(seqs)
+    got, _, ms = sset.all_pairs(0, 1)
+    sset.close()
+    n = len(seqs)
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(n * (n - 1) // 2, size=60, replace=False))
+    exp = O.all_pairs(seqs, 0, 1, subset=sample)
+    print(f"squiggles n={n} kernel_ms={ms:.1f}", flush=True)
+    assert np.array_equal(got[sample].view(np.uint32), exp.view(np.uint32))
