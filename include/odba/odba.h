@@ -25,6 +25,7 @@ extern "C" {
 
 typedef struct odba_ctx odba_ctx;
 typedef struct odba_seqset odba_seqset;
+typedef struct odba_dba odba_dba;
 
 /* Context on HIP device `device` (replaces the implicit cudaSetDevice/stream
  * pairs of dba.hpp:54-63 and the device queries of cuda_utils.hpp:42-57). */
@@ -88,6 +89,20 @@ int odba_dba_round(odba_ctx* ctx, int dtype, const void* centroid, uint64_t lc, 
                    uint64_t ref_free_mem, void* new_centroid, uint32_t* counts_out, double* max_delta);
 
 /*
+ * Device-resident DBA session for the round loop of performDBA
+ * (dba.hpp:1084-1110): the cluster's sequences are uploaded once;
+ * odba_dba_set_centroid loads a centroid (the medoid or a checkpoint),
+ * odba_dba_step runs one DBAUpdate (as odba_dba_round) and keeps the new
+ * centroid on the device for the next step; new_centroid / counts_out are
+ * optional host copies.
+ */
+int odba_dba_create(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                    int use_open_start, int use_open_end, int path_policy, uint64_t ref_free_mem, odba_dba** out);
+int odba_dba_destroy(odba_dba* dba);
+int odba_dba_set_centroid(odba_dba* dba, const void* centroid, uint64_t lc);
+int odba_dba_step(odba_dba* dba, void* new_centroid, uint32_t* counts_out, double* max_delta);
+
+/*
  * Z-normalise sequences in place (normalizeSequences, gpu_utils.hpp:131-170):
  * population sigma, sigma == 0 treated as 1; mu / sigma (optional) receive
  * each sequence's mean and standard deviation before that substitution.
@@ -107,6 +122,24 @@ int odba_perform_dba(odba_ctx* ctx, int dtype, const void* const* seqs, const ui
                      int use_open_start, int use_open_end, int norm, int max_rounds, int path_policy,
                      uint64_t ref_free_mem, void* centroid_out, uint64_t* centroid_len, int* medoid_sorted,
                      int* perm_out, int* rounds_out, void* dists_out);
+
+/*
+ * Adaptive k-segmentation (adaptive_segmentation<T>, segmentation.hpp:339-583;
+ * kernel adaptive_device_segmentation<T>, segmentation.hpp:48-337): each
+ * sequence is cut into chunks of 256*ceil(minseg/3) samples, every chunk is
+ * segmented by the Bellman DP with the largest non-noisy segment count, and
+ * the segment medians, after the reference's epsilon merge of chunk borders
+ * and the prefix skip, are written to out (concatenated; out_offs / out_lens
+ * per sequence). lds_bytes is the shared-memory budget the reference sizes
+ * its tables with (the CUDA device's opt-in maximum per block, 48 KiB when
+ * unknown); it sets the longest segment the DP considers.
+ * out_capacity (elements) must be at least odba_kseg_capacity(...).
+ */
+int odba_kseg(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n, int min_segment_length,
+              int prefix_skip, int lds_bytes, void* out, uint64_t out_capacity, uint64_t* out_lens,
+              uint64_t* out_offs);
+/* Upper bound on the total number of segments odba_kseg can produce. */
+uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length);
 
 #ifdef __cplusplus
 }

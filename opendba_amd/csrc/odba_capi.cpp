@@ -397,10 +397,31 @@ template <typename T> struct DbaState {
   int* err = nullptr;
   unsigned long long* dbits = nullptr;
   long long budget_words = 0;    // per-batch move-word budget
+  int policy = ODBA_PATH_FULL;
+  uint64_t free_mem = 0;
+  struct StripeJobH {             // layout of StripeJob<T> in the kernels
+    const T* y;
+    const T* x;
+    int H, LX;
+    long long slot_off, mv_off, top_off;
+  };
+  T* slots = nullptr;
+  size_t slots_cap = 0;
+  uint32_t* smv = nullptr;
+  size_t smv_cap = 0;
+  uint32_t* stop = nullptr;
+  size_t stop_cap = 0;
+  StripeState* sstate = nullptr;
+  size_t sstate_cap = 0;
+  StripeJobH* sjobs = nullptr;
+  size_t sjobs_cap = 0;
+  TbJob* stbj = nullptr;
+  size_t stbj_cap = 0;
 
   ~DbaState() {
     for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
-                    (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits})
+                    (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
+                    (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj})
       if (p) hipFree(p);
   }
 
@@ -472,9 +493,73 @@ template <typename T> struct DbaState {
   }
 
   // One DBA round: new centroid in newc, counts, delta. Swaps cen/newc.
+  // Stripe or full path matrix for sequence b, by the reference's rule
+  // (dba.hpp:523-548): stripe when the stated free memory is below the
+  // 1-byte-per-cell move matrix plus 5% and the edge buffer.
+  bool use_stripe(int b) const {
+    if (policy == ODBA_PATH_STRIPE) return true;
+    if (policy != ODBA_PATH_AUTO) return false;
+    const uint64_t ls = (uint64_t)h_lens[b];
+    const bool flip = oe && (uint64_t)lc < ls;
+    const double need = (double)((flip ? (uint64_t)lc : ls) * sizeof(T)) + (double)ls * (double)lc * 1.05;
+    return (double)free_mem < need;
+  }
+
+  int run_stripe_batch(const std::vector<int>& idx, int b0) {
+    const int nj = (int)idx.size();
+    if (!nj) return ODBA_OK;
+    std::vector<StripeJobH> sj(nj);
+    std::vector<TbJob> tb(nj);
+    long long slot_elems = 0, words = 0;
+    int maxK = 0;
+    for (int q = 0; q < nj; q++) {
+      const int b = idx[q];
+      const int ls = h_lens[b];
+      const int flip = (oe && lc < ls) ? 1 : 0;
+      const int H = flip ? lc : ls, LX = flip ? ls : lc;
+      const int K = (LX + ODBA_W - 1) / ODBA_W;
+      const T* sq = seqdata + h_offs[b];
+      sj[q].y = flip ? cen : sq;
+      sj[q].x = flip ? sq : cen;
+      sj[q].H = H;
+      sj[q].LX = LX;
+      sj[q].slot_off = slot_elems;
+      sj[q].mv_off = words;
+      sj[q].top_off = (long long)q * 64;
+      tb[q].ly = H; tb[q].lx = LX; tb[q].flip = flip; tb[q].seq = b - b0; tb[q].mv_off = words; tb[q].top_off = (long long)q * 64;
+      slot_elems += (long long)K * (H + 256);
+      words += std::max(1LL, swath_words(H, os, oe));
+      maxK = std::max(maxK, K);
+    }
+    int rc = grow(c, slots, slots_cap, (size_t)std::max(1LL, slot_elems));
+    if (rc) return rc;
+    rc = grow(c, smv, smv_cap, (size_t)words);
+    if (rc) return rc;
+    rc = grow(c, stop, stop_cap, (size_t)nj * 64);
+    if (rc) return rc;
+    rc = grow(c, sstate, sstate_cap, (size_t)nj);
+    if (rc) return rc;
+    rc = grow(c, sjobs, sjobs_cap, (size_t)nj);
+    if (rc) return rc;
+    rc = grow(c, stbj, stbj_cap, (size_t)nj);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(sjobs, sj.data(), nj * sizeof(StripeJobH), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(stbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+    unsigned int* counter = (unsigned int*)((char*)c->small + 128);
+    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned int), c->stream));
+    HIPC(c, launch_stripe(dtype_of<T>(), minfirst, true, os, oe, grid_blocks(c, nj), c->stream, sjobs, nj, 0, counter,
+                          slots, smv, stop, sstate));
+    const int steps_blocks = (nj + 3) / 4;
+    for (int m = 0; m < maxK; m++) {
+      HIPC(c, launch_stripe(dtype_of<T>(), minfirst, false, os, oe, steps_blocks, c->stream, sjobs, nj, m, counter,
+                            slots, smv, stop, sstate));
+      HIPC(c, launch_stripe_traceback(c->stream, stbj, nj, m, smv, stop, sstate, runs, lc, os, oe, err));
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return ODBA_OK;
+  }
+
   int round(double* delta) {
-    const int W = ODBA_W;
-    (void)W;
     HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPC(c, hipMemsetAsync(dbits, 0, sizeof(unsigned long long), c->stream));
     int rc = grow(c, runs, runs_cap, (size_t)n * lc);
@@ -485,37 +570,49 @@ template <typename T> struct DbaState {
     while (b0 < n) {
       std::vector<PJ<T>> pj;
       std::vector<TbJob> tb;
-      long long mvw = 0, tw = 0;
+      std::vector<int> stripe_idx;
+      long long mvw = 0, tw = 0, sbytes = 0;
       int maxly = 1;
       int b1 = b0;
       while (b1 < n) {
         const int ls = h_lens[b1];
         const int flip = (oe && lc < ls) ? 1 : 0;
         const int ly = flip ? lc : ls, lx = flip ? ls : lc;
-        long long need = (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
-        if (b1 > b0 && mvw + need > budget_words) break;
-        long long mo, to;
-        plan_job(ly, lx, os, oe, mvw, tw, mo, to);
-        const T* sq = seqdata + h_offs[b1];
-        PJ<T> j{flip ? cen : sq, flip ? sq : cen, ly, lx, mo, to};
-        pj.push_back(j);
-        TbJob t;
-        t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
-        tb.push_back(t);
-        maxly = std::max(maxly, ly);
+        const bool st = use_stripe(b1);
+        const long long need = st ? ((long long)((lx + ODBA_W - 1) / ODBA_W) * (ly + 256) * (long long)sizeof(T) + swath_words(ly, os, oe) * 4) / 4
+                                  : (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        if (b1 > b0 && mvw + sbytes + need > budget_words) break;
+        if (st) {
+          stripe_idx.push_back(b1);
+          sbytes += need;
+        } else {
+          long long mo, to;
+          plan_job(ly, lx, os, oe, mvw, tw, mo, to);
+          const T* sq = seqdata + h_offs[b1];
+          PJ<T> j{flip ? cen : sq, flip ? sq : cen, ly, lx, mo, to};
+          pj.push_back(j);
+          TbJob t;
+          t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
+          tb.push_back(t);
+          maxly = std::max(maxly, ly);
+        }
         b1++;
       }
-      const int nb = b1 - b0;
-      rc = grow(c, mv, mv_cap, (size_t)std::max(1LL, mvw));
+      const int nfull = (int)pj.size();
+      if (nfull) {
+        rc = grow(c, mv, mv_cap, (size_t)std::max(1LL, mvw));
+        if (rc) return rc;
+        rc = grow(c, mtop, mtop_cap, (size_t)std::max(1LL, tw));
+        if (rc) return rc;
+        HIPC(c, hipMemcpyAsync(jobs, pj.data(), nfull * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr);
+        if (rc) return rc;
+        HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err));
+      }
+      rc = run_stripe_batch(stripe_idx, b0);
       if (rc) return rc;
-      rc = grow(c, mtop, mtop_cap, (size_t)std::max(1LL, tw));
-      if (rc) return rc;
-      HIPC(c, hipMemcpyAsync(jobs, pj.data(), nb * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
-      HIPC(c, hipMemcpyAsync(tbj, tb.data(), nb * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
-      rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nb, maxly, mv, mtop, abr);
-      if (rc) return rc;
-      HIPC(c, launch_traceback(c->stream, tbj, nb, mv, mtop, abr, runs, lc, os, oe, err));
-      HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, nb, lc, acc, counts, first ? 1 : 0));
+      HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, b1 - b0, lc, acc, counts, first ? 1 : 0));
       // the host vectors must outlive the async copies
       HIPC(c, hipStreamSynchronize(c->stream));
       first = false;
@@ -539,10 +636,12 @@ template <typename T> struct DbaState {
 
 template <typename T>
 int dba_round_t(odba_ctx* c, const T* C, int lc, const T* const* seqs, const uint64_t* lens, int n, int os, int oe,
-                T* newc, uint32_t* counts_out, double* delta) {
+                int policy, uint64_t free_mem, T* newc, uint32_t* counts_out, double* delta) {
   DbaState<T> st;
   int rc = st.init(c, seqs, lens, n, os, oe);
   if (rc) return rc;
+  st.policy = policy;
+  st.free_mem = free_mem;
   rc = st.set_centroid(C, lc);
   if (rc) return rc;
   rc = st.round(delta);
@@ -625,7 +724,7 @@ template <typename T> int medoid_single(const T* d, const std::vector<uint64_t>&
 
 template <typename T>
 int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
-                  int max_rounds, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
+                  int max_rounds, int policy, uint64_t free_mem, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
                   T* dists_out) {
   if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "at least two sequences are needed, got %d", n);
   std::vector<uint64_t> lens(in_lens, in_lens + n);
@@ -655,6 +754,8 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   DbaState<T> st;
   rc = st.init(c, (const T* const*)sp.data(), l.data(), n, os, oe);
   if (rc) return rc;
+  st.policy = policy;
+  st.free_mem = free_mem;
   rc = st.set_centroid(s[med].data(), lc);
   if (rc) return rc;
   std::vector<T> nb(lc), prev(lc, T(0)), prev2(lc, T(0));
@@ -709,16 +810,9 @@ int odba_dba_round(odba_ctx* c, int dtype, const void* centroid, uint64_t lc, co
                    void* new_centroid, uint32_t* counts_out, double* max_delta) {
   if (!c || !centroid || !seqs || !lens || n <= 0 || lc == 0 || !new_centroid) return fail(c, ODBA_E_INVALID, "invalid DBA round arguments");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
-  if (path_policy == ODBA_PATH_STRIPE) return fail(c, ODBA_E_UNSUPPORTED, "stripe mode is not available in this build");
-  if (path_policy == ODBA_PATH_AUTO) {
-    for (int i = 0; i < n; i++) {
-      const bool flip = oe && lc < lens[i];
-      const double need = (double)((flip ? lc : lens[i]) * dtype_size(dtype)) + (double)lens[i] * (double)lc * 1.05;
-      if ((double)ref_free_mem < need) return fail(c, ODBA_E_UNSUPPORTED, "the reference would use stripe mode for sequence %d; stripe mode is not available in this build", i);
-    }
-  }
+  if (path_policy < ODBA_PATH_AUTO || path_policy > ODBA_PATH_STRIPE) return fail(c, ODBA_E_INVALID, "unknown path policy %d", path_policy);
   ODBA_DISPATCH(dtype, dba_round_t<T>(c, (const T*)centroid, (int)lc, (const T* const*)seqs, lens, n, os, oe,
-                                      (T*)new_centroid, counts_out, max_delta))
+                                      path_policy, ref_free_mem, (T*)new_centroid, counts_out, max_delta))
 }
 
 int odba_znorm(odba_ctx* c, int dtype, void* const* seqs, const uint64_t* lens, int n, double* mu, double* sigma) {
@@ -732,10 +826,243 @@ int odba_perform_dba(odba_ctx* c, int dtype, const void* const* seqs, const uint
                      uint64_t* centroid_len, int* medoid_sorted, int* perm_out, int* rounds_out, void* dists_out) {
   if (!c || !seqs || !lens || !centroid_out) return fail(c, ODBA_E_INVALID, "invalid performDBA arguments");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
-  if (path_policy == ODBA_PATH_STRIPE) return fail(c, ODBA_E_UNSUPPORTED, "stripe mode is not available in this build");
-  (void)ref_free_mem;
-  ODBA_DISPATCH(dtype, perform_dba_t<T>(c, (const T* const*)seqs, lens, n, os, oe, norm, max_rounds, (T*)centroid_out,
+  if (path_policy < ODBA_PATH_AUTO || path_policy > ODBA_PATH_STRIPE) return fail(c, ODBA_E_INVALID, "unknown path policy %d", path_policy);
+  ODBA_DISPATCH(dtype, perform_dba_t<T>(c, (const T* const*)seqs, lens, n, os, oe, norm, max_rounds, path_policy,
+                                        ref_free_mem, (T*)centroid_out,
                                         centroid_len, medoid_sorted, perm_out, rounds_out, (T*)dists_out))
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adaptive k-segmentation (adaptive_segmentation<T>, segmentation.hpp:339-583)
+// ---------------------------------------------------------------------------
+namespace {
+
+int kseg_mk(int minseg) {
+  const int ds = (minseg + 2) / 3;
+  const float x = ((float)minseg) / ds;
+  return (int)((256 + x - 1) / x);  // DIV_ROUNDUP with a float divisor (segmentation.hpp:350)
+}
+
+// Host post-pass over one sequence's padded medians (segmentation.hpp:486-578):
+// epsilon = smallest non-zero step between neighbours produced together,
+// equal neighbours collapsed, chunk-boundary neighbours closer than epsilon
+// averaged, unused (max-valued) slots dropped, a trailing half-max value
+// dropped, then the prefix skip.
+template <typename T>
+long kseg_post(T* seg, size_t SL, int prefix_skip, long* skipped_out) {
+  const T MX = std::numeric_limits<T>::max();
+  T eps = MX;
+  T prev = seg[0];
+  for (size_t j = 1; j < SL; ++j) {
+    const T cv = seg[j];
+    if (cv != MX && prev != MX && ((cv <= prev && (T)(prev - cv) < eps) || (cv > prev && (T)(cv - prev) < eps))) {
+      if ((T)(cv - prev) == 0) {
+        for (size_t k = j; k < SL; k++) {
+          seg[k - 1] = seg[k];
+          if (seg[k] == MX) break;
+        }
+      } else {
+        eps = cv < prev ? (T)(prev - cv) : (T)(cv - prev);
+      }
+    }
+    prev = cv;
+  }
+  bool undef = false;
+  long cursor = 0;
+  for (size_t j = 0; j < SL; ++j) {
+    if (seg[j] == MX) { undef = true; continue; }
+    if (undef) {
+      undef = false;
+      if (cursor >= 1 && seg[cursor - 1] < (T)(seg[j] + eps) && seg[cursor - 1] > (T)(seg[j] - eps))
+        seg[cursor - 1] = (T)((seg[cursor - 1] + seg[j]) / 2);
+    }
+    if ((size_t)cursor != j) seg[cursor] = seg[j];
+    cursor++;
+  }
+  if (cursor >= 1 && seg[cursor - 1] >= MX / 2) cursor--;
+  long skipped = 0;
+  if (prefix_skip) {
+    if (cursor <= prefix_skip) { skipped = cursor - 1; cursor = 1; }
+    else { skipped = prefix_skip; cursor -= prefix_skip; }
+  }
+  if (cursor < 0) cursor = 0;
+  *skipped_out = skipped < 0 ? 0 : skipped;
+  return cursor;
+}
+
+template <typename T>
+int kseg_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int n, int minseg, int prefix_skip, int lds_bytes,
+           T* out, uint64_t cap, uint64_t* out_lens, uint64_t* out_offs) {
+  const int ds = (minseg + 2) / 3;
+  const int spb = 256 * ds;
+  const int mk = kseg_mk(minseg);
+  std::vector<long long> offs(n), obase(n), pbase(n);
+  std::vector<int> L(n);
+  long long total = 0, ototal = 0, ptotal = 0;
+  int maxch = 1;
+  for (int i = 0; i < n; i++) {
+    offs[i] = total;
+    L[i] = (int)lens[i];
+    total += padded_len((long long)lens[i]) + spb;
+    const int nch = (int)((lens[i] + spb - 1) / spb);
+    maxch = std::max(maxch, nch);
+    obase[i] = ototal;
+    ototal += (long long)nch * mk;
+    pbase[i] = ptotal;
+    ptotal += (long long)nch * 256 * mk;
+  }
+  std::vector<T> hb((size_t)total, T(0));
+  for (int i = 0; i < n; i++) std::memcpy(&hb[offs[i]], seqs[i], lens[i] * sizeof(T));
+  T *d = nullptr, *dout = nullptr;
+  long long *doffs = nullptr, *dob = nullptr, *dpb = nullptr;
+  int* dl = nullptr;
+  unsigned short* kp = nullptr;
+  HIPC(c, hipMalloc((void**)&d, hb.size() * sizeof(T)));
+  HIPC(c, hipMalloc((void**)&dout, std::max(1LL, ototal) * sizeof(T)));
+  HIPC(c, hipMalloc((void**)&doffs, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&dob, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&dpb, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&dl, n * sizeof(int)));
+  HIPC(c, hipMalloc((void**)&kp, std::max(1LL, ptotal) * sizeof(unsigned short)));
+  HIPC(c, hipMemcpyAsync(d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(doffs, offs.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dob, obase.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dpb, pbase.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dl, L.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const int e = kseg_extent_host(mk, 256, (int)sizeof(T), lds_bytes);
+  if (e < 1) return fail(c, ODBA_E_INVALID, "lds_bytes %d leaves no room for the segmentation tables", lds_bytes);
+  const size_t dyn = std::max((size_t)256 * e * 6, (size_t)spb * sizeof(T));
+  if (dyn > 150 * 1024) return fail(c, ODBA_E_UNSUPPORTED, "k-segmentation needs %zu bytes of LDS", dyn);
+  HIPC(c, launch_kseg(dtype_of<T>(), c->stream, d, doffs, dl, n, maxch, dob, dpb, spb, mk, minseg, lds_bytes, kp, dout, dyn));
+  std::vector<T> hm((size_t)std::max(1LL, ototal));
+  HIPC(c, hipMemcpyAsync(hm.data(), dout, std::max(1LL, ototal) * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  hipFree(d); hipFree(dout); hipFree(doffs); hipFree(dob); hipFree(dpb); hipFree(dl); hipFree(kp);
+  uint64_t cur = 0;
+  for (int i = 0; i < n; i++) {
+    const size_t SL = (size_t)((lens[i] + spb - 1) / spb) * mk;
+    long skipped = 0;
+    const long len = kseg_post<T>(&hm[obase[i]], SL, prefix_skip, &skipped);
+    if (cur + (uint64_t)len > cap) return fail(c, ODBA_E_INVALID, "k-segmentation output capacity %llu too small", (unsigned long long)cap);
+    std::memcpy(out + cur, &hm[obase[i] + skipped], (size_t)len * sizeof(T));
+    out_offs[i] = cur;
+    out_lens[i] = (uint64_t)len;
+    cur += (uint64_t)len;
+  }
+  return ODBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int odba_kseg(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int min_segment_length,
+              int prefix_skip, int lds_bytes, void* out, uint64_t out_capacity, uint64_t* out_lens,
+              uint64_t* out_offs) {
+  if (!c || !seqs || !lens || n <= 0 || !out || !out_lens || !out_offs || min_segment_length <= 0)
+    return fail(c, ODBA_E_INVALID, "invalid k-segmentation arguments");
+  for (int i = 0; i < n; i++)
+    if (lens[i] == 0) return fail(c, ODBA_E_INVALID, "sequence %d is empty", i);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(dtype, kseg_t<T>(c, (const T* const*)seqs, lens, n, min_segment_length, prefix_skip, lds_bytes,
+                                 (T*)out, out_capacity, out_lens, out_offs))
+}
+
+uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length) {
+  if (!lens || n <= 0 || min_segment_length <= 0) return 0;
+  const int ds = (min_segment_length + 2) / 3;
+  const uint64_t spb = 256ull * ds;
+  const uint64_t mk = (uint64_t)kseg_mk(min_segment_length);
+  uint64_t t = 0;
+  for (int i = 0; i < n; i++) t += mk * ((lens[i] + spb - 1) / spb);
+  return t;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-resident DBA session: member sequences and the centroid stay in
+// HBM across rounds (the reference re-reads managed memory every round).
+// ---------------------------------------------------------------------------
+struct odba_dba {
+  odba_ctx* ctx = nullptr;
+  int dtype = 0;
+  void* st = nullptr;  // DbaState<T>*
+};
+
+namespace {
+template <typename T> int dba_create_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int n, int os, int oe,
+                                       int policy, uint64_t free_mem, void** out) {
+  auto* st = new DbaState<T>();
+  int rc = st->init(c, seqs, lens, n, os, oe);
+  if (rc) { delete st; return rc; }
+  st->policy = policy;
+  st->free_mem = free_mem;
+  *out = st;
+  return ODBA_OK;
+}
+template <typename T> int dba_destroy_t(void* p) { delete (DbaState<T>*)p; return ODBA_OK; }
+template <typename T> int dba_set_t(void* p, const T* c, int lc) { return ((DbaState<T>*)p)->set_centroid(c, lc); }
+template <typename T> int dba_step_t(odba_ctx* c, void* p, T* newc, uint32_t* counts, double* delta) {
+  auto* st = (DbaState<T>*)p;
+  int rc = st->round(delta);
+  if (rc) return rc;
+  if (newc) HIPC(c, hipMemcpyAsync(newc, st->cen, st->lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  if (counts) HIPC(c, hipMemcpyAsync(counts, st->counts, st->lc * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ODBA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int odba_dba_create(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int use_open_start,
+                    int use_open_end, int path_policy, uint64_t ref_free_mem, odba_dba** out) {
+  if (!c || !seqs || !lens || n <= 0 || !out) return fail(c, ODBA_E_INVALID, "invalid DBA session arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  auto* d = new odba_dba();
+  d->ctx = c;
+  d->dtype = dtype;
+  int rc;
+  switch (dtype) {
+    case ODBA_I32: rc = dba_create_t<int32_t>(c, (const int32_t* const*)seqs, lens, n, use_open_start, use_open_end, path_policy, ref_free_mem, &d->st); break;
+    case ODBA_U32: rc = dba_create_t<uint32_t>(c, (const uint32_t* const*)seqs, lens, n, use_open_start, use_open_end, path_policy, ref_free_mem, &d->st); break;
+    case ODBA_U64: rc = dba_create_t<uint64_t>(c, (const uint64_t* const*)seqs, lens, n, use_open_start, use_open_end, path_policy, ref_free_mem, &d->st); break;
+    case ODBA_F32: rc = dba_create_t<float>(c, (const float* const*)seqs, lens, n, use_open_start, use_open_end, path_policy, ref_free_mem, &d->st); break;
+    case ODBA_F64: rc = dba_create_t<double>(c, (const double* const*)seqs, lens, n, use_open_start, use_open_end, path_policy, ref_free_mem, &d->st); break;
+    default: rc = fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype);
+  }
+  if (rc) { delete d; return rc; }
+  *out = d;
+  return ODBA_OK;
+}
+
+int odba_dba_destroy(odba_dba* d) {
+  if (!d) return ODBA_OK;
+  switch (d->dtype) {
+    case ODBA_I32: dba_destroy_t<int32_t>(d->st); break;
+    case ODBA_U32: dba_destroy_t<uint32_t>(d->st); break;
+    case ODBA_U64: dba_destroy_t<uint64_t>(d->st); break;
+    case ODBA_F32: dba_destroy_t<float>(d->st); break;
+    case ODBA_F64: dba_destroy_t<double>(d->st); break;
+  }
+  delete d;
+  return ODBA_OK;
+}
+
+int odba_dba_set_centroid(odba_dba* d, const void* centroid, uint64_t lc) {
+  if (!d || !centroid || lc == 0) return ODBA_E_INVALID;
+  odba_ctx* c = d->ctx;
+  ODBA_DISPATCH(d->dtype, dba_set_t<T>(d->st, (const T*)centroid, (int)lc))
+}
+
+int odba_dba_step(odba_dba* d, void* new_centroid, uint32_t* counts_out, double* max_delta) {
+  if (!d) return ODBA_E_INVALID;
+  odba_ctx* c = d->ctx;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(d->dtype, dba_step_t<T>(c, d->st, (T*)new_centroid, counts_out, max_delta))
 }
 
 }  // extern "C"

@@ -106,6 +106,85 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
   R[curc] = make_int2(lo, hi);
 }
 
+// Stripe-mode traceback for stripe k = K-1-m of every job (updateCentroid
+// with column_offset / stripe_rows, dba.hpp:374-423 launched at 733): walk
+// from the carried cell until the path leaves the stripe on the left (the
+// new height is the row reached + 1) or reaches the (0,0) anchor.
+__global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m, const uint32_t* __restrict__ mv,
+                                   const uint32_t* __restrict__ mtop, StripeState* __restrict__ ss,
+                                   int2* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err) {
+  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jid >= njobs) return;
+  const TbJob J = jobs[jid];
+  StripeState S = ss[jid];
+  const int K = (J.lx + ODBA_W - 1) / ODBA_W;
+  const int k = K - 1 - m;
+  if (k < 0 || S.done || S.err) return;
+  const int left = k * ODBA_W;
+  const int h = S.h;
+  const int wl = J.lx - left;
+  const int C = (k == K - 1) ? (wl <= 64 ? 1 : wl <= 128 ? 2 : wl <= 256 ? 4 : wl <= 512 ? 8 : wl <= 768 ? 12 : 16) : 16;
+  const int r0 = os ? 1 : 0;
+  const uint32_t* M = mv + J.mv_off;
+  const uint32_t* TP = mtop + J.top_off;
+  int2* R = runs + (long long)J.seq * lc;
+  int i = S.i, j = S.j;
+  if (S.curc < 0) S.curc = J.flip ? i : j;
+  long long caddr = -1;
+  uint32_t cword = 0;
+  bool done = false;
+  while (j >= left) {
+    uint8_t code;
+    if (i == 0 && j == 0) code = os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
+    else if (S.sab && i == h - 1) code = ODBA_OPEN_RIGHT;
+    else if (i == 0) code = os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
+    else {
+      const int t = j - left;
+      const int lane = t / C, q = t - lane * C;
+      if (oe && h >= 2 && i == h - 1) {
+        const uint32_t c = (TP[lane] >> (2 * q)) & 3u;
+        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
+      } else {
+        const long long addr = ((long long)(i - r0) + lane) * 64 + lane;
+        uint32_t w;
+        if (addr == caddr) w = cword;
+        else { w = M[addr]; caddr = addr; cword = w; }
+        const uint32_t c = (w >> (2 * q)) & 3u;
+        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
+      }
+    }
+    const int c = J.flip ? i : j;
+    const int sidx = J.flip ? j : i;
+    if (c != S.curc) {
+      R[S.curc] = make_int2(S.lo, S.hi);
+      S.curc = c;
+      S.lo = 1;
+      S.hi = 0;
+    }
+    const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
+    const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);
+    if (counted) {
+      if (S.hi < S.lo) { S.hi = sidx; S.lo = sidx; }
+      else S.lo = sidx;
+    }
+    if (corner) { done = true; break; }
+    if (code == ODBA_DIAGONAL) { i--; j--; }
+    else if (code == ODBA_UP) { i--; }
+    else { j--; }
+    if (i < 0) { S.err = 1; atomicOr(err, 1); break; }
+  }
+  if (done) {
+    R[S.curc] = make_int2(S.lo, S.hi);
+    S.done = 1;
+  } else if (!S.err) {
+    if (j != left - 1) { S.err = 1; atomicOr(err, 1); }
+    S.h = i + 1;
+  }
+  S.i = i;
+  S.j = j;
+  ss[jid] = S;
+}
+
 template <typename T>
 __global__ void k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
                              const int2* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
@@ -128,8 +207,16 @@ __global__ void k_accumulate(const T* __restrict__ data, const long long* __rest
 
 // mean = sum / count with the host's T / unsigned arithmetic (dba.hpp:820-822)
 template <typename T> __device__ __forceinline__ T div_count(T s, uint32_t c) { return (T)(s / (T)c); }
-template <> __device__ __forceinline__ float div_count<float>(float s, uint32_t c) { return __fdiv_rn(s, (float)c); }
-template <> __device__ __forceinline__ double div_count<double>(double s, uint32_t c) { return s / (double)c; }
+// A zero count gives 0/0 on the host in the reference; x86 returns its
+// default NaN (sign bit set) for that, which is reproduced here.
+template <> __device__ __forceinline__ float div_count<float>(float s, uint32_t c) {
+  if (c == 0u && s == 0.0f) return __uint_as_float(0xFFC00000u);
+  return __fdiv_rn(s, (float)c);
+}
+template <> __device__ __forceinline__ double div_count<double>(double s, uint32_t c) {
+  if (c == 0u && s == 0.0) return __longlong_as_double((long long)0xFFF8000000000000ull);
+  return s / (double)c;
+}
 template <> __device__ __forceinline__ int32_t div_count<int32_t>(int32_t s, uint32_t c) { return (int32_t)((uint32_t)s / c); }
 template <> __device__ __forceinline__ uint64_t div_count<uint64_t>(uint64_t s, uint32_t c) { return s / (uint64_t)c; }
 
@@ -263,6 +350,15 @@ hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const 
                             const int* abr, int2* runs, int lc, int os, int oe, int* err) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
+                                   const uint32_t* mtop, StripeState* ss, int2* runs, int lc, int os, int oe,
+                                   int* err) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
+                     lc, os, oe, err);
   return hipGetLastError();
 }
 

@@ -48,6 +48,23 @@ struct TbJob {
   long long top_off;   // word offset of the open-end top-row words
 };
 
+// Per-job traceback state of stripe mode, carried across stripes.
+struct StripeState {
+  int i, j;      // current cell (global column)
+  int h;         // current truncated height (rows still in play)
+  int curc, lo, hi;  // open run of the current centroid position
+  int fab;       // swath where the forward pass abandoned the search, -1 none
+  int sab;       // 1: this stripe's top row is all OPEN_RIGHT
+  int done, err;
+};
+
+hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
+                         const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
+                         uint32_t* mtop, StripeState* ss);
+hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
+                                   const uint32_t* mtop, StripeState* ss, int2* runs, int lc, int os, int oe,
+                                   int* err);
+
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
                             const int* abr, int2* runs, int lc, int os, int oe, int* err);
 hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int2* runs,
@@ -58,5 +75,10 @@ hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* 
                         double* mu, double* sigma);
 hipError_t launch_decode_moves(hipStream_t st, const uint32_t* mv, const uint32_t* mtop, int ab, int ly, int lx, int os,
                                int oe, uint8_t* out);
+
+hipError_t launch_kseg(int dtype, hipStream_t st, const void* data, const long long* offs, const int* lens, int n,
+                       int maxch, const long long* out_base, const long long* path_base, int spb, int mk, int minseg,
+                       int lds_bytes, unsigned short* kpath, void* out, size_t dyn);
+int kseg_extent_host(int mk, int N_ds, int tsize, int lds_bytes);
 
 }  // namespace odba

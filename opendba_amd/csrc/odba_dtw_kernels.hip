@@ -137,6 +137,19 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave-uniform copies (SGPR) of values that are equal in every lane. Loop
+// bounds and work indices derived from them are then scalar, so the
+// compiler keeps all 64 lanes of the wave in the same loop iteration
+// around the cross-lane operations (a work index broadcast with a shuffle
+// is not known to be uniform, and a loop exiting on it gets structured
+// per lane).
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffull));
+  unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 template <typename T> __device__ __forceinline__ T wave_min(T v) {
   for (int d = 32; d > 0; d >>= 1) v = Ar<T>::mn(v, __shfl_xor(v, d, 64));
   return v;
@@ -550,16 +563,16 @@ __global__ void __launch_bounds__(256) k_all_pairs(const T* __restrict__ data, c
   __shared__ T s_e[4][64];
   __shared__ T s_o[4][128];
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
   const long long gw = (long long)blockIdx.x * 4 + wid;
   T* e0 = escratch + gw * 2 * escratch_stride;
   T* e1 = e0 + escratch_stride;
   const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
   const WaveWin<T>* W = &win;
   for (;;) {
-    unsigned long long p = 0;
-    if (lane == 0) p = atomicAdd(counter, 1ull);
-    p = __shfl(p, 0, 64);
+    // Every lane takes part in the fetch (only lane 0 adds 1): no divergent
+    // branch around the atomic, so the loop stays one wave-uniform loop.
+    const unsigned long long p = uniform_u64(atomicAdd(counter, lane == 0 ? 1ull : 0ull));
     if ((long long)p >= npairs) break;
     int r, c;
     decode_pair((long long)p, npairs, r, c);
@@ -595,16 +608,14 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
   __shared__ T s_e[4][64];
   __shared__ T s_o[4][128];
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
   const long long gw = (long long)blockIdx.x * 4 + wid;
   T* e0 = escratch + gw * 2 * escratch_stride;
   T* e1 = e0 + escratch_stride;
   const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
   const WaveWin<T>* W = &win;
   for (;;) {
-    unsigned int j = 0;
-    if (lane == 0) j = atomicAdd(counter, 1u);
-    j = __shfl(j, 0, 64);
+    const unsigned int j = (unsigned int)uniform_i32((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
     if ((int)j >= njobs) break;
     PathJob<T> jb = jobs[j];
     PairOut po;
@@ -612,6 +623,114 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
                                                   W, lane, po);
     if (lane == 0) abrogated[j] = po.abrogated;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Stripe mode (dba.hpp:523-548, 646-763). Forward: every swath's right edge
+// column goes to its own slot (slots + k*H), no step codes. Backward: the
+// host steps m = 0, 1, ...; job j recomputes its stripe k = K_j-1-m with the
+// height cut to the current traceback row (state.h), exactly as the
+// reference relaunches DTWDistance with first_seq_length = backtrace rows.
+// ---------------------------------------------------------------------------
+template <typename T> struct StripeJob {
+  const T* y;          // Y (rows), padded
+  const T* x;          // X (columns), padded
+  int H, LX;           // full height, width
+  long long slot_off;  // element offset of this job's edge slots (stride H + 256)
+  long long mv_off;    // word offset of this job's stripe step words
+  long long top_off;   // word offset of this job's 64 top-row words
+};
+
+template <typename T, bool OS, bool OE, bool MINFIRST>
+__global__ void __launch_bounds__(256) k_stripe_forward(const StripeJob<T>* __restrict__ jobs, int njobs,
+                                                        unsigned int* __restrict__ counter, T* __restrict__ slots,
+                                                        StripeState* __restrict__ st) {
+  __shared__ T s_y[4][128];
+  __shared__ T s_e[4][64];
+  __shared__ T s_o[4][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  for (;;) {
+    const unsigned int j = (unsigned int)uniform_i32((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
+    if ((int)j >= njobs) break;
+    const StripeJob<T> jb = jobs[j];
+    T* sl = slots + jb.slot_off;
+    const int K = (jb.LX + ODBA_W - 1) / ODBA_W;
+    int ab = -1;
+    T colmin = T(0), top = T(0);
+    for (int k = 0; k < K - 1; k++) {  // the last swath's edge is never read back
+      const int off = k * ODBA_W;
+      if (k > 0 && OE && !OS && off > jb.H && top == colmin) { ab = k; break; }
+      SwathIO<T> io;
+      io.y = jb.y; io.ly = jb.H; io.x = jb.x; io.lx = jb.LX; io.off = off;
+      io.ein = k > 0 ? sl + (long long)(k - 1) * (jb.H + 256) : nullptr;
+      io.eout = sl + (long long)k * (jb.H + 256);
+      io.mv = nullptr; io.mtop = nullptr;
+      SwathRes<T> r = run_swath<T, 16, OS, OE, MINFIRST, false>(io, &win, lane);
+      colmin = r.colmin; top = r.top;
+    }
+    if (ab < 0 && K > 1) {
+      const int off = (K - 1) * ODBA_W;
+      if (OE && !OS && off > jb.H && top == colmin) ab = K - 1;
+    }
+    if (lane == 0) {
+      StripeState z;
+      z.i = jb.H - 1; z.j = jb.LX - 1; z.h = jb.H; z.curc = -1; z.lo = 1; z.hi = 0;
+      z.fab = ab; z.sab = 0; z.done = 0; z.err = 0;
+      st[j] = z;
+    }
+  }
+}
+
+// One stripe per job: recompute with the truncated height (or mark the
+// stripe open-right when the reference's abrogation test fires at that
+// height), step codes into the job's stripe buffer.
+template <typename T, bool OS, bool OE, bool MINFIRST>
+__global__ void __launch_bounds__(256) k_stripe_step(const StripeJob<T>* __restrict__ jobs, int njobs, int m,
+                                                     const T* __restrict__ slots, uint32_t* __restrict__ mv,
+                                                     uint32_t* __restrict__ mtop, StripeState* __restrict__ st) {
+  __shared__ T s_y[4][128];
+  __shared__ T s_e[4][64];
+  __shared__ T s_o[4][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int j = blockIdx.x * 4 + wid;
+  if (j >= njobs) return;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  const StripeJob<T> jb = jobs[j];
+  StripeState S = st[j];
+  const int K = (jb.LX + ODBA_W - 1) / ODBA_W;
+  const int k = K - 1 - m;
+  if (k < 0 || S.done || S.err) return;
+  const int h = S.h;
+  if (S.fab >= 0 && k >= S.fab) {  // already abandoned in the forward pass
+    if (lane == 0) st[j].sab = 1;
+    return;
+  }
+  const int off = k * ODBA_W;
+  const T* prev = k > 0 ? slots + jb.slot_off + (long long)(k - 1) * (jb.H + 256) : nullptr;
+  int sab = 0;
+  if (OE && !OS && off > h) {  // dtw.hpp:74-129 at the truncated height
+    const T MAXV = Ar<T>::maxv();
+    if (prev[0] == MAXV) sab = 1;
+    else {
+      T mn = MAXV;
+      for (int r = lane; r < h; r += 64) mn = Ar<T>::mn(mn, prev[r]);
+      mn = wave_min<T>(mn);
+      if (prev[h - 1] == mn) sab = 1;
+    }
+  }
+  if (!sab) {
+    SwathIO<T> io;
+    io.y = jb.y; io.ly = h; io.x = jb.x; io.lx = jb.LX; io.off = off;
+    io.ein = prev; io.eout = nullptr;
+    io.mv = mv + jb.mv_off; io.mtop = mtop + jb.top_off;
+    const int wcols = (k == K - 1) ? jb.LX - off : ODBA_W;
+    const int C = (k == K - 1) ? last_swath_c(wcols) : 16;
+    (void)swath_dispatch<T, OS, OE, MINFIRST, true>(C, io, &win, lane);
+  }
+  if (lane == 0) st[j].sab = sab;
 }
 
 // ---------------------------------------------------------------------------
@@ -670,6 +789,37 @@ hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblock
     case ODBA_I32: return launch_path_jobs_t<int32_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (int32_t*)esc, stride);
     case ODBA_U32: return launch_path_jobs_t<uint32_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (uint32_t*)esc, stride);
     case ODBA_U64: return launch_path_jobs_t<uint64_t, false>(os, oe, g, st, jobs, njobs, counter, mv, mtop, abr, (uint64_t*)esc, stride);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T, bool MINFIRST>
+static hipError_t launch_stripe_t(bool forward, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                                  int m, unsigned int* counter, void* slots, uint32_t* mv, uint32_t* mtop,
+                                  StripeState* ss) {
+  const StripeJob<T>* J = (const StripeJob<T>*)jobs;
+  dim3 g(nblocks), b(256);
+#define ODBA_SFWD(A, B) hipLaunchKernelGGL((k_stripe_forward<T, A, B, MINFIRST>), g, b, 0, st, J, njobs, counter, (T*)slots, ss)
+#define ODBA_SSTEP(A, B) hipLaunchKernelGGL((k_stripe_step<T, A, B, MINFIRST>), g, b, 0, st, J, njobs, m, (const T*)slots, mv, mtop, ss)
+  if (forward) {
+    if (os && oe) ODBA_SFWD(true, true); else if (os) ODBA_SFWD(true, false); else if (oe) ODBA_SFWD(false, true); else ODBA_SFWD(false, false);
+  } else {
+    if (os && oe) ODBA_SSTEP(true, true); else if (os) ODBA_SSTEP(true, false); else if (oe) ODBA_SSTEP(false, true); else ODBA_SSTEP(false, false);
+  }
+#undef ODBA_SFWD
+#undef ODBA_SSTEP
+  return hipGetLastError();
+}
+
+hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
+                         const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
+                         uint32_t* mtop, StripeState* ss) {
+  switch (dtype) {
+    case ODBA_F32: return launch_stripe_t<float, true>(forward, os, oe, nblocks, st, jobs, njobs, m, counter, slots, mv, mtop, ss);
+    case ODBA_F64: return launch_stripe_t<double, true>(forward, os, oe, nblocks, st, jobs, njobs, m, counter, slots, mv, mtop, ss);
+    case ODBA_I32: return launch_stripe_t<int32_t, false>(forward, os, oe, nblocks, st, jobs, njobs, m, counter, slots, mv, mtop, ss);
+    case ODBA_U32: return launch_stripe_t<uint32_t, false>(forward, os, oe, nblocks, st, jobs, njobs, m, counter, slots, mv, mtop, ss);
+    case ODBA_U64: return launch_stripe_t<uint64_t, false>(forward, os, oe, nblocks, st, jobs, njobs, m, counter, slots, mv, mtop, ss);
   }
   return hipErrorInvalidValue;
 }

@@ -20,7 +20,8 @@ PATH_AUTO, PATH_FULL, PATH_STRIPE = 0, 1, 2
 EXPORTS = [
     "odba_ctx_create", "odba_ctx_destroy", "odba_last_error", "odba_device_info",
     "odba_dtw_all_pairs", "odba_seqset_create", "odba_seqset_destroy", "odba_dtw_all_pairs_resident",
-    "odba_dtw_path", "odba_dba_round", "odba_znorm", "odba_perform_dba",
+    "odba_dtw_path", "odba_dba_round", "odba_znorm", "odba_perform_dba", "odba_kseg", "odba_kseg_capacity",
+    "odba_dba_create", "odba_dba_destroy", "odba_dba_set_centroid", "odba_dba_step",
 ]
 
 
@@ -55,6 +56,9 @@ def load():
     L.odba_znorm.argtypes = [vp, i32, vp, vp, i32, vp, vp]
     L.odba_perform_dba.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, i32, u64, vp, C.POINTER(u64),
                                    C.POINTER(i32), vp, C.POINTER(i32), vp]
+    L.odba_kseg.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, vp, u64, vp, vp]
+    L.odba_kseg_capacity.argtypes = [vp, i32, i32]
+    L.odba_kseg_capacity.restype = u64
     _lib = L
     return L
 
@@ -156,6 +160,21 @@ class Context:
                                             open_end, norm, max_rounds, path_policy, ref_free_mem, _p(cen),
                                             C.byref(clen), C.byref(med), _p(perm), C.byref(rounds), _p(dists)))
         return dict(centroid=cen[: clen.value], medoid=med.value, perm=perm, rounds=rounds.value, dists=dists)
+
+
+def _kseg(ctx, seqs, min_segment_length, prefix_skip=0, lds_bytes=65536):
+    seqs = [np.ascontiguousarray(s) for s in seqs]
+    lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+    cap = int(ctx.L.odba_kseg_capacity(_p(lens), len(seqs), min_segment_length))
+    out = np.zeros(max(cap, 1), dtype=seqs[0].dtype)
+    ol = np.zeros(len(seqs), dtype=np.uint64)
+    oo = np.zeros(len(seqs), dtype=np.uint64)
+    ctx._check(ctx.L.odba_kseg(ctx.h, CODE[seqs[0].dtype], _ptrs(seqs), _p(lens), len(seqs), min_segment_length,
+                               prefix_skip, lds_bytes, _p(out), cap, _p(ol), _p(oo)))
+    return [out[int(oo[i]):int(oo[i]) + int(ol[i])].copy() for i in range(len(seqs))]
+
+
+Context.kseg = _kseg
 
 
 class SeqSet:

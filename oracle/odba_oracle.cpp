@@ -604,6 +604,245 @@ int perform_dba(T* const* in_seqs, const uint64_t* in_lens, size_t n, int os, in
   return ODBA_OK;
 }
 
+/*
+ * Adaptive k-segmentation (adaptive_device_segmentation<T>,
+ * segmentation.hpp:48-337, one call per (sequence, chunk) block, and the
+ * host post-pass of adaptive_segmentation, segmentation.hpp:486-578).
+ * Thread-level steps are restated serially; the warp reductions are
+ * simulated lane by lane (32-lane shuffle-down trees) because NaN values
+ * produced by the downsample bound (segmentation.hpp:114) make their order
+ * visible. lds_bytes is the dynamic shared memory size the reference asks
+ * for (the device's opt-in maximum, segmentation.hpp:458-462): it fixes the
+ * maximum segment extent e. Float -> unsigned char uses the GPU conversion
+ * rule (round toward zero, saturate, NaN -> 0); inputs that reach it with
+ * NaN or out of range are parity unpinned.
+ */
+template <typename T> T lim_min();
+template <> int32_t lim_min<int32_t>() { return INT_MIN; }
+template <> uint32_t lim_min<uint32_t>() { return 0; }
+template <> uint64_t lim_min<uint64_t>() { return 0; }
+template <> float lim_min<float>() { return FLT_MIN; }
+template <> double lim_min<double>() { return DBL_MIN; }
+
+template <typename T> T warp_reduce(const T* v32, bool is_max) {
+  T a[32];
+  for (int i = 0; i < 32; i++) a[i] = v32[i];
+  for (int off = 16; off > 0; off /= 2) {
+    T b[32];
+    for (int i = 0; i < 32; i++) {
+      T tmp = (i + off < 32) ? a[i + off] : a[i];
+      T val = a[i];
+      if (is_max) { if (tmp > val) val = tmp; }
+      else { if (tmp < val) val = tmp; }
+      b[i] = val;
+    }
+    std::memcpy(a, b, sizeof(a));
+  }
+  return a[0];
+}
+
+inline unsigned char d2u8(double d) {
+  if (d != d) return 0;
+  if (d <= 0.0) return 0;
+  if (d >= 255.0) return 255;
+  return (unsigned char)d;
+}
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <typename T>
+void kseg_block(const T* series, int orig_N, int chunk, int spb, int mk, int minseg, int lds_bytes,
+                T* out /* mk slots of this chunk */) {
+  const int ds = (minseg + 2) / 3;
+  const int N = ((chunk + 1) * spb > orig_N) ? orig_N % spb : spb;
+  int expected_k = mk;
+  if ((chunk + 1) * spb > orig_N) expected_k = ((orig_N % spb) + ((spb + mk - 1) / mk) - 1) / ((spb + mk - 1) / mk);
+  const int N_ds = N / ds;
+  const T* ser = series + (size_t)chunk * spb;
+  // downsample (segmentation.hpp:112-121), including the +threadIdx.x bound
+  std::vector<T> dq(std::max(N_ds, 1), T(0));
+  for (int t = 0; t < 256; t++) {
+    if (t * ds <= N - ds) {
+      T acc = T(0);
+      int cnt = 0;
+      for (int i = 0; i < ds && ds * t + t + i < N; i++) { acc = (T)(acc + ser[ds * t + i]); cnt++; }
+      if (std::is_floating_point<T>::value) dq[t] = (T)(acc / (T)cnt);
+      else dq[t] = cnt ? (T)(acc / (T)cnt) : T(0);  /* integer 0/0 has no defined value: parity unpinned */
+    }
+  }
+  // min / max with warp trees over full warps only (segmentation.hpp:126-150)
+  const int nw = N_ds / 32;
+  std::vector<T> wmax(8), wmin(8);
+  for (int w = 0; w < 8; w++) {
+    T vmx[32], vmn[32];
+    for (int l = 0; l < 32; l++) {
+      int t = w * 32 + l;
+      vmx[l] = t < N_ds ? dq[t] : lim_min<T>();
+      vmn[l] = t < N_ds ? dq[t] : Lim<T>::max();
+    }
+    wmax[w] = warp_reduce(vmx, true);
+    wmin[w] = warp_reduce(vmn, false);
+  }
+  T bmx[32], bmn[32];
+  for (int l = 0; l < 32; l++) {
+    bmx[l] = l < nw ? wmax[l] : lim_min<T>();
+    bmn[l] = l < nw ? wmin[l] : Lim<T>::max();
+  }
+  const T Tmax = warp_reduce(bmx, true);
+  const T Tmin = warp_reduce(bmn, false);
+  // layout of the dynamic shared memory (segmentation.hpp:93-107, 158-165)
+  size_t off = (size_t)(mk + 1) * 2;                  // breakpoints
+  off = align_up(off, 4) + (size_t)4 * (N_ds + 1);    // options
+  off += (size_t)4 * 2 * (N_ds + 1);                  // k_seg_dist
+  off = align_up(off, sizeof(T)) + sizeof(T) * (size_t)(N_ds + 2);  // downsample_qtype + max/min
+  off = align_up(off, 4);                             // squares start
+  const int e = N_ds > 0 ? (int)(((long long)lds_bytes - (long long)off) / ((long long)N_ds * 6)) : 0;
+  // 8-bit rescale, cumulative sums and squares over spans < e
+  std::vector<unsigned short> sums((size_t)std::max(1, N_ds) * std::max(1, e), 0);
+  std::vector<uint32_t> squares((size_t)std::max(1, N_ds) * std::max(1, e), 0);
+  auto SUMS = [&](int l, int r) -> unsigned short& { return sums[(size_t)(r - l) + (size_t)e * l]; };
+  auto SQ = [&](int l, int r) -> uint32_t& { return squares[(size_t)(r - l) + (size_t)e * l]; };
+  for (int t = 0; t < N_ds; t++) {
+    /* quotient in T: integer division for integer T (segmentation.hpp:168) */
+    const T q = (T)((T)(dq[t] - Tmin) / (T)(Tmax - Tmin + 1));
+    SUMS(t, t) = d2u8(256.0 * (double)q);
+    SQ(t, t) = (uint32_t)SUMS(t, t) * (uint32_t)SUMS(t, t);
+  }
+  for (int t = 0; t < N_ds; t++)
+    for (int sp = 1; sp < e; sp++)
+      if (t + sp < N_ds) {
+        SUMS(t, t + sp) = (unsigned short)(SUMS(t, t + sp - 1) + SUMS(t + sp, t + sp));
+        SQ(t, t + sp) = SQ(t, t + sp - 1) + SQ(t + sp, t + sp);
+      }
+  auto DIST = [&](int l, int r) -> uint32_t {  // 1-based (segmentation.hpp:31-35)
+    int a = l - 1, b = r - 1;
+    int s = SUMS(a, b);
+    int ms = s * s / (b - a + 1);
+    return SQ(a, b) - (uint32_t)ms;
+  };
+  std::vector<uint32_t> kdist((size_t)2 * (N_ds + 1), 0), options((size_t)N_ds + 1, 0);
+  std::vector<unsigned short> kpath((size_t)std::max(1, N_ds) * (size_t)(mk + 1), 0);
+  auto KD = [&](int p, int n) -> uint32_t& { return kdist[(size_t)N_ds * (p % 2) + n]; };
+  auto KP = [&](int p, int n) -> unsigned short& { return kpath[(size_t)N_ds * (p - 1) + n]; };
+  std::vector<unsigned short> bp((size_t)mk + 1, 0);
+  bool ex = false;
+  short smallest_noisy = (short)(expected_k + 1);
+  short test_k = (short)((expected_k + 1) / 2);
+  short delta = (short)(test_k / 2);
+  while (!ex) {
+    for (int t = 0; t < N_ds; t++) {
+      KD(1, t + 1) = (t + 1 <= e) ? DIST(1, t + 1) : (uint32_t)(INT_MAX / 2);
+      if (t < test_k) KP(t + 1, t) = (unsigned short)(test_k - 1);
+    }
+    for (int p = 2; p <= test_k; p++) {
+      for (int n = p; n <= N_ds; n++) {
+        for (int t = 0; t < N_ds; t++)
+          if (t + 1 >= p && t + 1 <= n && n - t - 1 < e) options[t + 1] = KD(p - 1, t) + DIST(t + 1, n);
+        int minval = INT_MAX, minidx = -1;
+        for (int idx = (n - e + 1 > p) ? n - e + 1 : p, start = idx; idx - start < e && idx <= n; idx++)
+          if (options[idx] < (uint32_t)minval) { minval = (int)options[idx]; minidx = idx; }
+        KD(p, n) = (uint32_t)minval;
+        KP(p, n - 1) = (unsigned short)((unsigned short)minidx - 1);
+      }
+    }
+    bp[test_k] = (unsigned short)N_ds;
+    for (int p = test_k - 1; p >= 1; p--) {
+      int n = bp[p + 1] - 1;
+      bp[p] = (n >= 0 && n < N_ds) ? KP(p + 1, n) : 0;
+    }
+    bp[0] = 0;
+    bool noisy = false;
+    for (int i = 1; i <= test_k; i++)
+      if ((int)bp[i] - (int)bp[i - 1] < minseg / ds) noisy = true;
+    if (delta == 0 || test_k + delta > mk) ex = true;
+    else if (test_k != 1 && noisy) { smallest_noisy = test_k; test_k = (short)(test_k - delta); }
+    else {
+      if (test_k == 1 || smallest_noisy == test_k + 1) ex = true;
+      else test_k = (short)(test_k + delta);
+    }
+    delta = (short)((delta + 1) / 2);
+    if (ex) {
+      std::vector<T> orig((size_t)spb + 2, T(0));
+      for (int t = 0; t < 256; t++)
+        if (t * ds + chunk * spb < orig_N)
+          for (int i = 0; i < ds && t * ds + i < spb && t * ds + chunk * spb + i < orig_N; i++)
+            orig[(size_t)t * ds + i] = ser[(size_t)t * ds + i];
+      for (int t = 1; t <= 255; t++) {
+        if (t > test_k && t <= mk) { out[t - 1] = Lim<T>::max(); }
+        else if (t > 0 && t <= test_k) {
+          int right = bp[t] * ds;
+          if (right > N) right = N;
+          int left = bp[t - 1] * ds;
+          for (int i = left; i < right - 1; ++i)
+            for (int j = i + 1; j < right; ++j)
+              if (orig[i] > orig[j]) std::swap(orig[i], orig[j]);
+          if ((left - right) % 2) out[t - 1] = orig[(left + right) / 2];
+          else out[t - 1] = (T)((orig[(left + right) / 2] + orig[(left + right) / 2 + 1]) / T(2));
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+int kseg(const T* const* seqs, const uint64_t* lens, size_t n, int minseg, int prefix_skip, int lds_bytes,
+         T* out, uint64_t* out_lens, uint64_t* out_offs) {
+  if (minseg <= 0) return ODBA_E_INVALID;
+  const int ds = (minseg + 2) / 3;
+  const int spb = 256 * ds;
+  const int mk = (int)((256 + ((float)minseg) / ds - 1) / (((float)minseg) / ds));
+  uint64_t cur = 0;
+  for (size_t s = 0; s < n; s++) {
+    const int L = (int)lens[s];
+    const int nch = (L + spb - 1) / spb;
+    std::vector<T> seg((size_t)nch * mk, T(0));
+    for (int ch = 0; ch < nch; ch++) kseg_block<T>(seqs[s], L, ch, spb, mk, minseg, lds_bytes, &seg[(size_t)ch * mk]);
+    // host post-pass (segmentation.hpp:486-578)
+    const T MX = Lim<T>::max();
+    T eps = MX;
+    T prev = seg[0];
+    const size_t SL = seg.size();
+    for (size_t j = 1; j < SL; ++j) {
+      T cv = seg[j];
+      if (cv != MX && prev != MX && ((cv <= prev && (T)(prev - cv) < eps) || (cv > prev && (T)(cv - prev) < eps))) {
+        if ((T)(cv - prev) == 0) {
+          for (size_t k = j; k < SL; k++) {
+            seg[k - 1] = seg[k];
+            if (seg[k] == MX) break;
+          }
+        } else {
+          eps = cv < prev ? (T)(prev - cv) : (T)(cv - prev);
+        }
+      }
+      prev = cv;
+    }
+    bool undef = false;
+    long cursor = 0;
+    for (size_t j = 0; j < SL; ++j) {
+      if (seg[j] == MX) { undef = true; continue; }
+      if (undef) {
+        undef = false;
+        if (cursor >= 1 && seg[cursor - 1] < (T)(seg[j] + eps) && seg[cursor - 1] > (T)(seg[j] - eps))
+          seg[cursor - 1] = (T)((seg[cursor - 1] + seg[j]) / 2);
+      }
+      if ((size_t)cursor != j) seg[cursor] = seg[j];
+      cursor++;
+    }
+    if (cursor >= 1 && seg[cursor - 1] >= MX / 2) cursor--;
+    long skipped = 0;
+    if (prefix_skip) {
+      if (cursor <= prefix_skip) { skipped = cursor - 1; cursor = 1; }
+      else { skipped = prefix_skip; cursor -= prefix_skip; }
+    }
+    if (cursor < 0) cursor = 0;
+    std::memcpy(out + cur, seg.data() + skipped, (size_t)cursor * sizeof(T));
+    out_offs[s] = cur;
+    out_lens[s] = (uint64_t)cursor;
+    cur += (uint64_t)cursor;
+  }
+  return ODBA_OK;
+}
+
 }  // namespace
 
 #define ORC_DISPATCH(dtype, CALL)                            \
@@ -666,6 +905,12 @@ int orc_perform_dba(int dtype, void* const* seqs, const uint64_t* lens, uint64_t
                     uint64_t* centroid_len, int* medoid_sorted, int* perm_out, int* rounds_out, void* dists_out) {
   ORC_DISPATCH(dtype, perform_dba<T>((T* const*)seqs, lens, n, os, oe, norm, max_rounds, medoid_w, path_policy, free_mem,
                                      (T*)centroid_out, centroid_len, medoid_sorted, perm_out, rounds_out, (T*)dists_out))
+}
+
+/* out must hold sum over sequences of mk * ceil(L / (256 * ds)) values. */
+int orc_kseg(int dtype, const void* const* seqs, const uint64_t* lens, uint64_t n, int minseg, int prefix_skip,
+             int lds_bytes, void* out, uint64_t* out_lens, uint64_t* out_offs) {
+  ORC_DISPATCH(dtype, kseg<T>((const T* const*)seqs, lens, n, minseg, prefix_skip, lds_bytes, (T*)out, out_lens, out_offs))
 }
 
 }  // extern "C"

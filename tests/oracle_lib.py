@@ -174,3 +174,29 @@ def read_tokens(path, dtype=np.float64):
         toks = f.read().split()
     conv = _libc.strtof if dtype == np.float32 else _libc.strtod
     return np.array([conv(t, None) for t in toks], dtype=dtype)
+
+
+def kseg_params(minseg):
+    """ds, samples per block and max segments per block (segmentation.hpp:347-353)."""
+    ds = (minseg + 2) // 3
+    spb = 256 * ds
+    x = np.float32(minseg) / np.float32(ds)
+    mk = int((np.float32(256) + x - np.float32(1)) / x)
+    return ds, spb, mk
+
+
+def kseg(seqs, minseg, prefix_skip=0, lds_bytes=65536):
+    seqs = [np.ascontiguousarray(s) for s in seqs]
+    lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+    ds, spb, mk = kseg_params(minseg)
+    cap = int(sum(mk * ((int(L) + spb - 1) // spb) for L in lens))
+    out = np.zeros(max(cap, 1), dtype=seqs[0].dtype)
+    ol = np.zeros(len(seqs), dtype=np.uint64)
+    oo = np.zeros(len(seqs), dtype=np.uint64)
+    L = lib()
+    L.orc_kseg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                           C.c_void_p, C.c_void_p]
+    rc = L.orc_kseg(dtype_code(seqs[0]), _ptrs(seqs), _p(lens), len(seqs), minseg, prefix_skip, lds_bytes, _p(out),
+                    _p(ol), _p(oo))
+    assert rc == 0, rc
+    return [out[int(oo[i]):int(oo[i]) + int(ol[i])].copy() for i in range(len(seqs))]

@@ -22,7 +22,7 @@ def lib():
 
 def declared_symbols():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(odba_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|uint64_t|const char\*)\s+(odba_\w+)\(", txt, re.M)))
 
 
 def test_header_declares_expected_api():

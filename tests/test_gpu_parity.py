@@ -207,3 +207,60 @@ def test_squiggle_workload_sampled_pairs(ctx):
     exp = O.all_pairs(seqs, 0, 1, subset=sample)
     print(f"squiggles n={n} kernel_ms={ms:.1f}", flush=True)
     assert np.array_equal(got[sample].view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_dba_round_stripe_mode_bitexact(ctx, os_, oe):
+    """Stripe mode (dba.hpp:523-548, 646-763): edge columns stored per swath,
+    stripes recomputed right to left with the height cut to the traceback
+    row. In open-end mode the cut row gets the top-row treatment, so stripe
+    paths can differ from full-matrix paths; both follow the reference."""
+    rng = np.random.default_rng(40 + os_ * 2 + oe)
+    lens = [900, 2600, 1500, 3100, 700]
+    seqs = _znorm_cpu(_walks(rng, lens))
+    cen = seqs[1].copy()
+    got, counts, delta = ctx.dba_round(cen, seqs, os_, oe, path_policy=N.PATH_STRIPE)
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_, oe, path_policy=2)
+    assert rc == 0
+    assert np.array_equal(counts, ecounts)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert delta == edelta
+
+
+def test_dba_round_auto_policy_mixes_modes(ctx):
+    """ODBA_PATH_AUTO applies the reference's free-memory rule per sequence:
+    with a 3 MB budget the long sequences take stripe mode, the short ones
+    the full matrix, in one round."""
+    rng = np.random.default_rng(44)
+    lens = [300, 2600, 500, 3100]
+    seqs = _znorm_cpu(_walks(rng, lens))
+    cen = seqs[3].copy()
+    free = 3_000_000
+    got, counts, delta = ctx.dba_round(cen, seqs, 0, 1, path_policy=N.PATH_AUTO, ref_free_mem=free)
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, 0, 1, path_policy=0, free_mem=free)
+    assert rc == 0
+    assert np.array_equal(counts, ecounts) and np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("minseg", [3, 4, 8])
+def test_kseg_bitexact(ctx, minseg):
+    """Adaptive k-segmentation on squiggles: events (segment medians after
+    the chunk-border merge) equal the oracle's. Chunks whose downsampled
+    points come out NaN through the reference's loop bound
+    (segmentation.hpp:114) are part of the comparison: the NaN handling of
+    the warp reductions is reproduced, the NaN -> 8-bit conversion uses the
+    GPU's saturating rule (no fixture pins it)."""
+    raw = WL.short_to_float(WL.squiggles(12, 700, 9000, seed=30 + minseg))
+    got = ctx.kseg(raw, minseg)
+    exp = O.kseg(raw, minseg)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_kseg_prefix_skip_and_lds_budget(ctx):
+    raw = WL.short_to_float(WL.squiggles(6, 2000, 5000, seed=77))
+    for lds in (49152, 65536):
+        got = ctx.kseg(raw, 8, prefix_skip=5, lds_bytes=lds)
+        exp = O.kseg(raw, 8, prefix_skip=5, lds_bytes=lds)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)

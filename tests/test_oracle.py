@@ -212,3 +212,34 @@ def test_medoid_rules():
     d = np.array([1.0, 2.0, 1.0], np.float32)  # (0,1)=1 (0,2)=2 (1,2)=1
     m, sos = O.medoid_single(d, [3, 3, 3])
     assert m == 1 and list(sos) == [5.0, 2.0, 5.0]
+
+
+def test_kseg_parameters_match_reference_formulas():
+    # ds = ceil(minseg/3), 256 threads, max_k = DIV_ROUNDUP(256, minseg/ds) in float (segmentation.hpp:347-353)
+    assert O.kseg_params(8) == (3, 768, 96)
+    assert O.kseg_params(4) == (2, 512, 128)
+    assert O.kseg_params(3) == (1, 256, 86)
+
+
+def test_kseg_constant_and_step_signals():
+    """Closed forms: a constant signal is one level per chunk, merged across
+    chunk borders; a clean two-level step yields exactly its two levels."""
+    flat = [np.full(3000, 90.0, np.float32)]
+    ev = O.kseg(flat, 8)[0]
+    assert len(ev) >= 1 and np.all(ev == 90.0)
+    step = np.concatenate([np.full(300, 80.0), np.full(300, 110.0)]).astype(np.float32)
+    ev = O.kseg([step], 8)[0]
+    assert set(np.unique(ev)) <= {80.0, 110.0, 95.0}
+    assert ev[0] == 80.0 and ev[-1] == 110.0
+
+
+def test_kseg_events_are_segment_medians_of_input_values():
+    from opendba_amd import workloads as WL
+    raw = WL.short_to_float(WL.squiggles(4, 3000, 6000, seed=9))
+    ev = O.kseg(raw, 8)
+    for r, e in zip(raw, ev):
+        assert 0 < len(e) < len(r)
+        # medians of integer-valued samples are integers or half-integers; the
+        # chunk-border average can add quarter steps
+        assert np.all(np.abs(e * 4 - np.round(e * 4)) == 0)
+        assert e.min() >= r.min() and e.max() <= r.max()
