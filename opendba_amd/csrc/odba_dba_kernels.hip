@@ -211,7 +211,8 @@ template <typename T> __device__ __forceinline__ T div_count(T s, uint32_t c) { 
 // default NaN (sign bit set) for that, which is reproduced here.
 template <> __device__ __forceinline__ float div_count<float>(float s, uint32_t c) {
   if (c == 0u && s == 0.0f) return __uint_as_float(0xFFC00000u);
-  return __fdiv_rn(s, (float)c);
+  // correctly rounded: double quotient rounded once to float (53 >= 2*24 + 2)
+  return (float)((double)s / (double)(float)c);
 }
 template <> __device__ __forceinline__ double div_count<double>(double s, uint32_t c) {
   if (c == 0u && s == 0.0) return __longlong_as_double((long long)0xFFF8000000000000ull);

@@ -465,9 +465,16 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
 // ---------------------------------------------------------------------------
 // Whole pair: swaths left to right with the abrogation check between them.
 // ---------------------------------------------------------------------------
+// Correctly rounded float sqrt and division (the IEEE results nvcc's
+// sqrtf and / give by default): evaluated in double and rounded once to
+// float, which is exact for these operations because 53 >= 2*24 + 2. The
+// single-precision hardware sqrt is only accurate to 1 ulp.
+__device__ __forceinline__ float sqrt_rn_f32(float x) { return (float)__builtin_sqrt((double)x); }
+__device__ __forceinline__ float div_rn_f32(float a, float b) { return (float)((double)a / (double)b); }
+
 template <typename T> __device__ __forceinline__ T pair_distance(T cost, int ly, bool normalise) {
-  float f = __fsqrt_rn(Ar<T>::tofloat(cost));
-  if (normalise) f = __fdiv_rn(f, (float)ly);
+  float f = sqrt_rn_f32(Ar<T>::tofloat(cost));
+  if (normalise) f = div_rn_f32(f, (float)ly);
   return Ar<T>::fromfloat(f);
 }
 
