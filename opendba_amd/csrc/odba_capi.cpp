@@ -385,7 +385,7 @@ template <typename T> struct DbaState {
   T* newc = nullptr;
   T* acc = nullptr;
   uint32_t* counts = nullptr;
-  int2* runs = nullptr;
+  int4* runs = nullptr;
   size_t runs_cap = 0;
   uint32_t* mv = nullptr;
   size_t mv_cap = 0;
@@ -625,6 +625,7 @@ template <typename T> struct DbaState {
     HIPC(c, hipMemcpyAsync(&bits, dbits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+    if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
     if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
     double d;
     std::memcpy(&d, &bits, sizeof(d));

@@ -5,11 +5,13 @@
 // sums one sequence after the other, each in traceback order
 // (updateCentroid, dba.hpp:374-423, serialised per sequence at
 // dba.hpp:641-643). Along a monotone path the cells that feed one centroid
-// position form one contiguous run of sequence indices, visited in
-// descending order. The traceback kernel therefore emits one (lo, hi) run
-// per (sequence, centroid position); the accumulation kernel then walks, for
-// each centroid position in parallel, the sequences in cluster order and
-// each run from hi down to lo. The float additions happen in exactly the
+// position are visited in descending sequence-index order and form at most
+// two contiguous counted runs: a block, then (only when the position is a
+// row that became the top row of a height-cut stripe, whose RIGHT steps are
+// OPEN_RIGHT and not counted) the single cell that leaves it. The traceback
+// kernel emits (lo1, hi1, lo2, hi2) per (sequence, centroid position); the
+// accumulation kernel then walks, for each centroid position in parallel,
+// the sequences in cluster order and each run from hi down to lo. The float additions happen in exactly the
 // reference's order, so the sums are bit-identical, with no atomics.
 
 #include <hip/hip_runtime.h>
@@ -63,10 +65,28 @@ __device__ __forceinline__ uint8_t step_code(const Geo& g, const uint32_t* __res
   return c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
 }
 
-// One thread per aligned sequence. runs[seq][c] = (lo, hi) of the sequence
-// indices counted into centroid position c (empty when lo > hi).
+// Counted runs of one centroid position: block 1, then after an uncounted
+// gap block 2 (empty runs have lo > hi).
+struct RunRec {
+  int lo1, hi1, lo2, hi2, gap;
+  __device__ void reset() { lo1 = 1; hi1 = 0; lo2 = 1; hi2 = 0; gap = 0; }
+  __device__ void add(bool counted, int sidx, int* err) {
+    if (!counted) { if (hi1 >= lo1) gap = 1; return; }
+    if (!gap) {
+      if (hi1 < lo1) { hi1 = sidx; lo1 = sidx; } else lo1 = sidx;
+    } else {
+      if (hi2 < lo2) { hi2 = sidx; lo2 = sidx; }
+      else if (lo2 == sidx + 1) lo2 = sidx;
+      else atomicOr(err, 4);  // a third counted block cannot occur on a monotone path
+    }
+  }
+  __device__ int4 out() const { return make_int4(lo1, hi1, lo2, hi2); }
+};
+
+// One thread per aligned sequence. runs[seq][c] = counted runs of sequence
+// indices for centroid position c.
 __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uint32_t* __restrict__ mv,
-                            const uint32_t* __restrict__ mtop, const int* __restrict__ abr, int2* __restrict__ runs,
+                            const uint32_t* __restrict__ mtop, const int* __restrict__ abr, int4* __restrict__ runs,
                             int lc, int os, int oe, int* __restrict__ err) {
   const int jid = blockIdx.x * blockDim.x + threadIdx.x;
   if (jid >= njobs) return;
@@ -74,10 +94,11 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
   const Geo g = make_geo(J.ly, J.lx, os, oe, abr[jid]);
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
-  int2* R = runs + (long long)J.seq * lc;
+  int4* R = runs + (long long)J.seq * lc;
   int i = J.ly - 1, j = J.lx - 1;
   int curc = J.flip ? i : j;
-  int lo = 1, hi = 0;
+  RunRec rr;
+  rr.reset();
   long long caddr = -1;
   uint32_t cword = 0;
   for (;;) {
@@ -85,17 +106,13 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != curc) {
-      R[curc] = make_int2(lo, hi);
+      R[curc] = rr.out();
       curc = c;
-      lo = 1;
-      hi = 0;
+      rr.reset();
     }
     const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);  // dba.hpp:395-415
-    if (counted) {
-      if (hi < lo) { hi = sidx; lo = sidx; }
-      else lo = sidx;
-    }
+    rr.add(counted, sidx, err);
     if (corner) break;
     if (code == ODBA_DIAGONAL) { i--; j--; }
     else if (code == ODBA_UP) { i--; }
@@ -103,7 +120,7 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
     else { atomicOr(err, 1); break; }
     if (i < 0 || j < 0) { atomicOr(err, 1); break; }
   }
-  R[curc] = make_int2(lo, hi);
+  R[curc] = rr.out();
 }
 
 // Stripe-mode traceback for stripe k = K-1-m of every job (updateCentroid
@@ -112,7 +129,7 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
 // new height is the row reached + 1) or reaches the (0,0) anchor.
 __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m, const uint32_t* __restrict__ mv,
                                    const uint32_t* __restrict__ mtop, StripeState* __restrict__ ss,
-                                   int2* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err) {
+                                   int4* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err) {
   const int jid = blockIdx.x * blockDim.x + threadIdx.x;
   if (jid >= njobs) return;
   const TbJob J = jobs[jid];
@@ -127,9 +144,11 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
   const int r0 = os ? 1 : 0;
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
-  int2* R = runs + (long long)J.seq * lc;
+  int4* R = runs + (long long)J.seq * lc;
   int i = S.i, j = S.j;
-  if (S.curc < 0) S.curc = J.flip ? i : j;
+  RunRec rr;
+  rr.lo1 = S.lo; rr.hi1 = S.hi; rr.lo2 = S.lo2; rr.hi2 = S.hi2; rr.gap = S.gap;
+  if (S.curc < 0) { S.curc = J.flip ? i : j; rr.reset(); }
   long long caddr = -1;
   uint32_t cword = 0;
   bool done = false;
@@ -156,17 +175,13 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != S.curc) {
-      R[S.curc] = make_int2(S.lo, S.hi);
+      R[S.curc] = rr.out();
       S.curc = c;
-      S.lo = 1;
-      S.hi = 0;
+      rr.reset();
     }
     const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);
-    if (counted) {
-      if (S.hi < S.lo) { S.hi = sidx; S.lo = sidx; }
-      else S.lo = sidx;
-    }
+    rr.add(counted, sidx, err);
     if (corner) { done = true; break; }
     if (code == ODBA_DIAGONAL) { i--; j--; }
     else if (code == ODBA_UP) { i--; }
@@ -174,7 +189,7 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
     if (i < 0) { S.err = 1; atomicOr(err, 1); break; }
   }
   if (done) {
-    R[S.curc] = make_int2(S.lo, S.hi);
+    R[S.curc] = rr.out();
     S.done = 1;
   } else if (!S.err) {
     if (j != left - 1) { S.err = 1; atomicOr(err, 1); }
@@ -182,23 +197,28 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
   }
   S.i = i;
   S.j = j;
+  S.lo = rr.lo1; S.hi = rr.hi1; S.lo2 = rr.lo2; S.hi2 = rr.hi2; S.gap = rr.gap;
   ss[jid] = S;
 }
 
 template <typename T>
 __global__ void k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
-                             const int2* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
+                             const int4* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
                              uint32_t* __restrict__ counts, int first) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= lc) return;
   T a = first ? T(0) : acc[c];
   uint32_t n = first ? 0u : counts[c];
   for (int s = 0; s < nseq; s++) {
-    const int2 r = runs[(long long)s * lc + c];
+    const int4 r = runs[(long long)s * lc + c];
+    const T* p = data + offs[s];
     if (r.y >= r.x) {
-      const T* p = data + offs[s];
       for (int k = r.y; k >= r.x; k--) a = (T)(a + p[k]);
       n += (uint32_t)(r.y - r.x + 1);
+    }
+    if (r.w >= r.z) {
+      for (int k = r.w; k >= r.z; k--) a = (T)(a + p[k]);
+      n += (uint32_t)(r.w - r.z + 1);
     }
   }
   acc[c] = a;
@@ -348,14 +368,14 @@ __global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* 
 
 // ---------------------------------------------------------------------------
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
-                            const int* abr, int2* runs, int lc, int os, int oe, int* err) {
+                            const int* abr, int4* runs, int lc, int os, int oe, int* err) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err);
   return hipGetLastError();
 }
 
 hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
-                                   const uint32_t* mtop, StripeState* ss, int2* runs, int lc, int os, int oe,
+                                   const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
                                    int* err) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
@@ -364,14 +384,14 @@ hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs,
 }
 
 template <typename T>
-static hipError_t acc_t(hipStream_t st, const void* data, const long long* offs, const int2* runs, int nseq, int lc,
+static hipError_t acc_t(hipStream_t st, const void* data, const long long* offs, const int4* runs, int nseq, int lc,
                         void* acc, uint32_t* counts, int first) {
   hipLaunchKernelGGL(k_accumulate<T>, dim3((lc + 255) / 256), dim3(256), 0, st, (const T*)data, offs, runs, nseq, lc,
                      (T*)acc, counts, first);
   return hipGetLastError();
 }
 
-hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int2* runs,
+hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int4* runs,
                              int nseq, int lc, void* acc, uint32_t* counts, int first) {
   switch (dtype) {
     case ODBA_F32: return acc_t<float>(st, data, offs, runs, nseq, lc, acc, counts, first);

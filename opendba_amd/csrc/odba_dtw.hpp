@@ -52,7 +52,8 @@ struct TbJob {
 struct StripeState {
   int i, j;      // current cell (global column)
   int h;         // current truncated height (rows still in play)
-  int curc, lo, hi;  // open run of the current centroid position
+  int curc, lo, hi;  // open runs of the current centroid position
+  int lo2, hi2, gap;
   int fab;       // swath where the forward pass abandoned the search, -1 none
   int sab;       // 1: this stripe's top row is all OPEN_RIGHT
   int done, err;
@@ -62,12 +63,12 @@ hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe,
                          const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
                          uint32_t* mtop, StripeState* ss);
 hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
-                                   const uint32_t* mtop, StripeState* ss, int2* runs, int lc, int os, int oe,
+                                   const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
                                    int* err);
 
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
-                            const int* abr, int2* runs, int lc, int os, int oe, int* err);
-hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int2* runs,
+                            const int* abr, int4* runs, int lc, int os, int oe, int* err);
+hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int4* runs,
                              int nseq, int lc, void* acc, uint32_t* counts, int first_batch);
 hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const uint32_t* counts, const void* old_c,
                              void* new_c, int lc, unsigned long long* delta_bits, int* err);
