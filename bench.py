@@ -7,9 +7,17 @@ open-end all-vs-all DTW of all 1,999,000 pairs in one persistent launch.
 A step is one full all-vs-all over the resident set.
 
 Metric: GCUPS = sum over pairs of |Y|*|X| (the full unbanded matrix, also for
-pairs whose open-end search stops early) / wall time of the step. The
-roofline object reports the VALU work the kernel actually evaluated
-(4 lane-ops per evaluated cell) against the MI355X VALU peak.
+pairs whose open-end search stops early) / wall time of the step.
+
+Roofline: the swath kernel is bound by VALU issue, not by HBM or MFMA. The
+algorithmic work is 4 VALU lane-ops per evaluated cell (sub, mul, min3, add;
+fp-contract off), counted over the swaths the kernel actually evaluated (an
+open-end pair stops at its abrogated swath). The peak is MI355X's published
+FP32 vector rate, 157.3 TFLOP/s = 78.6e12 lane-ops/s (128 lane-ops per CU per
+clock, an FMA counting as two flops). The same object carries the issue rate
+the cell update's own instruction mix reaches in a probe kernel on the same
+GPU, and the HBM traffic of one launch read from the committed FETCH_SIZE /
+WRITE_SIZE counter summary for this workload.
 """
 import argparse
 import json
@@ -24,7 +32,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "all-vs-all DTW GCUPS + DBA jobs/s with p50/p99 latency vs plain per-job path"
 VALU_OPS_PER_CELL = 4      # sub, mul, min3, add (fp-contract off)
-VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: 3.93e13 lane-ops/s
+FP32_VECTOR_TFLOPS = 157.3  # MI355X published FP32 vector peak; 2 flops per FMA lane-op
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_all_pairs_squiggles{n}_open_end_hbm_traffic.md")
+
+
+def committed_traffic(n):
+    """HBM bytes per all-pairs launch from the committed counter summary for
+    this workload size, or None when there is none."""
+    import re
+    path = PMC_SUMMARY.format(n=n)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def evaluated_cells(lens_sorted, flags):
@@ -121,7 +141,9 @@ def main():
     kavg = float(np.mean(kms))
     ev_cells = evaluated_cells(lens, flags)
     achieved = ev_cells * VALU_OPS_PER_CELL / (kavg / 1000.0) / 1e12
-    peak = info["compute_units"] * 4 * 16 * (info["clock_khz"] * 1e3) / 1e12 if info["clock_khz"] else VALU_PEAK_LANE_OPS / 1e12
+    peak = FP32_VECTOR_TFLOPS / 2.0
+    from opendba_amd import probe
+    mix_rate, _ = probe.cell_mix_rate(16)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -149,10 +171,17 @@ def main():
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(peak, 3),
-            "unit": "TOP/s (VALU lane-ops, 4 per evaluated cell)",
+            "unit": "TOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": committed_traffic(args.n),
+            "kernel": "k_all_pairs<float, os=0, oe=1>",
             "kernel_ms": round(kavg, 3),
+            "ops_per_cell": VALU_OPS_PER_CELL,
+            "peak_basis": "published FP32 vector 157.3 TFLOP/s / 2 flops per lane-op",
+            "cell_mix_probe": round(mix_rate / 1e12, 3),
+            "frac_of_probe": round(achieved / (mix_rate / 1e12), 4),
+            "device_cus": info["compute_units"],
+            "device_clock_mhz": info["clock_khz"] / 1000.0,
         },
     }
     if not args.no_cpu_baseline:

@@ -560,8 +560,15 @@ __device__ __forceinline__ void decode_pair(long long p, long long npairs, int& 
   r = (int)(q - cc * (cc - 1) / 2);
 }
 
+// Float kernels are held to 128 VGPRs (4 waves per SIMD): the cell update's
+// VALU mix only reaches its full issue rate with two or more waves resident
+// per SIMD, and at 129+ VGPRs only 3 fit. 8-byte types would spill under the
+// same limit and keep the compiler's choice.
+template <typename T> struct WavesPerEU { static constexpr int value = 1; };
+template <> struct WavesPerEU<float> { static constexpr int value = 4; };
+
 template <typename T, bool OS, bool OE, bool MINFIRST>
-__global__ void __launch_bounds__(256) k_all_pairs(const T* __restrict__ data, const long long* __restrict__ offs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<T>::value))) k_all_pairs(const T* __restrict__ data, const long long* __restrict__ offs,
                                                    const int* __restrict__ lens, int n, long long npairs,
                                                    unsigned long long* __restrict__ counter, T* __restrict__ dist,
                                                    T* __restrict__ escratch, long long escratch_stride,
