@@ -452,7 +452,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
   }
   for (int i = 0; i < n; i++) { char* z = std::strchr(names[i], '.'); if (z) *z = '\0'; }  // openDBA.cuh:59
   if (seqprefix_file_name != nullptr) {
-    std::cerr << "Prefix chopping (open_prefix mode) is not part of this build's hot path" << std::endl;
+    std::cerr << "Prefix chopping (open_prefix mode) is not supported by this library" << std::endl;
     std::exit(CANNOT_READ_SEQUENCE_PREFIX_FILE);
   }
   if (minseg > 0) {  // openDBA.cuh:85-177
