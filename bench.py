@@ -64,7 +64,7 @@ def evaluated_cells(lens_sorted, flags):
     return tot
 
 
-def cpu_baseline(seqs, target_s=15.0):
+def cpu_baseline(seqs, target_s=15.0, os_=0, oe=1):
     """Oracle all-vs-all on a seeded uniform sample of pairs, timed on this
     host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -85,17 +85,71 @@ def cpu_baseline(seqs, target_s=15.0):
     size = 8 * threads
     sub = np.sort(rng.choice(npairs, size=size, replace=False))
     t0 = time.perf_counter()
-    O.all_pairs(seqs, 0, 1, nthreads=threads, subset=sub)
+    O.all_pairs(seqs, os_, oe, nthreads=threads, subset=sub)
     dt = time.perf_counter() - t0
     size2 = int(max(size, min(npairs, size * target_s / max(dt, 1e-3))))
     sub = np.sort(rng.choice(npairs, size=size2, replace=False))
+    reps = max(1, int(target_s / max(dt * size2 / size, 1e-3))) if size2 == npairs else 1
     t0 = time.perf_counter()
-    O.all_pairs(seqs, 0, 1, nthreads=threads, subset=sub)
+    for _ in range(reps):
+        O.all_pairs(seqs, os_, oe, nthreads=threads, subset=sub)
     dt = time.perf_counter() - t0
-    gc = cells_of(sub) / dt / 1e9
+    gc = reps * cells_of(sub) / dt / 1e9
+    mode = "open_end" if oe else "global"
     return dict(value=gc, unit="GCUPS", cores=threads, kind="port",
-                sample=f"{size2} seeded uniform pairs of the same workload ({cells_of(sub):.3g} full-matrix cells, "
-                       f"open_end, oracle with {threads} threads, {dt:.1f} s)")
+                sample=f"{size2} seeded uniform pairs of the same workload x {reps} ({reps * cells_of(sub):.3g} "
+                       f"full-matrix cells, {mode}, oracle with {threads} threads, {dt:.1f} s)")
+
+
+def run_small32(args, N, WL):
+    """The plain small method: 32 float32 random walks (step sd 1), lengths
+    uniform in [200, 400], z-normalised, global all-vs-all (496 pairs), then
+    medoid and 3 DBA rounds. GCUPS from the median all-pairs kernel time."""
+    raw = WL.random_walks(32, 200, 400, seed=0)
+    steps = max(args.steps, 20)
+    with N.Context(0) as ctx:
+        info = ctx.device_info()
+        seqs, _, _ = ctx.znorm(raw)
+        seqs, _ = WL.stable_length_sort(seqs)
+        lens = [len(s) for s in seqs]
+        nominal = WL.pair_cells(lens)
+        sset = ctx.seqset(seqs)
+        for w in range(max(args.warmup, 3)):
+            sset.all_pairs(0, 0, copy_out=False)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _, _, ms = sset.all_pairs(0, 0, copy_out=False)
+            kms.append(ms)
+        wall = time.perf_counter() - t0
+        sset.close()
+        tp = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            r = ctx.perform_dba(raw, 0, 0, norm=1, max_rounds=3)
+            tp.append(time.perf_counter() - t1)
+    kmed = float(np.median(kms))
+    value = nominal / (kmed / 1000.0) / 1e9
+    achieved = nominal * VALU_OPS_PER_CELL / (kmed / 1000.0) / 1e12
+    peak = FP32_VECTOR_TFLOPS / 2.0
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": steps,
+        "warmup": max(args.warmup, 3), "ms_per_step": round(wall * 1000.0 / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain small: 32 float32 random walks (step sd 1), lengths U[200,400], z-norm, global "
+                               "all-vs-all (496 pairs), then medoid + 3 DBA rounds",
+                   "pairs": 496, "full_matrix_cells": nominal, "mode": "global", "device": info["name"],
+                   "kernel_ms_median": round(kmed, 4),
+                   "pipeline_ms_median": round(float(np.median(tp)) * 1000.0, 3),
+                   "pipeline": "perform_dba: z-norm, sort, all-vs-all, medoid, 3 DBA rounds, rescale"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "TOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_all_pairs<float, os=0, oe=0>", "kernel_ms": round(kmed, 4),
+                     "ops_per_cell": VALU_OPS_PER_CELL},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(seqs, target_s=10.0, os_=0, oe=0)
+    return line
 
 
 def main():
@@ -104,6 +158,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2000)
+    ap.add_argument("--workload", choices=["squiggles", "small32"], default="squiggles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -114,6 +169,9 @@ def main():
     from opendba_amd import native as N
     from opendba_amd import workloads as WL
 
+    if args.workload == "small32":
+        print(json.dumps(run_small32(args, N, WL)))
+        return
     raw = WL.squiggles(args.n, 2000, 20000, seed=1)
     seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))
     lens = [len(s) for s in seqs]

@@ -103,6 +103,21 @@ int odba_dba_set_centroid(odba_dba* dba, const void* centroid, uint64_t lc);
 int odba_dba_step(odba_dba* dba, void* new_centroid, uint32_t* counts_out, double* max_delta);
 
 /*
+ * Traceback recording for the path files DBAUpdate writes per sequence
+ * (writeDTWPath, io_utils.hpp:285-323, called at dba.hpp:756,793): with
+ * recording on, each odba_dba_step keeps the cells every traceback visits,
+ * from the top-right corner to the (0,0)-row anchor, with their step codes
+ * (the stripe-mode codes when a sequence took the stripe path, dba.hpp:756).
+ * odba_dba_get_path returns them for sequence `seq` (session order) packed
+ * as (i << 33) | (j << 3) | code, i = row, j = column of the aligned matrix
+ * (the centroid is on the rows when the sequence was flipped), code 0 for
+ * NIL and the reference code otherwise. steps == NULL only reports
+ * *n_steps.
+ */
+int odba_dba_record_paths(odba_dba* dba, int on);
+int odba_dba_get_path(odba_dba* dba, int seq, uint64_t* steps, uint64_t capacity, uint64_t* n_steps);
+
+/*
  * Z-normalise sequences in place (normalizeSequences, gpu_utils.hpp:131-170):
  * population sigma, sigma == 0 treated as 1; mu / sigma (optional) receive
  * each sequence's mean and standard deviation before that substitution.

@@ -46,6 +46,7 @@
 #define MEDOID_FINDING_ERROR 36
 #define UNKNOWN_ALGO 42
 #define CANNOT_READ_SEQUENCE_PREFIX_FILE 24
+#define CANNOT_WRITE_DTW_PATH_MATRIX 26
 
 namespace odba_host {
 
@@ -230,13 +231,71 @@ void normalizeSequences(T** sequences, size_t num_sequences, size_t* sequence_le
                    "normalizeSequences");
 }
 
+namespace odba_host {
+
+inline const char* move_name(int code) {  // io_utils.hpp:303-308
+  switch (code) {
+    case 1: return "DIAG";
+    case 2: return "RIGHT";
+    case 3: return "UP";
+    case 4: return "OPEN_RIGHT";
+    case 0: return "NIL";
+    case 5: return "NIL_OPEN_RIGHT";
+    default: return "?";
+  }
+}
+
+/* The per-sequence path files of one DBA round, <prefix>.path<k>.txt
+ * (dba.hpp:513-518, writeDTWPath io_utils.hpp:285-323): the sequence name,
+ * then one line per traceback cell from the top-right corner,
+ * "seq_index<TAB>seq_value<TAB>centroid_index<TAB>centroid_value<TAB>MOVE",
+ * then the anchor cell, printed as row, seq[row], column, centroid[column]
+ * whatever the orientation (as the reference prints it). With an empty
+ * prefix the reference creates the files and writes nothing into them. */
+template <typename T>
+void write_dba_paths(odba_dba* dba, const std::string& prefix, T* const* seqs, char* const* names, const size_t* lens,
+                     size_t n, const T* centroid, size_t lc, int use_open_end) {
+  std::vector<uint64_t> st;
+  for (size_t k = 0; k < n; k++) {
+    const std::string fn = prefix + ".path" + std::to_string(k) + ".txt";
+    std::ofstream f(fn.c_str());
+    if (!f.is_open()) {
+      std::cerr << "Cannot write to " << fn << std::endl;
+      std::exit(CANNOT_WRITE_DTW_PATH_MATRIX);
+    }
+    if (prefix.empty()) continue;
+    uint64_t ns = 0;
+    check(odba_dba_get_path(dba, (int)k, nullptr, 0, &ns), "DTW path");
+    st.resize(ns);
+    check(odba_dba_get_path(dba, (int)k, st.data(), ns, &ns), "DTW path");
+    const bool flip = use_open_end && lc < lens[k];
+    const T* sq = seqs[k];
+    f << names[k] << std::endl;
+    for (uint64_t s = 0; s < ns; s++) {
+      const size_t i = (size_t)(st[s] >> 33), j = (size_t)((st[s] >> 3) & 0x3fffffffu);
+      const int code = (int)(st[s] & 7u);
+      if (code == 0 || code == 5) {  // anchor (io_utils.hpp:316-318)
+        f << i << "\t" << (i < lens[k] ? sq[i] : T(0)) << "\t" << j << "\t" << (j < lc ? centroid[j] : T(0)) << "\t"
+          << move_name(code) << std::endl;
+      } else if (flip) {
+        f << j << "\t" << sq[j] << "\t" << i << "\t" << centroid[i] << "\t" << move_name(code) << std::endl;
+      } else {
+        f << i << "\t" << sq[i] << "\t" << j << "\t" << centroid[j] << "\t" << move_name(code) << std::endl;
+      }
+    }
+  }
+}
+
+}  // namespace odba_host
+
 /* Medoid search on sequences already sorted by length (dba.hpp:36-365):
  * all-vs-all distances, <prefix>.pair_dists.txt, sum of squares in T, one
  * cluster when *cdist == 1. Returns new int[num_clusters]. */
 template <typename T>
 int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* sequence_lengths, char** sequence_names,
                               int use_open_start, int use_open_end, char* output_prefix, double* cdist,
-                              int* memberships) {
+                              int* memberships, const void* stream = nullptr) {
+  (void)stream;
   const size_t n = num_sequences;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + n);
   std::vector<T> d(n * (n - 1) / 2);
@@ -265,7 +324,7 @@ int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* seque
   mats << "0" << std::endl;
   if (*cdist != 1) {
     std::cerr << "Clustering with a distance cutoff or K (cdist=" << *cdist
-              << ") is not part of this build's hot path; all sequences form one cluster" << std::endl;
+              << ") is not supported by this library; all sequences form one cluster" << std::endl;
   }
   for (size_t i = 0; i < n; i++) memberships[i] = 0;
   int* medoids = new int[1];
@@ -288,19 +347,26 @@ int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* seque
   return medoids;
 }
 
-/* One DBA round (dba.hpp:432-848); returns the max centroid movement. */
+/* One DBA round (dba.hpp:432-848): the new centroid in updatedMean, the
+ * path files <output_prefix>.path<k>.txt; returns the max centroid
+ * movement. */
 template <typename T>
 double DBAUpdate(T* C, size_t centerLength, T** sequences, char** sequence_names, size_t num_sequences,
                  size_t* sequence_lengths, int use_open_start, int use_open_end, T* updatedMean,
-                 std::string output_prefix) {
-  (void)sequence_names;
-  (void)output_prefix;
+                 std::string output_prefix, const void* stream = nullptr) {
+  (void)stream;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + num_sequences);
   double delta = 0;
-  odba_host::check(odba_dba_round(odba_host::ctx(), odba_host::DType<T>::v, C, centerLength,
-                                  (const void* const*)sequences, l.data(), (int)num_sequences, use_open_start,
-                                  use_open_end, ODBA_PATH_FULL, 0, updatedMean, nullptr, &delta),
+  odba_dba* dba = nullptr;
+  odba_host::check(odba_dba_create(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)sequences, l.data(),
+                                   (int)num_sequences, use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
                    "DBAUpdate");
+  odba_host::check(odba_dba_set_centroid(dba, C, centerLength), "DBAUpdate");
+  odba_host::check(odba_dba_record_paths(dba, 1), "DBAUpdate");
+  odba_host::check(odba_dba_step(dba, updatedMean, nullptr, &delta), "DBAUpdate");
+  odba_host::write_dba_paths<T>(dba, output_prefix, sequences, sequence_names, sequence_lengths, num_sequences, C,
+                                centerLength, use_open_end);
+  odba_dba_destroy(dba);
   return delta;
 }
 
@@ -334,8 +400,8 @@ int readCentroidCheckpointFromFile(const char* name, T* c, size_t len) {  // io_
 template <typename T>
 void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char** sequence_names, int use_open_start,
                 int use_open_end, char* output_prefix, int norm_sequences, double cdist, char** series_file_names,
-                int num_series, int read_mode, bool is_segmented, int algo_mode) {
-  (void)series_file_names; (void)num_series; (void)read_mode; (void)is_segmented;
+                int num_series, int read_mode, bool is_segmented, int algo_mode, const void* stream = nullptr) {
+  (void)series_file_names; (void)num_series; (void)read_mode; (void)is_segmented; (void)stream;
   const int n = num_sequences;
   for (int i = 0; i < n; i++)  // dba.hpp:868-872
     if (sequence_lengths[i] && (double)sequences[i][sequence_lengths[i] - 1] >= std::sqrt((double)std::numeric_limits<T>::max()))
@@ -377,10 +443,16 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
                                    use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
                    "DBA session");
   odba_host::check(odba_dba_set_centroid(dba, cen.data(), lc), "DBA centroid");
+  odba_host::check(odba_dba_record_paths(dba, 1), "DBA paths");
   std::cerr << "Processing cluster 1 of 1, " << n << " members, medoid " << sequence_names[med] << " has length " << lc
             << std::endl;
+  // Every round rewrites <prefix>.0.path<k>.txt against that round's input
+  // centroid; only the last round's files remain, so they are written once,
+  // from the last round's recorded tracebacks.
+  std::vector<T> round_in(cen);
   for (int i = 0; i < 250; i++) {  // dba.hpp:1078-1110
     double delta = 0;
+    if (i > 0) round_in = nb;
     odba_host::check(odba_dba_step(dba, nb.data(), nullptr, &delta), "DBAUpdate");
     std::cerr << "New delta is " << delta << std::endl;
     if (delta == 0) break;
@@ -394,6 +466,8 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
     }
     writeCentroidCheckpointToFile<T>(ck.c_str(), nb.data(), lc);
   }
+  odba_host::write_dba_paths<T>(dba, prefix + ".0", sequences, sequence_names, sequence_lengths, (size_t)n,
+                                round_in.data(), lc, use_open_end);
   odba_dba_destroy(dba);
   if (norm_sequences)
     for (size_t i = 0; i < lc; i++) nb[i] = (T)(mu[med] + nb[i] * sg[med]);

@@ -3,7 +3,9 @@
 //           <global|open_start|open_end|open|open_prefix_S_L> <output prefix>
 //           <minseg[,minseg2]> <prefix file|/dev/null> <cdist> <series...>
 // Options are parsed with getopt before the positional arguments are read
-// from argv[1...], as in the reference (so "-n" works after the series).
+// from argv[1...], as in the reference. GNU getopt permutes "-n" to argv[1],
+// where it is then rejected as a read mode, so, as with the reference
+// binary, a "-n" anywhere ends the run with exit code 1.
 #include <getopt.h>
 
 #include <cstdio>

@@ -417,11 +417,18 @@ template <typename T> struct DbaState {
   size_t sjobs_cap = 0;
   TbJob* stbj = nullptr;
   size_t stbj_cap = 0;
+  // Traceback recording (the cells writeDTWPath prints): per sequence a
+  // count slot then up to |seq| + |centroid| packed steps.
+  bool rec = false;
+  bool rec_valid = false;
+  unsigned long long* pbuf = nullptr;
+  size_t pbuf_cap = 0;
+  std::vector<long long> h_poff;
 
   ~DbaState() {
     for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
-                    (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj})
+                    (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf})
       if (p) hipFree(p);
   }
 
@@ -527,6 +534,7 @@ template <typename T> struct DbaState {
       sj[q].mv_off = words;
       sj[q].top_off = (long long)q * 64;
       tb[q].ly = H; tb[q].lx = LX; tb[q].flip = flip; tb[q].seq = b - b0; tb[q].mv_off = words; tb[q].top_off = (long long)q * 64;
+      tb[q].path_off = rec ? h_poff[b] : -1;
       slot_elems += (long long)K * (H + 256);
       words += std::max(1LL, swath_words(H, os, oe));
       maxK = std::max(maxK, K);
@@ -553,7 +561,8 @@ template <typename T> struct DbaState {
     for (int m = 0; m < maxK; m++) {
       HIPC(c, launch_stripe(dtype_of<T>(), minfirst, false, os, oe, steps_blocks, c->stream, sjobs, nj, m, counter,
                             slots, smv, stop, sstate));
-      HIPC(c, launch_stripe_traceback(c->stream, stbj, nj, m, smv, stop, sstate, runs, lc, os, oe, err));
+      HIPC(c, launch_stripe_traceback(c->stream, stbj, nj, m, smv, stop, sstate, runs, lc, os, oe, err,
+                                      rec ? pbuf : nullptr));
     }
     HIPC(c, hipStreamSynchronize(c->stream));
     return ODBA_OK;
@@ -564,6 +573,17 @@ template <typename T> struct DbaState {
     HIPC(c, hipMemsetAsync(dbits, 0, sizeof(unsigned long long), c->stream));
     int rc = grow(c, runs, runs_cap, (size_t)n * lc);
     if (rc) return rc;
+    rec_valid = false;
+    if (rec) {
+      h_poff.resize(n);
+      long long tot = 0;
+      for (int b = 0; b < n; b++) {
+        h_poff[b] = tot;
+        tot += 1 + (long long)h_lens[b] + lc;
+      }
+      rc = grow(c, pbuf, pbuf_cap, (size_t)tot);
+      if (rc) return rc;
+    }
     // batches of sequences whose move words fit the budget (sequence order kept)
     int b0 = 0;
     bool first = true;
@@ -593,6 +613,7 @@ template <typename T> struct DbaState {
           pj.push_back(j);
           TbJob t;
           t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
+          t.path_off = rec ? h_poff[b1] : -1;
           tb.push_back(t);
           maxly = std::max(maxly, ly);
         }
@@ -608,7 +629,7 @@ template <typename T> struct DbaState {
         HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
         rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr);
         if (rc) return rc;
-        HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err));
+        HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
       rc = run_stripe_batch(stripe_idx, b0);
       if (rc) return rc;
@@ -631,6 +652,23 @@ template <typename T> struct DbaState {
     std::memcpy(&d, &bits, sizeof(d));
     if (delta) *delta = d;
     std::swap(cen, newc);
+    rec_valid = rec;
+    return ODBA_OK;
+  }
+
+  // Recorded traceback of sequence `seq` from the last round.
+  int get_path(int seq, uint64_t* out, uint64_t cap, uint64_t* nsteps) {
+    if (!rec_valid) return fail(c, ODBA_E_INVALID, "no traceback was recorded in the last DBA step");
+    if (seq < 0 || seq >= n) return fail(c, ODBA_E_INVALID, "sequence index %d out of range", seq);
+    unsigned long long cnt = 0;
+    HIPC(c, hipMemcpyAsync(&cnt, pbuf + h_poff[seq], sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (nsteps) *nsteps = cnt;
+    if (!out) return ODBA_OK;
+    if (cap < cnt) return fail(c, ODBA_E_INVALID, "path buffer holds %llu steps, %llu needed",
+                               (unsigned long long)cap, cnt);
+    HIPC(c, hipMemcpyAsync(out, pbuf + h_poff[seq] + 1, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
     return ODBA_OK;
   }
 };
@@ -1064,6 +1102,19 @@ int odba_dba_step(odba_dba* d, void* new_centroid, uint32_t* counts_out, double*
   odba_ctx* c = d->ctx;
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
   ODBA_DISPATCH(d->dtype, dba_step_t<T>(c, d->st, (T*)new_centroid, counts_out, max_delta))
+}
+
+int odba_dba_record_paths(odba_dba* d, int on) {
+  if (!d) return ODBA_E_INVALID;
+  odba_ctx* c = d->ctx;
+  ODBA_DISPATCH(d->dtype, (((DbaState<T>*)d->st)->rec = (on != 0), ODBA_OK))
+}
+
+int odba_dba_get_path(odba_dba* d, int seq, uint64_t* steps, uint64_t capacity, uint64_t* n_steps) {
+  if (!d) return ODBA_E_INVALID;
+  odba_ctx* c = d->ctx;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(d->dtype, ((DbaState<T>*)d->st)->get_path(seq, steps, capacity, n_steps))
 }
 
 }  // extern "C"

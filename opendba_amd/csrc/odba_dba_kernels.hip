@@ -84,10 +84,12 @@ struct RunRec {
 };
 
 // One thread per aligned sequence. runs[seq][c] = counted runs of sequence
-// indices for centroid position c.
+// indices for centroid position c. With a path buffer and J.path_off >= 0
+// every visited cell is also recorded in traceback order (the cells and
+// codes writeDTWPath prints, io_utils.hpp:285-323), the count first.
 __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uint32_t* __restrict__ mv,
                             const uint32_t* __restrict__ mtop, const int* __restrict__ abr, int4* __restrict__ runs,
-                            int lc, int os, int oe, int* __restrict__ err) {
+                            int lc, int os, int oe, int* __restrict__ err, unsigned long long* __restrict__ pbuf) {
   const int jid = blockIdx.x * blockDim.x + threadIdx.x;
   if (jid >= njobs) return;
   const TbJob J = jobs[jid];
@@ -101,8 +103,11 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
   rr.reset();
   long long caddr = -1;
   uint32_t cword = 0;
+  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  long long np = 0;
   for (;;) {
     const uint8_t code = step_code(g, M, TP, i, j, caddr, cword);
+    if (P) P[1 + np++] = pack_step(i, j, code);
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != curc) {
@@ -121,6 +126,7 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
     if (i < 0 || j < 0) { atomicOr(err, 1); break; }
   }
   R[curc] = rr.out();
+  if (P) P[0] = (unsigned long long)np;
 }
 
 // Stripe-mode traceback for stripe k = K-1-m of every job (updateCentroid
@@ -129,7 +135,8 @@ __global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uin
 // new height is the row reached + 1) or reaches the (0,0) anchor.
 __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m, const uint32_t* __restrict__ mv,
                                    const uint32_t* __restrict__ mtop, StripeState* __restrict__ ss,
-                                   int4* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err) {
+                                   int4* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err,
+                                   unsigned long long* __restrict__ pbuf) {
   const int jid = blockIdx.x * blockDim.x + threadIdx.x;
   if (jid >= njobs) return;
   const TbJob J = jobs[jid];
@@ -152,6 +159,7 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
   long long caddr = -1;
   uint32_t cword = 0;
   bool done = false;
+  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
   while (j >= left) {
     uint8_t code;
     if (i == 0 && j == 0) code = os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
@@ -172,6 +180,7 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
       }
     }
+    if (P) P[1 + S.npath++] = pack_step(i, j, code);
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != S.curc) {
@@ -191,6 +200,7 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
   if (done) {
     R[S.curc] = rr.out();
     S.done = 1;
+    if (P) P[0] = (unsigned long long)S.npath;
   } else if (!S.err) {
     if (j != left - 1) { S.err = 1; atomicOr(err, 1); }
     S.h = i + 1;
@@ -368,18 +378,19 @@ __global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* 
 
 // ---------------------------------------------------------------------------
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
-                            const int* abr, int4* runs, int lc, int os, int oe, int* err) {
+                            const int* abr, int4* runs, int lc, int os, int oe, int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err);
+  hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe,
+                     err, pbuf);
   return hipGetLastError();
 }
 
 hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
                                    const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
-                                   int* err) {
+                                   int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
-                     lc, os, oe, err);
+                     lc, os, oe, err, pbuf);
   return hipGetLastError();
 }
 

@@ -46,7 +46,15 @@ struct TbJob {
   int seq;             // index of the sequence in the cluster order
   long long mv_off;    // word offset of the pair's step words
   long long top_off;   // word offset of the open-end top-row words
+  long long path_off;  // -1, or the slot in the path buffer: [count][steps...]
 };
+
+// One visited cell of a recorded traceback: row i (31 bits), column j
+// (30 bits), step code (3 bits; 0 = NIL, otherwise the reference code 1..5).
+__host__ __device__ inline unsigned long long pack_step(int i, int j, int code) {
+  return ((unsigned long long)(unsigned)i << 33) | ((unsigned long long)(unsigned)j << 3) |
+         (unsigned long long)(code == 255 ? 0 : code);
+}
 
 // Per-job traceback state of stripe mode, carried across stripes.
 struct StripeState {
@@ -54,6 +62,7 @@ struct StripeState {
   int h;         // current truncated height (rows still in play)
   int curc, lo, hi;  // open runs of the current centroid position
   int lo2, hi2, gap;
+  int npath;     // recorded traceback steps so far
   int fab;       // swath where the forward pass abandoned the search, -1 none
   int sab;       // 1: this stripe's top row is all OPEN_RIGHT
   int done, err;
@@ -64,10 +73,10 @@ hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe,
                          uint32_t* mtop, StripeState* ss);
 hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs, int m, const uint32_t* mv,
                                    const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
-                                   int* err);
+                                   int* err, unsigned long long* pbuf);
 
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
-                            const int* abr, int4* runs, int lc, int os, int oe, int* err);
+                            const int* abr, int4* runs, int lc, int os, int oe, int* err, unsigned long long* pbuf);
 hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const long long* offs, const int4* runs,
                              int nseq, int lc, void* acc, uint32_t* counts, int first_batch);
 hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const uint32_t* counts, const void* old_c,

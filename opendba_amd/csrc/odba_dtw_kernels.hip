@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(256) k_stripe_forward(const StripeJob<T>* __re
     }
     if (lane == 0) {
       StripeState z;
-      z.i = jb.H - 1; z.j = jb.LX - 1; z.h = jb.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0;
+      z.i = jb.H - 1; z.j = jb.LX - 1; z.h = jb.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0; z.npath = 0;
       z.fab = ab; z.sab = 0; z.done = 0; z.err = 0;
       st[j] = z;
     }

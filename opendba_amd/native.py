@@ -22,6 +22,7 @@ EXPORTS = [
     "odba_dtw_all_pairs", "odba_seqset_create", "odba_seqset_destroy", "odba_dtw_all_pairs_resident",
     "odba_dtw_path", "odba_dba_round", "odba_znorm", "odba_perform_dba", "odba_kseg", "odba_kseg_capacity",
     "odba_dba_create", "odba_dba_destroy", "odba_dba_set_centroid", "odba_dba_step",
+    "odba_dba_record_paths", "odba_dba_get_path",
 ]
 
 
@@ -59,6 +60,12 @@ def load():
     L.odba_kseg.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, vp, u64, vp, vp]
     L.odba_kseg_capacity.argtypes = [vp, i32, i32]
     L.odba_kseg_capacity.restype = u64
+    L.odba_dba_create.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, u64, C.POINTER(vp)]
+    L.odba_dba_destroy.argtypes = [vp]
+    L.odba_dba_set_centroid.argtypes = [vp, vp, u64]
+    L.odba_dba_step.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+    L.odba_dba_record_paths.argtypes = [vp, i32]
+    L.odba_dba_get_path.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
     _lib = L
     return L
 
@@ -160,6 +167,66 @@ class Context:
                                             open_end, norm, max_rounds, path_policy, ref_free_mem, _p(cen),
                                             C.byref(clen), C.byref(med), _p(perm), C.byref(rounds), _p(dists)))
         return dict(centroid=cen[: clen.value], medoid=med.value, perm=perm, rounds=rounds.value, dists=dists)
+
+
+def unpack_steps(packed):
+    """(i, j, code) arrays of packed traceback steps; code is the reference
+    step code (NIL = 255)."""
+    packed = np.asarray(packed, dtype=np.uint64)
+    i = (packed >> np.uint64(33)).astype(np.int64)
+    j = ((packed >> np.uint64(3)) & np.uint64(0x3FFFFFFF)).astype(np.int64)
+    code = (packed & np.uint64(7)).astype(np.uint8)
+    code[code == 0] = 255
+    return i, j, code
+
+
+class DbaSession:
+    """Device-resident DBA session (odba_dba_*): sequences uploaded once,
+    one DBAUpdate per step."""
+
+    def __init__(self, ctx, seqs, open_start=0, open_end=0, path_policy=PATH_FULL, ref_free_mem=0):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.seqs = [np.ascontiguousarray(s) for s in seqs]
+        self.dtype = self.seqs[0].dtype
+        lens = np.array([len(s) for s in self.seqs], dtype=np.uint64)
+        self.h = C.c_void_p()
+        ctx._check(self.L.odba_dba_create(ctx.h, CODE[self.dtype], _ptrs(self.seqs), _p(lens), len(self.seqs),
+                                          open_start, open_end, path_policy, ref_free_mem, C.byref(self.h)))
+        self.lc = 0
+
+    def set_centroid(self, c):
+        c = np.ascontiguousarray(c, dtype=self.dtype)
+        self.lc = len(c)
+        self.ctx._check(self.L.odba_dba_set_centroid(self.h, _p(c), len(c)))
+
+    def record_paths(self, on=True):
+        self.ctx._check(self.L.odba_dba_record_paths(self.h, 1 if on else 0))
+
+    def step(self):
+        out = np.zeros(self.lc, dtype=self.dtype)
+        counts = np.zeros(self.lc, dtype=np.uint32)
+        delta = C.c_double(0)
+        self.ctx._check(self.L.odba_dba_step(self.h, _p(out), _p(counts), C.byref(delta)))
+        return out, counts, delta.value
+
+    def path(self, seq):
+        n = C.c_uint64()
+        self.ctx._check(self.L.odba_dba_get_path(self.h, seq, None, 0, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint64)
+        self.ctx._check(self.L.odba_dba_get_path(self.h, seq, _p(buf), n.value, C.byref(n)))
+        return unpack_steps(buf)
+
+    def close(self):
+        if self.h:
+            self.L.odba_dba_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def _kseg(ctx, seqs, min_segment_length, prefix_skip=0, lds_bytes=65536):
