@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -215,6 +216,46 @@ int odba_seqset_destroy(odba_seqset* s) {
   return ODBA_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Pair schedule of the all-vs-all: one wave per pair (many pairs), or a
+// workgroup of waves per pair over column strips (too few pairs to give
+// every SIMD work). ODBA_PAIR_SCHEDULE=wave|strip overrides the choice.
+struct PairSchedule {
+  bool strips = false;
+  int C = 16, waves = 4, nblocks = 1;
+};
+
+PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, long long npairs) {
+  PairSchedule ps;
+  int minlen = s->h_lens.empty() ? 0 : s->h_lens[0];
+  for (int L : s->h_lens) minlen = std::min(minlen, L);
+  const bool eligible = (s->dtype == ODBA_F32 || ((s->dtype == ODBA_I32 || s->dtype == ODBA_U32) && s->int_no_wrap)) &&
+                        minlen >= 64;
+  const char* env = std::getenv("ODBA_PAIR_SCHEDULE");
+  // Fewer pairs than one wave per SIMD, and long enough that the strip
+  // pipeline's fill (about 128 rows per strip) is small against |Y|.
+  bool want = npairs < 4LL * c->cus && minlen >= 4096;
+  if (env && !std::strcmp(env, "wave")) want = false;
+  if (env && !std::strcmp(env, "strip")) want = true;
+  ps.strips = want && eligible;
+  if (!ps.strips) {
+    ps.nblocks = grid_blocks(c, npairs);
+    return ps;
+  }
+  ps.C = s->maxlen <= 16 * 64 ? 1 : (s->maxlen <= 16 * 256 ? 4 : 16);
+  ps.waves = (int)std::min<long long>(16, std::max<long long>(1, (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C)));
+  const long long per_cu = std::max(1, 16 / ps.waves);  // 16 waves of 128 VGPRs per CU
+  ps.nblocks = (int)std::max<long long>(1, std::min<long long>(npairs, per_cu * c->cus));
+  return ps;
+}
+
+}  // namespace
+
+extern "C" {
+
 int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, void* dist_out, int* flags_out,
                                 double* kernel_ms) {
   if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
@@ -223,9 +264,11 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences");
   const long long npairs = (long long)n * (n - 1) / 2;
   const size_t es = dtype_size(s->dtype);
-  const int nblocks = grid_blocks(c, npairs);
+  const PairSchedule ps = choose_schedule(c, s, npairs);
+  const int nblocks = ps.nblocks;
   const long long stride = s->maxlen + 256;
-  int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
+  const size_t scratch = ps.strips ? (size_t)nblocks * 2 * ps.waves * stride * es : (size_t)nblocks * 4 * 2 * stride * es;
+  int rc = ensure_scratch(c, scratch);
   if (rc) return rc;
   void* dist = nullptr;
   int* flags = nullptr;
@@ -237,8 +280,11 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   unsigned long long* counter = (unsigned long long*)c->small;
   HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
   HIPC(c, hipEventRecord(c->ev0, c->stream));
-  hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs, s->lens, n,
-                                  npairs, counter, dist, c->esc, stride, flags);
+  hipError_t e = ps.strips
+                     ? launch_all_pairs_strips(s->dtype, ps.C, ps.waves, os, oe, nblocks, c->stream, s->data, s->offs,
+                                               s->lens, n, npairs, counter, dist, c->esc, stride, flags)
+                     : launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs,
+                                        s->lens, n, npairs, counter, dist, c->esc, stride, flags);
   if (e != hipSuccess) { hipFree(dist); if (flags) hipFree(flags); return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e)); }
   HIPC(c, hipEventRecord(c->ev1, c->stream));
   if (dist_out) HIPC(c, hipMemcpyAsync(dist_out, dist, npairs * es, hipMemcpyDeviceToHost, c->stream));

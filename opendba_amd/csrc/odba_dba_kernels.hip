@@ -66,78 +66,139 @@ __device__ __forceinline__ uint8_t step_code(const Geo& g, const uint32_t* __res
 }
 
 // Counted runs of one centroid position: block 1, then after an uncounted
-// gap block 2 (empty runs have lo > hi).
+// gap block 2 (empty runs have lo > hi). add() returns false when a third
+// counted block would start, which a monotone path cannot produce.
 struct RunRec {
   int lo1, hi1, lo2, hi2, gap;
   __device__ void reset() { lo1 = 1; hi1 = 0; lo2 = 1; hi2 = 0; gap = 0; }
-  __device__ void add(bool counted, int sidx, int* err) {
-    if (!counted) { if (hi1 >= lo1) gap = 1; return; }
+  __device__ bool add(bool counted, int sidx) {
+    if (!counted) { if (hi1 >= lo1) gap = 1; return true; }
     if (!gap) {
       if (hi1 < lo1) { hi1 = sidx; lo1 = sidx; } else lo1 = sidx;
     } else {
       if (hi2 < lo2) { hi2 = sidx; lo2 = sidx; }
       else if (lo2 == sidx + 1) lo2 = sidx;
-      else atomicOr(err, 4);  // a third counted block cannot occur on a monotone path
+      else return false;
     }
+    return true;
   }
   __device__ int4 out() const { return make_int4(lo1, hi1, lo2, hi2); }
 };
 
-// One thread per aligned sequence. runs[seq][c] = counted runs of sequence
-// indices for centroid position c. With a path buffer and J.path_off >= 0
-// every visited cell is also recorded in traceback order (the cells and
-// codes writeDTWPath prints, io_utils.hpp:285-323), the count first.
-__global__ void k_traceback(const TbJob* __restrict__ jobs, int njobs, const uint32_t* __restrict__ mv,
-                            const uint32_t* __restrict__ mtop, const int* __restrict__ abr, int4* __restrict__ runs,
-                            int lc, int os, int oe, int* __restrict__ err, unsigned long long* __restrict__ pbuf) {
-  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Wave-cooperative cache of path words for a traceback that walks up and to
+// the left: lane t holds the words of row i0 - t for the word-lanes l0,
+// l0-1, l0-2, l0-3 of swath k0 (64 rows x 4*C columns). The walk reads a
+// word with readlane; a tile is refilled only when the path leaves it, so a
+// path costs one round of loads per 64 rows or 4*C columns instead of one
+// dependent load per row.
+struct WordTile {
+  uint32_t w0, w1, w2, w3;
+  int i0, l0, k0;
+  __device__ void init() { i0 = -1; l0 = -1; k0 = -1; w0 = w1 = w2 = w3 = 0; }
+  // word of main-region cell row i, word-lane l of swath k; step s = row - r0 + l
+  __device__ uint32_t get(const uint32_t* __restrict__ M, long long S64, int r0, int k, int i, int l, int lane) {
+    if (!(k == k0 && i <= i0 && i > i0 - 64 && l <= l0 && l > l0 - 4)) {
+      i0 = i; l0 = l; k0 = k;
+      const int row = i - lane;
+      const long long base = (long long)k * S64;
+      w0 = (row >= r0) ? M[base + (long long)(row - r0 + l) * 64 + l] : 0u;
+      w1 = (row >= r0 && l >= 1) ? M[base + (long long)(row - r0 + l - 1) * 64 + (l - 1)] : 0u;
+      w2 = (row >= r0 && l >= 2) ? M[base + (long long)(row - r0 + l - 2) * 64 + (l - 2)] : 0u;
+      w3 = (row >= r0 && l >= 3) ? M[base + (long long)(row - r0 + l - 3) * 64 + (l - 3)] : 0u;
+    }
+    const int dl = l0 - l;
+    const uint32_t sel = dl == 0 ? w0 : (dl == 1 ? w1 : (dl == 2 ? w2 : w3));
+    return (uint32_t)__builtin_amdgcn_readlane((int)sel, i0 - i);
+  }
+};
+
+// One wavefront per aligned sequence (all lanes walk the same path; lane 0
+// stores). runs[seq][c] = counted runs of sequence indices for centroid
+// position c. With a path buffer and J.path_off >= 0 every visited cell is
+// also recorded in traceback order (the cells and codes writeDTWPath prints,
+// io_utils.hpp:285-323), the count first.
+__global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ jobs, int njobs,
+                                                   const uint32_t* __restrict__ mv, const uint32_t* __restrict__ mtop,
+                                                   const int* __restrict__ abr, int4* __restrict__ runs, int lc, int os,
+                                                   int oe, int* __restrict__ err, unsigned long long* __restrict__ pbuf) {
+  const int lane = threadIdx.x & 63;
+  const int jid = wave_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (jid >= njobs) return;
   const TbJob J = jobs[jid];
   const Geo g = make_geo(J.ly, J.lx, os, oe, abr[jid]);
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
   int4* R = runs + (long long)J.seq * lc;
-  int i = J.ly - 1, j = J.lx - 1;
+  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  int i = wave_uniform(J.ly - 1), j = wave_uniform(J.lx - 1);
   int curc = J.flip ? i : j;
   RunRec rr;
   rr.reset();
-  long long caddr = -1;
-  uint32_t cword = 0;
-  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  WordTile tile;
+  tile.init();
+  int topk = -1;
+  uint32_t topw = 0;
   long long np = 0;
+  int e = 0;
   for (;;) {
-    const uint8_t code = step_code(g, M, TP, i, j, caddr, cword);
-    if (P) P[1 + np++] = pack_step(i, j, code);
+    uint8_t code;
+    if (i == 0 && j == 0) code = g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
+    else if (g.ab >= 0 && j >= g.ab * ODBA_W) code = (i == g.ly - 1) ? ODBA_OPEN_RIGHT : 0;
+    else if (i == 0) code = g.os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
+    else {
+      const int k = j / ODBA_W, t = j - k * ODBA_W;
+      const int C = (k == g.K - 1) ? g.lastC : 16;
+      const int l = t / C, q = t - l * C;
+      if (g.oe && g.ly >= 2 && i == g.ly - 1) {
+        if (k != topk) { topk = k; topw = TP[(long long)k * 64 + lane]; }
+        const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
+        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
+      } else {
+        const uint32_t c = (tile.get(M, g.S64, g.r0, k, i, l, lane) >> (2 * q)) & 3u;
+        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
+      }
+    }
+    if (P && lane == 0) P[1 + np] = pack_step(i, j, code);
+    np++;
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != curc) {
-      R[curc] = rr.out();
+      if (lane == 0) R[curc] = rr.out();
       curc = c;
       rr.reset();
     }
     const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);  // dba.hpp:395-415
-    rr.add(counted, sidx, err);
+    if (!rr.add(counted, sidx)) e |= 4;
     if (corner) break;
     if (code == ODBA_DIAGONAL) { i--; j--; }
     else if (code == ODBA_UP) { i--; }
     else if (code == ODBA_RIGHT || code == ODBA_OPEN_RIGHT) { j--; }
-    else { atomicOr(err, 1); break; }
-    if (i < 0 || j < 0) { atomicOr(err, 1); break; }
+    else { e |= 1; break; }
+    if (i < 0 || j < 0) { e |= 1; break; }
   }
-  R[curc] = rr.out();
-  if (P) P[0] = (unsigned long long)np;
+  if (lane == 0) {
+    R[curc] = rr.out();
+    if (P) P[0] = (unsigned long long)np;
+    if (e) atomicOr(err, e);
+  }
 }
 
 // Stripe-mode traceback for stripe k = K-1-m of every job (updateCentroid
-// with column_offset / stripe_rows, dba.hpp:374-423 launched at 733): walk
-// from the carried cell until the path leaves the stripe on the left (the
-// new height is the row reached + 1) or reaches the (0,0) anchor.
-__global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m, const uint32_t* __restrict__ mv,
-                                   const uint32_t* __restrict__ mtop, StripeState* __restrict__ ss,
-                                   int4* __restrict__ runs, int lc, int os, int oe, int* __restrict__ err,
-                                   unsigned long long* __restrict__ pbuf) {
-  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+// with column_offset / stripe_rows, dba.hpp:374-423 launched at 733), one
+// wavefront per job: walk from the carried cell until the path leaves the
+// stripe on the left (the new height is the row reached + 1) or reaches the
+// (0,0) anchor.
+__global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m,
+                                                          const uint32_t* __restrict__ mv,
+                                                          const uint32_t* __restrict__ mtop,
+                                                          StripeState* __restrict__ ss, int4* __restrict__ runs,
+                                                          int lc, int os, int oe, int* __restrict__ err,
+                                                          unsigned long long* __restrict__ pbuf) {
+  const int lane = threadIdx.x & 63;
+  const int jid = wave_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (jid >= njobs) return;
   const TbJob J = jobs[jid];
   StripeState S = ss[jid];
@@ -152,14 +213,16 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
   int4* R = runs + (long long)J.seq * lc;
-  int i = S.i, j = S.j;
+  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  int i = wave_uniform(S.i), j = wave_uniform(S.j);
   RunRec rr;
   rr.lo1 = S.lo; rr.hi1 = S.hi; rr.lo2 = S.lo2; rr.hi2 = S.hi2; rr.gap = S.gap;
   if (S.curc < 0) { S.curc = J.flip ? i : j; rr.reset(); }
-  long long caddr = -1;
-  uint32_t cword = 0;
+  WordTile tile;
+  tile.init();
+  const uint32_t topw = TP[lane];
   bool done = false;
-  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  int e = 0;
   while (j >= left) {
     uint8_t code;
     if (i == 0 && j == 0) code = os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
@@ -167,48 +230,48 @@ __global__ void k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, in
     else if (i == 0) code = os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
     else {
       const int t = j - left;
-      const int lane = t / C, q = t - lane * C;
+      const int l = t / C, q = t - l * C;
       if (oe && h >= 2 && i == h - 1) {
-        const uint32_t c = (TP[lane] >> (2 * q)) & 3u;
+        const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
       } else {
-        const long long addr = ((long long)(i - r0) + lane) * 64 + lane;
-        uint32_t w;
-        if (addr == caddr) w = cword;
-        else { w = M[addr]; caddr = addr; cword = w; }
-        const uint32_t c = (w >> (2 * q)) & 3u;
+        const uint32_t c = (tile.get(M, 0, r0, 0, i, l, lane) >> (2 * q)) & 3u;
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
       }
     }
-    if (P) P[1 + S.npath++] = pack_step(i, j, code);
+    if (P && lane == 0) P[1 + S.npath] = pack_step(i, j, code);
+    S.npath++;
     const int c = J.flip ? i : j;
     const int sidx = J.flip ? j : i;
     if (c != S.curc) {
-      R[S.curc] = rr.out();
+      if (lane == 0) R[S.curc] = rr.out();
       S.curc = c;
       rr.reset();
     }
     const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);
-    rr.add(counted, sidx, err);
+    if (!rr.add(counted, sidx)) e |= 4;
     if (corner) { done = true; break; }
     if (code == ODBA_DIAGONAL) { i--; j--; }
     else if (code == ODBA_UP) { i--; }
     else { j--; }
-    if (i < 0) { S.err = 1; atomicOr(err, 1); break; }
+    if (i < 0) { S.err = 1; e |= 1; break; }
   }
   if (done) {
-    R[S.curc] = rr.out();
+    if (lane == 0) R[S.curc] = rr.out();
     S.done = 1;
-    if (P) P[0] = (unsigned long long)S.npath;
+    if (P && lane == 0) P[0] = (unsigned long long)S.npath;
   } else if (!S.err) {
-    if (j != left - 1) { S.err = 1; atomicOr(err, 1); }
+    if (j != left - 1) { S.err = 1; e |= 1; }
     S.h = i + 1;
   }
   S.i = i;
   S.j = j;
   S.lo = rr.lo1; S.hi = rr.hi1; S.lo2 = rr.lo2; S.hi2 = rr.hi2; S.gap = rr.gap;
-  ss[jid] = S;
+  if (lane == 0) {
+    ss[jid] = S;
+    if (e) atomicOr(err, e);
+  }
 }
 
 template <typename T>
@@ -219,16 +282,35 @@ __global__ void k_accumulate(const T* __restrict__ data, const long long* __rest
   if (c >= lc) return;
   T a = first ? T(0) : acc[c];
   uint32_t n = first ? 0u : counts[c];
-  for (int s = 0; s < nseq; s++) {
-    const int4 r = runs[(long long)s * lc + c];
-    const T* p = data + offs[s];
-    if (r.y >= r.x) {
-      for (int k = r.y; k >= r.x; k--) a = (T)(a + p[k]);
-      n += (uint32_t)(r.y - r.x + 1);
+  // Eight sequences at a time: their run records and the first value of
+  // each run are loaded together; the additions stay in sequence order.
+  constexpr int B = 8;
+  for (int s0 = 0; s0 < nseq; s0 += B) {
+    int4 r[B];
+    const T* p[B];
+    T f1[B], f2[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      r[u] = (s0 + u < nseq) ? runs[(long long)(s0 + u) * lc + c] : make_int4(1, 0, 1, 0);
+      p[u] = data + ((s0 + u < nseq) ? offs[s0 + u] : 0);
     }
-    if (r.w >= r.z) {
-      for (int k = r.w; k >= r.z; k--) a = (T)(a + p[k]);
-      n += (uint32_t)(r.w - r.z + 1);
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      f1[u] = (r[u].y >= r[u].x) ? p[u][r[u].y] : T(0);
+      f2[u] = (r[u].w >= r[u].z) ? p[u][r[u].w] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if (r[u].y >= r[u].x) {
+        a = (T)(a + f1[u]);
+        for (int k = r[u].y - 1; k >= r[u].x; k--) a = (T)(a + p[u][k]);
+        n += (uint32_t)(r[u].y - r[u].x + 1);
+      }
+      if (r[u].w >= r[u].z) {
+        a = (T)(a + f2[u]);
+        for (int k = r[u].w - 1; k >= r[u].z; k--) a = (T)(a + p[u][k]);
+        n += (uint32_t)(r[u].w - r[u].z + 1);
+      }
     }
   }
   acc[c] = a;
@@ -380,7 +462,7 @@ __global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* 
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
                             const int* abr, int4* runs, int lc, int os, int oe, int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe,
+  hipLaunchKernelGGL(k_traceback, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe,
                      err, pbuf);
   return hipGetLastError();
 }
@@ -389,7 +471,7 @@ hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs,
                                    const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
                                    int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
+  hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
                      lc, os, oe, err, pbuf);
   return hipGetLastError();
 }

@@ -68,6 +68,13 @@ struct StripeState {
   int done, err;
 };
 
+// Several waves per pair (column strips 64*C wide, C in {1, 4, 16}; float,
+// int32 and uint32 whose costs cannot wrap). `waves` per workgroup <= 16;
+// esc holds 2*waves edge slots of `stride` elements per workgroup.
+hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
+                                  const void* data, const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, void* dist, void* esc, long long stride, int* flags);
+
 hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
                          const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
                          uint32_t* mtop, StripeState* ss);

@@ -230,6 +230,17 @@ template <typename T> struct SwathIO {
   T* eout;         // this swath's right edge column (null: not needed)
   uint32_t* mv;    // path words for this swath (null: distance only)
   uint32_t* mtop;  // path words of the open-end top row (64 per swath)
+  int ref_left;    // 1: `off` is a reference swath start (a multiple of 1024)
+};
+
+// Pipeline hooks of run_swath. A wave that sweeps a whole pair needs none.
+// A wave that sweeps one column strip of a pair shared by several waves
+// waits until its left neighbour has published the edge rows it is about to
+// read, and publishes its own edge rows as they are flushed.
+struct NoPipe {
+  static constexpr bool kPrefetch = true;  // load the next block's edge values one block early
+  __device__ void wait(int) const {}
+  __device__ void publish(int) const {}
 };
 
 template <typename T> struct SwathRes {
@@ -240,8 +251,8 @@ template <typename T> struct SwathRes {
 
 // Runs one swath of C columns per lane. Rows r0..r1 go through the skewed
 // main loop; with open_end the top row goes through the prefix-min scan.
-template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH>
-__device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int lane) {
+template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH, class Pipe = NoPipe>
+__device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int lane, const Pipe& pipe = Pipe()) {
   const T MAXV = Ar<T>::maxv();
   const T RINF = Ar<T>::rowinf();
   const int ly = a.ly, off = a.off;
@@ -260,11 +271,12 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     x[q] = a.x[col0 + q];  // padded: always in bounds
     D[q] = OS ? T(0) : RINF;
   }
-  // diagonal input of the lane's first row
+  // diagonal input of the lane's first row (with open start, row 0 of every
+  // edge column is 0)
   T diag_in;
   if (lane == 0) {
     if (first_swath) diag_in = OS ? MAXV : T(0);
-    else diag_in = OS ? a.ein[0] : RINF;
+    else diag_in = OS ? T(0) : RINF;
   } else {
     diag_in = OS ? T(0) : RINF;
   }
@@ -279,18 +291,27 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     int r = r0 + 64 * b + lane;
     return a.ein[r];  // edge buffers are padded by 128 entries
   };
+  auto rows_of_block = [&](int b) -> int { return min(r0 + 64 * b + 64, r1 + 1); };
   T ynext = nblk > 0 ? load_y(0) : T(0);
-  T enext = nblk > 0 ? load_e(0) : T(0);
+  T enext = T(0);
+  if (Pipe::kPrefetch && nblk > 0) enext = load_e(0);
 
   for (int b = 0; b < nblk; b++) {
     // --- stage the window for block b
+    if (!Pipe::kPrefetch) {
+      if (!first_swath) pipe.wait(rows_of_block(b));
+      enext = load_e(b);
+    }
     T yshift = W->y[64 + lane];
     wave_sync();
     W->y[lane] = yshift;
     W->y[64 + lane] = ynext;
     W->e[lane] = enext;
     wave_sync();
-    if (b + 1 < nblk) { ynext = load_y(b + 1); enext = load_e(b + 1); }
+    if (b + 1 < nblk) {
+      ynext = load_y(b + 1);
+      if (Pipe::kPrefetch) enext = load_e(b + 1);
+    }
 
     const int sbase = 64 * b;
     const bool steady = (sbase >= 63) && (sbase + 63 < nrows);
@@ -329,6 +350,9 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
         W->o[u - lane + 64] = D[C - 1];
       }
     } else {
+      // Skew prologue / epilogue: lanes outside their rows keep their state
+      // through selects, so the block stays straight-line and unrolled.
+#pragma unroll 8
       for (int u = 0; u < 64; u++) {
         const int s = sbase + u;
         T yv = W->y[u - lane + 64];
@@ -336,32 +360,30 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
         T left_in = Ar<T>::shr(ev, D[C - 1]);
         const int row = r0 + s - lane;
         const bool active = (unsigned)(s - lane) < (unsigned)nrows;
-        if (active) {
-          T dg = diag_in;
-          diag_in = left_in;
-          T l = left_in;
-          uint32_t word = 0;
+        T dg = diag_in;
+        diag_in = active ? left_in : diag_in;
+        T l = left_in;
+        uint32_t word = 0;
 #pragma unroll
-          for (int q = 0; q < C; q++) {
-            T up = D[q];
-            T df = yv - x[q];
-            T dd = df * df;
-            uint32_t code = 0;
-            T v;
-            if (MINFIRST || Ar<T>::kFloat) {
-              v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
-            } else {
-              CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
-              T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
-              v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
-            }
-            if (PATH) word |= code << (2 * q);
-            dg = up;
-            D[q] = v;
-            l = v;
+        for (int q = 0; q < C; q++) {
+          T up = D[q];
+          T df = yv - x[q];
+          T dd = df * df;
+          uint32_t code = 0;
+          T v;
+          if (MINFIRST || Ar<T>::kFloat) {
+            v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
+          } else {
+            CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
+            T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
+            v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
           }
-          if (PATH) a.mv[(size_t)s * 64 + lane] = word;
+          if (PATH) word |= code << (2 * q);
+          dg = up;
+          D[q] = active ? v : up;
+          l = v;
         }
+        if (PATH && active) a.mv[(size_t)s * 64 + lane] = word;
         W->o[u - lane + 64] = D[C - 1];
       }
     }
@@ -374,6 +396,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
         a.eout[row] = v;
         runmin = Ar<T>::mn(runmin, v);
       }
+      pipe.publish(min(r0 + sbase, r1 + 1));
       T sh = W->o[64 + lane];
       wave_sync();
       W->o[lane] = sh;
@@ -388,6 +411,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
       runmin = Ar<T>::mn(runmin, v);
     }
   }
+  if (edge_out) pipe.publish(r1 + 1);
   if (OS && edge_out && lane == 0) a.eout[0] = T(0);
   if (OS) runmin = Ar<T>::mn(runmin, T(0));
 
@@ -396,12 +420,15 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
   if (OE && ly >= 2) {
     const int t = ly - 1;
     const T yv = a.y[t];
+    if (!first_swath) pipe.wait(ly);
     const T lane0_diag = first_swath ? MAXV : a.ein[t - 1];
     const T left0 = first_swath ? MAXV : a.ein[t];
     const T nb = Ar<T>::shr(lane0_diag, D[C - 1]);
     // pass 1: m[q] = running min over the lane of min(U, G); U = up + dd,
-    // G = diag + dd except at the swath-left column where the reference adds
-    // no local cost to the diagonal (and at column 0 the diagonal is MAX).
+    // G = diag + dd except at a reference swath-left column where the
+    // reference adds no local cost to the diagonal (and at column 0 the
+    // diagonal is MAX).
+    const bool bare0 = (lane == 0) && (a.ref_left != 0);
     T m[C];
     T run = T(0);
 #pragma unroll
@@ -409,7 +436,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
       const T df = yv - x[q];
       const T dd = df * df;
       const T U = D[q] + dd;
-      const T G = (q == 0 && lane == 0) ? nb : ((q == 0 ? nb : D[q - 1]) + dd);
+      const T G = (q == 0 && bare0) ? nb : ((q == 0 ? nb : D[q - 1]) + dd);
       const T mm = (G > U) ? U : G;
       run = (q == 0) ? mm : Ar<T>::mn(run, mm);
       m[q] = run;
@@ -423,7 +450,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
       const T df = yv - x[q];
       const T dd = df * df;
       const T U = D[q] + dd;
-      const T G = (q == 0 && lane == 0) ? nb : (dprev_old + dd);
+      const T G = (q == 0 && bare0) ? nb : (dprev_old + dd);
       const T v = Ar<T>::mn(pre, m[q]);
       if (PATH) word |= ((G == v) ? 0u : ((U == v) ? 2u : 1u)) << (2 * q);
       dprev_old = D[q];
@@ -434,6 +461,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
       T top = __shfl(D[C - 1], 63, 64);
       if (lane == 0) a.eout[t] = top;
       runmin = Ar<T>::mn(runmin, top);
+      pipe.publish(ly);
     }
   }
   if (edge_out) {
@@ -534,6 +562,7 @@ __device__ T dtw_one_pair(const T* y, int ly, const T* x, int lx, T* ebuf0, T* e
     io.eout = last ? nullptr : ((k & 1) ? ebuf1 : ebuf0);
     io.mv = PATH ? mvbase + (size_t)k * swath_words : nullptr;
     io.mtop = PATH ? mtop + (size_t)k * 64 : nullptr;
+    io.ref_left = 1;
     const int C = last ? last_swath_c(wcols) : 16;
     SwathRes<T> r = swath_dispatch<T, OS, OE, MINFIRST, PATH>(C, io, W, lane);
     if (!last) { colmin = r.colmin; top = r.top; }
@@ -597,6 +626,137 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesP
       long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
       dist[idx] = d;
       if (flags) flags[idx] = (po.abrogated >= 0 ? (1 | (po.abrogated << 8)) : 0) | (po.q3 ? 2 : 0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// All-vs-all with a workgroup of NW waves per pair, for workloads with too
+// few pairs to give every SIMD its own waves (a few hundred short pairs, or
+// a few very long ones). The pair's columns are cut into strips 64*C wide;
+// wave w sweeps strips w, w+NW, w+2NW, ... top to bottom, one strip a few
+// 64-row blocks behind its left neighbour. Strip s reads the right-edge
+// column of strip s-1 from a global edge slot; the slot's owner and the
+// number of rows published so far are LDS words, so the hand-off costs one
+// LDS poll per 64-row block. Slots are reused round-robin (2*NW of them); a
+// producer claims a slot only after the previous owner's consumer released
+// it. Waiting only ever goes to a lower strip index, which is already
+// running or done, so the pipeline cannot deadlock.
+//
+// The open-end early stop (dtw.hpp:74-129) is not taken: every strip is
+// swept, and the stop is evaluated afterwards at each reference swath
+// start. This gives the same distance: when the column minimum at a swath
+// boundary equals its top value, every later cell is at least that minimum
+// (costs only grow by non-negative terms; min-first types only), while the
+// free right moves of the open-end top row carry exactly that value to the
+// last column. The flags report the swath the reference stops at.
+// ---------------------------------------------------------------------------
+struct StripPipe {
+  static constexpr bool kPrefetch = false;
+  int* in_owner;  // LDS: strip whose edge is in the input slot
+  int* in_cnt;    // LDS: edge rows published into the input slot
+  int in_strip;   // the left neighbour's strip index
+  int* out_cnt;   // LDS: rows published of this strip's edge (null: none)
+  int lane;
+  __device__ void wait(int rows) const {
+    for (;;) {
+      const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (o == in_strip && c >= rows) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __device__ void publish(int rows) const {
+    if (!out_cnt) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(out_cnt, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+};
+
+constexpr int kMaxStripWaves = 16;
+
+template <typename T, int C, bool OS, bool OE>
+__global__ void __launch_bounds__(1024) k_all_pairs_strips(const T* __restrict__ data, const long long* __restrict__ offs,
+                                                           const int* __restrict__ lens, int n, long long npairs,
+                                                           unsigned long long* __restrict__ counter,
+                                                           T* __restrict__ dist, T* __restrict__ escratch,
+                                                           long long stride, int* __restrict__ flags) {
+  __shared__ T s_y[kMaxStripWaves][128];
+  __shared__ T s_e[kMaxStripWaves][64];
+  __shared__ T s_o[kMaxStripWaves][128];
+  __shared__ int s_owner[2 * kMaxStripWaves];
+  __shared__ int s_cnt[2 * kMaxStripWaves];
+  __shared__ int s_free[2 * kMaxStripWaves];
+  __shared__ unsigned long long s_p;
+  __shared__ int s_abr;
+  __shared__ T s_dist;
+  constexpr int SW = 64 * C;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int nw = uniform_i32((int)(blockDim.x >> 6));
+  const int nslot = 2 * nw;
+  T* ebase = escratch + (long long)blockIdx.x * nslot * stride;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous pair
+    if (threadIdx.x == 0) {
+      s_p = atomicAdd(counter, 1ull);
+      s_abr = INT_MAX;
+    }
+    if (threadIdx.x < 2 * kMaxStripWaves) {
+      s_owner[threadIdx.x] = -1;
+      s_cnt[threadIdx.x] = 0;
+      s_free[threadIdx.x] = 1;
+    }
+    __syncthreads();
+    const long long p = (long long)uniform_u64(s_p);
+    if (p >= npairs) break;
+    int r, c;
+    decode_pair(p, npairs, r, c);
+    const T* y = data + offs[r];
+    const T* x = data + offs[c];
+    const int ly = lens[r], lx = lens[c];
+    const int S = (lx + SW - 1) / SW;
+    for (int s = wid; s < S; s += nw) {
+      const int off = s * SW;
+      const int so = s % nslot, si = (s + nslot - 1) % nslot;
+      const bool has_out = (s < S - 1);
+      if (has_out) {  // claim the output slot
+        for (;;) {
+          const int f = uniform_i32(__hip_atomic_load(&s_free[so], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+          if (f) break;
+          __builtin_amdgcn_s_sleep(2);
+        }
+        if (lane == 0) {
+          __hip_atomic_store(&s_free[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(&s_cnt[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(&s_owner[so], s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+      StripPipe pp{&s_owner[si], &s_cnt[si], s - 1, has_out ? &s_cnt[so] : nullptr, lane};
+      SwathIO<T> io;
+      io.y = y; io.ly = ly; io.x = x; io.lx = lx; io.off = off;
+      io.ein = s > 0 ? ebase + (long long)si * stride : nullptr;
+      io.eout = has_out ? ebase + (long long)so * stride : nullptr;
+      io.mv = nullptr; io.mtop = nullptr;
+      io.ref_left = (off % ODBA_W) == 0;
+      SwathRes<T> res = run_swath<T, C, OS, OE, true, false, StripPipe>(io, &win, lane, pp);
+      if (s > 0) {  // the input slot is consumed
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&s_free[si], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (OE && !OS && has_out) {
+        const int next = off + SW;
+        if ((next % ODBA_W) == 0 && next > ly && res.top == res.colmin && lane == 0) atomicMin(&s_abr, next / ODBA_W);
+      }
+      if (!has_out && lane == 0) s_dist = pair_distance<T>(res.last, ly, OE != OS);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
+      dist[idx] = s_dist;
+      if (flags) flags[idx] = (s_abr != INT_MAX) ? (1 | (s_abr << 8)) : 0;
     }
   }
 }
@@ -680,7 +840,7 @@ __global__ void __launch_bounds__(256) k_stripe_forward(const StripeJob<T>* __re
       io.y = jb.y; io.ly = jb.H; io.x = jb.x; io.lx = jb.LX; io.off = off;
       io.ein = k > 0 ? sl + (long long)(k - 1) * (jb.H + 256) : nullptr;
       io.eout = sl + (long long)k * (jb.H + 256);
-      io.mv = nullptr; io.mtop = nullptr;
+      io.mv = nullptr; io.mtop = nullptr; io.ref_left = 1;
       SwathRes<T> r = run_swath<T, 16, OS, OE, MINFIRST, false>(io, &win, lane);
       colmin = r.colmin; top = r.top;
     }
@@ -739,7 +899,7 @@ __global__ void __launch_bounds__(256) k_stripe_step(const StripeJob<T>* __restr
     SwathIO<T> io;
     io.y = jb.y; io.ly = h; io.x = jb.x; io.lx = jb.LX; io.off = off;
     io.ein = prev; io.eout = nullptr;
-    io.mv = mv + jb.mv_off; io.mtop = mtop + jb.top_off;
+    io.mv = mv + jb.mv_off; io.mtop = mtop + jb.top_off; io.ref_left = 1;
     const int wcols = (k == K - 1) ? jb.LX - off : ODBA_W;
     const int C = (k == K - 1) ? last_swath_c(wcols) : 16;
     (void)swath_dispatch<T, OS, OE, MINFIRST, true>(C, io, &win, lane);
@@ -759,6 +919,28 @@ static hipError_t launch_all_pairs_t(int os, int oe, dim3 grid, hipStream_t st, 
   else if (oe) hipLaunchKernelGGL((k_all_pairs<T, false, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
   else hipLaunchKernelGGL((k_all_pairs<T, false, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
   return hipGetLastError();
+}
+
+template <typename T, int C>
+static hipError_t launch_strips_c(int os, int oe, dim3 grid, dim3 block, hipStream_t st, const T* data,
+                                  const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, T* dist, T* esc, long long stride, int* flags) {
+#define ODBA_STRIPS(A, B) hipLaunchKernelGGL((k_all_pairs_strips<T, C, A, B>), grid, block, 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags)
+  if (os && oe) ODBA_STRIPS(true, true); else if (os) ODBA_STRIPS(true, false); else if (oe) ODBA_STRIPS(false, true); else ODBA_STRIPS(false, false);
+#undef ODBA_STRIPS
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_strips_t(int C, int os, int oe, dim3 grid, dim3 block, hipStream_t st, const T* data,
+                                  const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, T* dist, T* esc, long long stride, int* flags) {
+  switch (C) {
+    case 1: return launch_strips_c<T, 1>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+    case 4: return launch_strips_c<T, 4>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+    case 16: return launch_strips_c<T, 16>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <typename T, bool MINFIRST>
@@ -789,6 +971,19 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
     case ODBA_U64:
       if (minfirst) return launch_all_pairs_t<uint64_t, true>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
       return launch_all_pairs_t<uint64_t, false>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
+                                  const void* data, const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, void* dist, void* esc, long long stride, int* flags) {
+  if (waves < 1 || waves > kMaxStripWaves) return hipErrorInvalidValue;
+  dim3 g(nblocks), b(64 * waves);
+  switch (dtype) {
+    case ODBA_F32: return launch_strips_t<float>(C, os, oe, g, b, st, (const float*)data, offs, lens, n, npairs, counter, (float*)dist, (float*)esc, stride, flags);
+    case ODBA_I32: return launch_strips_t<int32_t>(C, os, oe, g, b, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
+    case ODBA_U32: return launch_strips_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
   }
   return hipErrorInvalidValue;
 }
