@@ -152,13 +152,51 @@ def run_small32(args, N, WL):
     return line
 
 
+def run_serve(args, N, WL):
+    """Serving: a seeded Poisson trace (mean 20 jobs/s) of DBA jobs with
+    16-512 series (scaled by --scale), lengths log-uniform 500-50k (scaled),
+    float32 random walks or int16 squiggles as float, global or open-end;
+    each job is z-norm -> all-vs-all -> medoid -> DBA to convergence. The same
+    trace runs through the plain per-job path (one job after another) and
+    through the serving loop; jobs/s = jobs / makespan, latency = completion
+    - arrival."""
+    from opendba_amd import serve as S
+    jobs = WL.poisson_jobs(n_jobs=args.jobs, rate=20.0, seed=4, scale=args.scale)
+    cells = sum(WL.pair_cells([len(s) for s in j["seqs"]]) for j in jobs)
+    log = lambda tag: (lambda k, o: print(f"{tag} job {k}: latency {o['latency']:.3f} s rounds {o['rounds']}",
+                                          file=sys.stderr, flush=True))
+    plain = S.run_trace(jobs, "plain", max_rounds=args.max_rounds, progress=log("plain"))
+    served = S.run_trace(jobs, "served", slots=args.slots, max_rounds=args.max_rounds, progress=log("served"))
+    same = all(np.array_equal(a["centroid"], b["centroid"]) and a["medoid"] == b["medoid"]
+               for a, b in zip(plain["results"], served["results"]))
+    return {
+        "metric": METRIC, "value": round(served["jobs_per_s"], 4), "unit": "jobs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(served["makespan_s"] * 1000.0, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving: {args.jobs} Poisson jobs (mean 20/s, seed 4), 16-512 series and lengths "
+                               f"500-50k scaled by {args.scale}, f32 walks / int16 squiggles, global / open_end, "
+                               f"z-norm -> all-vs-all -> medoid -> DBA (max {args.max_rounds} rounds)",
+                   "slots": args.slots, "all_pairs_cells": cells},
+        "served": {"jobs_per_s": round(served["jobs_per_s"], 4), "p50_s": round(served["p50_s"], 4),
+                   "p99_s": round(served["p99_s"], 4), "makespan_s": round(served["makespan_s"], 3)},
+        "plain": {"jobs_per_s": round(plain["jobs_per_s"], 4), "p50_s": round(plain["p50_s"], 4),
+                  "p99_s": round(plain["p99_s"], 4), "makespan_s": round(plain["makespan_s"], 3)},
+        "speedup_jobs_per_s": round(served["jobs_per_s"] / plain["jobs_per_s"], 3),
+        "results_identical": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--workload", choices=["squiggles", "small32"], default="squiggles")
+    ap.add_argument("--workload", choices=["squiggles", "small32", "serve"], default="squiggles")
+    ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
+    ap.add_argument("--scale", type=float, default=0.5, help="serve: series-count and length scale")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent job slots")
+    ap.add_argument("--max-rounds", type=int, default=250, help="serve: DBA round limit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -171,6 +209,9 @@ def main():
 
     if args.workload == "small32":
         print(json.dumps(run_small32(args, N, WL)))
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args, N, WL)))
         return
     raw = WL.squiggles(args.n, 2000, 20000, seed=1)
     seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))

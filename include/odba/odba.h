@@ -34,6 +34,11 @@ int odba_ctx_destroy(odba_ctx* ctx);
 const char* odba_last_error(const odba_ctx* ctx);
 /* Compute units, peak engine clock (kHz), device memory (bytes), name. */
 int odba_device_info(odba_ctx* ctx, int* compute_units, int* clock_khz, uint64_t* mem_bytes, char* name, int name_len);
+/* Device memory this context plans its DBA working set against (default:
+ * all of it). No counterpart in the reference, which plans against the
+ * free memory of the whole device (dba.hpp:523-548); contexts that share a
+ * device (serving slots) each get a share. */
+int odba_ctx_set_mem_cap(odba_ctx* ctx, uint64_t bytes);
 
 /*
  * All-vs-all DTW distances in the given order: pair (r, c > r) aligns
@@ -155,6 +160,28 @@ int odba_kseg(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t*
               uint64_t* out_offs);
 /* Upper bound on the total number of segments odba_kseg can produce. */
 uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length);
+
+/*
+ * Serving (no counterpart in the reference, which runs one job per process):
+ * DBA jobs - the performDBA pipeline for one cluster, as odba_perform_dba
+ * with the full path matrix - are submitted at any time, admitted in
+ * arrival order when one of `slots` worker contexts is free and the job's
+ * estimated device working set fits in what is left of hbm_budget (0: 80%
+ * of device memory), and run concurrently, each on its own HIP stream.
+ * A job's result is the same alone and in a mix. Times are seconds since
+ * the server was created: times[0] submit, [1] admission, [2] completion.
+ */
+typedef struct odba_server odba_server;
+int odba_server_create(int device, uint64_t hbm_budget, int slots, odba_server** out);
+int odba_server_destroy(odba_server* server);  /* waits for running jobs; queued ones fail */
+int odba_server_submit(odba_server* server, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                       int use_open_start, int use_open_end, int norm, int max_rounds, uint64_t* job_id);
+/* Blocks until the job is done or failed (timeout_s < 0: no limit). */
+int odba_server_wait(odba_server* server, uint64_t job_id, double timeout_s, int* state);
+int odba_server_result(odba_server* server, uint64_t job_id, void* centroid_out, uint64_t capacity,
+                       uint64_t* centroid_len, int* medoid_sorted, int* rounds, double* times, int* job_rc);
+int odba_server_release(odba_server* server, uint64_t job_id);
+int odba_server_stats(odba_server* server, uint64_t* bytes_in_use, uint64_t* budget, int* running, int* queued);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,12 @@
                                    the reference divides by zero (dba.hpp:821) */
 #define ODBA_E_NO_DEVICE 105    /* no HIP device or the device index is invalid */
 #define ODBA_E_UNSUPPORTED 106  /* a mode this build rejects explicitly         */
-#define ODBA_E_BUSY 107         /* serving queue full                           */
+#define ODBA_E_BUSY 107         /* serving: job not finished, or server stopping */
+
+/* Serving job states (odba_server_wait). */
+#define ODBA_JOB_QUEUED 0
+#define ODBA_JOB_RUNNING 1
+#define ODBA_JOB_DONE 2
+#define ODBA_JOB_FAILED 3
 
 #endif

@@ -24,6 +24,7 @@ struct odba_ctx {
   int cus = 0;
   int clock_khz = 0;
   uint64_t mem = 0;
+  uint64_t mem_cap = 0;   // device memory this context may plan for (serving slots get a share)
   std::string name;
   std::string err;
   // grow-only device scratch
@@ -131,6 +132,7 @@ int odba_ctx_create(int device, odba_ctx** out) {
   c->cus = p.multiProcessorCount;
   c->clock_khz = p.clockRate;
   c->mem = p.totalGlobalMem;
+  c->mem_cap = c->mem;
   c->name = p.gcnArchName;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ODBA_E_HIP; }
   if (hipMalloc(&c->small, 256) != hipSuccess) { hipStreamDestroy(c->stream); delete c; return ODBA_E_NOMEM; }
@@ -256,6 +258,12 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, long long npairs
 
 extern "C" {
 
+int odba_ctx_set_mem_cap(odba_ctx* c, uint64_t bytes) {
+  if (!c || bytes == 0) return ODBA_E_INVALID;
+  c->mem_cap = std::min<uint64_t>(bytes, c->mem);
+  return ODBA_OK;
+}
+
 int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, void* dist_out, int* flags_out,
                                 double* kernel_ms) {
   if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
@@ -343,17 +351,35 @@ void plan_job(int ly, int lx, int os, int oe, long long& mv_words, long long& to
   top_words += (long long)K * 64;
 }
 
+// Path jobs: one wave per job, or (float, few long jobs) a workgroup of
+// waves per job sweeping its swaths as a pipeline. ODBA_PATH_SCHEDULE =
+// wave|strip overrides the choice.
 int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
-                  uint32_t* mv, uint32_t* mtop, int* abr) {
+                  uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0) {
   const size_t es = dtype_size(dtype);
-  const int nblocks = grid_blocks(c, njobs);
   const long long stride = maxly + 256;
-  int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
-  if (rc) return rc;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
   HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned int), c->stream));
-  hipError_t e = launch_path_jobs(dtype, minfirst, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr,
-                                  c->esc, stride);
+  const char* env = std::getenv("ODBA_PATH_SCHEDULE");
+  bool strips = njobs < 4 * c->cus && maxlx >= 2 * ODBA_W && minly >= 64;
+  if (env && !std::strcmp(env, "wave")) strips = false;
+  if (env && !std::strcmp(env, "strip")) strips = maxlx > 0 && minly >= 64;
+  strips = strips && dtype == ODBA_F32;
+  hipError_t e;
+  if (strips) {
+    const int waves = std::min(16, (maxlx + ODBA_W - 1) / ODBA_W);
+    const int per_cu = std::max(1, 16 / waves);
+    const int nblocks = std::max(1, std::min(njobs, per_cu * c->cus));
+    int rc = ensure_scratch(c, (size_t)nblocks * 2 * waves * stride * es);
+    if (rc) return rc;
+    e = launch_path_strips(waves, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr, c->esc, stride);
+  } else {
+    const int nblocks = grid_blocks(c, njobs);
+    int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
+    if (rc) return rc;
+    e = launch_path_jobs(dtype, minfirst, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr, c->esc,
+                         stride);
+  }
   if (e != hipSuccess) return fail(c, ODBA_E_HIP, "path launch: %s", hipGetErrorString(e));
   return ODBA_OK;
 }
@@ -384,7 +410,7 @@ int dtw_path_t(odba_ctx* c, const T* y, int ly, const T* x, int lx, int os, int 
   bool minfirst = int_cannot_wrap<T>(sp.data(), sl.data(), 2);
   int rc = run_path_jobs(c, std::is_same<T, float>::value ? ODBA_F32 : std::is_same<T, double>::value ? ODBA_F64
                               : std::is_same<T, int32_t>::value ? ODBA_I32 : std::is_same<T, uint32_t>::value ? ODBA_U32 : ODBA_U64,
-                         minfirst, os, oe, jd, 1, ly, mv, mtop, abr);
+                         minfirst, os, oe, jd, 1, ly, mv, mtop, abr, lx, ly);
   if (rc) return rc;
   int ab = -1;
   HIPC(c, hipMemcpyAsync(&ab, abr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -472,10 +498,40 @@ template <typename T> struct DbaState {
   std::vector<long long> h_poff;
 
   ~DbaState() {
+    if (!own_seqs) seqdata = nullptr, offs = nullptr;
     for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
                     (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf})
       if (p) hipFree(p);
+  }
+
+  bool own_seqs = true;
+
+  // Members already resident on the device (padded layout, as a seqset):
+  // the session reads them in place and does not free them.
+  int init_shared(odba_ctx* ctx, T* d, long long* doffs, const std::vector<long long>& ho,
+                  const std::vector<int>& hl, bool no_wrap, int os_, int oe_) {
+    c = ctx; n = (int)hl.size(); os = os_; oe = oe_;
+    minfirst = no_wrap;
+    own_seqs = false;
+    seqdata = d;
+    offs = doffs;
+    h_offs = ho;
+    h_lens = hl;
+    for (int L : hl) maxlen = std::max(maxlen, L);
+    return alloc_common();
+  }
+
+  int alloc_common() {
+    HIPC(c, hipMalloc((void**)&abr, n * sizeof(int)));
+    HIPC(c, hipMalloc(&jobs, n * sizeof(PJ<T>)));
+    HIPC(c, hipMalloc((void**)&tbj, n * sizeof(TbJob)));
+    HIPC(c, hipMalloc((void**)&err, sizeof(int)));
+    HIPC(c, hipMalloc((void**)&dbits, sizeof(unsigned long long)));
+    // move-word budget: a quarter of the context's memory, at least one job
+    budget_words = (long long)(c->mem_cap / 4 / 4);
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return ODBA_OK;
   }
 
   int init(odba_ctx* ctx, const T* const* seqs, const uint64_t* lens, int nn, int os_, int oe_) {
@@ -496,15 +552,7 @@ template <typename T> struct DbaState {
     HIPC(c, hipMalloc((void**)&offs, n * sizeof(long long)));
     HIPC(c, hipMemcpyAsync(seqdata, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(offs, h_offs.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMalloc((void**)&abr, n * sizeof(int)));
-    HIPC(c, hipMalloc(&jobs, n * sizeof(PJ<T>)));
-    HIPC(c, hipMalloc((void**)&tbj, n * sizeof(TbJob)));
-    HIPC(c, hipMalloc((void**)&err, sizeof(int)));
-    HIPC(c, hipMalloc((void**)&dbits, sizeof(unsigned long long)));
-    // move-word budget: a quarter of device memory, at least one job
-    budget_words = (long long)(c->mem / 4 / 4);
-    HIPC(c, hipStreamSynchronize(c->stream));
-    return ODBA_OK;
+    return alloc_common();
   }
 
   int set_centroid(const T* h, int len) {
@@ -638,7 +686,7 @@ template <typename T> struct DbaState {
       std::vector<TbJob> tb;
       std::vector<int> stripe_idx;
       long long mvw = 0, tw = 0, sbytes = 0;
-      int maxly = 1;
+      int maxly = 1, maxlx = 0, minly = INT32_MAX;
       int b1 = b0;
       while (b1 < n) {
         const int ls = h_lens[b1];
@@ -662,6 +710,8 @@ template <typename T> struct DbaState {
           t.path_off = rec ? h_poff[b1] : -1;
           tb.push_back(t);
           maxly = std::max(maxly, ly);
+          maxlx = std::max(maxlx, lx);
+          minly = std::min(minly, ly);
         }
         b1++;
       }
@@ -673,7 +723,7 @@ template <typename T> struct DbaState {
         if (rc) return rc;
         HIPC(c, hipMemcpyAsync(jobs, pj.data(), nfull * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
         HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
-        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr);
+        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly);
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
@@ -807,6 +857,9 @@ template <typename T> int medoid_single(const T* d, const std::vector<uint64_t>&
   return 0;
 }
 
+// Device-resident pipeline of performDBA for one cluster: the sanitized,
+// sorted members are packed and uploaded once, z-normalised in place, and
+// that one buffer feeds the all-vs-all and every DBA round.
 template <typename T>
 int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
                   int max_rounds, int policy, uint64_t free_mem, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
@@ -818,30 +871,65 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   std::vector<int> perm(n);
   for (int i = 0; i < n; i++) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return lens[a] < lens[b]; });  // dba.hpp:876-885
-  std::vector<std::vector<T>> s(n);
   std::vector<uint64_t> l(n);
-  for (int i = 0; i < n; i++) { l[i] = lens[perm[i]]; s[i].assign(in[perm[i]], in[perm[i]] + l[i]); }
-  std::vector<T*> sp(n);
-  for (int i = 0; i < n; i++) sp[i] = s[i].data();
-  std::vector<double> mu(n, 0.0), sg(n, 1.0);
-  int rc;
-  if (norm) {
-    rc = znorm_t<T>(c, sp.data(), l.data(), n, mu.data(), sg.data());
-    if (rc) return rc;
+  std::vector<long long> ho(n);
+  std::vector<int> hl(n);
+  long long total = 0;
+  int maxlen = 0;
+  for (int i = 0; i < n; i++) {
+    l[i] = lens[perm[i]];
+    if (l[i] == 0 || l[i] > (uint64_t)INT32_MAX / 2) return fail(c, ODBA_E_INVALID, "sequence %d has length %llu", perm[i], (unsigned long long)l[i]);
+    ho[i] = total;
+    hl[i] = (int)l[i];
+    maxlen = std::max(maxlen, hl[i]);
+    total += padded_len((long long)l[i]);
   }
+  std::vector<T> hb((size_t)total, T(0));
+  for (int i = 0; i < n; i++) std::memcpy(&hb[ho[i]], in[perm[i]], l[i] * sizeof(T));
+  struct Buf {
+    T* d = nullptr; long long* o = nullptr; int* ln = nullptr; double* mu = nullptr; double* sg = nullptr;
+    ~Buf() { for (void* p : {(void*)d, (void*)o, (void*)ln, (void*)mu, (void*)sg}) if (p) hipFree(p); }
+  } B;
+  HIPC(c, hipMalloc((void**)&B.d, hb.size() * sizeof(T)));
+  HIPC(c, hipMalloc((void**)&B.o, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&B.ln, n * sizeof(int)));
+  HIPC(c, hipMemcpyAsync(B.d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(B.o, ho.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(B.ln, hl.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> mu(n, 0.0), sg(n, 1.0);
+  if (norm) {
+    HIPC(c, hipMalloc((void**)&B.mu, n * sizeof(double)));
+    HIPC(c, hipMalloc((void**)&B.sg, n * sizeof(double)));
+    HIPC(c, launch_znorm(dtype_of<T>(), c->stream, B.d, B.o, B.ln, n, B.mu, B.sg));
+    HIPC(c, hipMemcpyAsync(mu.data(), B.mu, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(sg.data(), B.sg, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  bool no_wrap = true;
+  if (!is_float<T>()) {  // the wrap bound needs the (normalised) values
+    if (norm) HIPC(c, hipMemcpyAsync(hb.data(), B.d, hb.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    std::vector<const T*> sp(n);
+    for (int i = 0; i < n; i++) sp[i] = &hb[ho[i]];
+    no_wrap = int_cannot_wrap<T>(sp.data(), l.data(), n);
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  // the all-vs-all on the resident buffer (a seqset view that owns nothing)
+  odba_seqset view;
+  view.ctx = c; view.dtype = dtype_of<T>(); view.n = n; view.data = B.d; view.offs = B.o; view.lens = B.ln;
+  view.h_offs = ho; view.h_lens = hl; view.maxlen = maxlen; view.int_no_wrap = no_wrap;
   std::vector<T> d((size_t)n * (n - 1) / 2);
-  rc = odba_dtw_all_pairs(c, dtype_of<T>(), (const void* const*)sp.data(), l.data(), n, os, oe, ODBA_W, d.data());
+  int rc = odba_dtw_all_pairs_resident(c, &view, os, oe, d.data(), nullptr, nullptr);
   if (rc) return rc;
   if (dists_out) std::memcpy(dists_out, d.data(), d.size() * sizeof(T));
   const int med = medoid_single<T>(d.data(), l, n);
   if (med < 0) return fail(c, ODBA_E_MEDOID, "no medoid found (all sums of squares at the type maximum)");
-  const int lc = (int)l[med];
+  const int lc = hl[med];
   DbaState<T> st;
-  rc = st.init(c, (const T* const*)sp.data(), l.data(), n, os, oe);
+  rc = st.init_shared(c, B.d, B.o, ho, hl, no_wrap, os, oe);
   if (rc) return rc;
   st.policy = policy;
   st.free_mem = free_mem;
-  rc = st.set_centroid(s[med].data(), lc);
+  rc = st.set_centroid_device(B.d + ho[med], lc);
   if (rc) return rc;
   std::vector<T> nb(lc), prev(lc, T(0)), prev2(lc, T(0));
   int rounds = 0;

@@ -87,6 +87,10 @@ struct RunRec {
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their
+// maxima (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 // Wave-cooperative cache of path words for a traceback that walks up and to
 // the left: lane t holds the words of row i0 - t for the word-lanes l0,
 // l0-1, l0-2, l0-3 of swath k0 (64 rows x 4*C columns). The walk reads a
@@ -107,6 +111,10 @@ struct WordTile {
       w1 = (row >= r0 && l >= 1) ? M[base + (long long)(row - r0 + l - 1) * 64 + (l - 1)] : 0u;
       w2 = (row >= r0 && l >= 2) ? M[base + (long long)(row - r0 + l - 2) * 64 + (l - 2)] : 0u;
       w3 = (row >= r0 && l >= 3) ? M[base + (long long)(row - r0 + l - 3) * 64 + (l - 3)] : 0u;
+      // Wait for the refill here: otherwise the wait lands after the branch,
+      // on every step, and since vmcnt also counts stores every step would
+      // wait for the previous steps' run-record stores to complete.
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     }
     const int dl = l0 - l;
     const uint32_t sel = dl == 0 ? w0 : (dl == 1 ? w1 : (dl == 2 ? w2 : w3));
@@ -114,11 +122,33 @@ struct WordTile {
   }
 };
 
+// Column j of the aligned matrix as (swath k, word-lane l, column q inside
+// the lane's word), stepped left without divisions. Every swath but the
+// last is full (16 columns per lane).
+struct ColCursor {
+  int k, l, q, C;
+  __device__ void set(int j, int K, int lastC) {
+    k = j >> 10;
+    const int t = j & (ODBA_W - 1);
+    C = (k == K - 1) ? lastC : 16;
+    l = t / C;
+    q = t - l * C;
+  }
+  __device__ void left() {
+    if (q > 0) { q--; return; }
+    if (l > 0) { l--; q = C - 1; return; }
+    k--; C = 16; l = 63; q = 15;
+  }
+};
+
 // One wavefront per aligned sequence (all lanes walk the same path; lane 0
 // stores). runs[seq][c] = counted runs of sequence indices for centroid
 // position c. With a path buffer and J.path_off >= 0 every visited cell is
 // also recorded in traceback order (the cells and codes writeDTWPath prints,
-// io_utils.hpp:285-323), the count first.
+// io_utils.hpp:285-323), the count first. Cells of the main region (rows
+// with stored step words) take a short path; row 0, the open-end top row,
+// abrogated swaths and the corner are decided positionally.
+template <bool REC>
 __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ jobs, int njobs,
                                                    const uint32_t* __restrict__ mv, const uint32_t* __restrict__ mtop,
                                                    const int* __restrict__ abr, int4* __restrict__ runs, int lc, int os,
@@ -131,9 +161,15 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
   int4* R = runs + (long long)J.seq * lc;
-  unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  unsigned long long* P = (REC && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  const bool flip = J.flip != 0;
+  const int top = (g.oe && g.ly >= 2) ? g.ly - 1 : INT32_MAX;  // open-end top row (its own words)
+  const int rmin = g.r0 > 1 ? g.r0 : 1;                        // rows below are positional
+  const int abcol = g.ab >= 0 ? g.ab * ODBA_W : INT32_MAX;
   int i = wave_uniform(J.ly - 1), j = wave_uniform(J.lx - 1);
-  int curc = J.flip ? i : j;
+  ColCursor cc;
+  cc.set(j, g.K, g.lastC);
+  int curc = flip ? i : j;
   RunRec rr;
   rr.reset();
   WordTile tile;
@@ -143,27 +179,63 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
   long long np = 0;
   int e = 0;
   for (;;) {
-    uint8_t code;
-    if (i == 0 && j == 0) code = g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
-    else if (g.ab >= 0 && j >= g.ab * ODBA_W) code = (i == g.ly - 1) ? ODBA_OPEN_RIGHT : 0;
-    else if (i == 0) code = g.os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
-    else {
-      const int k = j / ODBA_W, t = j - k * ODBA_W;
-      const int C = (k == g.K - 1) ? g.lastC : 16;
-      const int l = t / C, q = t - l * C;
-      if (g.oe && g.ly >= 2 && i == g.ly - 1) {
-        if (k != topk) { topk = k; topw = TP[(long long)k * 64 + lane]; }
-        const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
-        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
-      } else {
-        const uint32_t c = (tile.get(M, g.S64, g.r0, k, i, l, lane) >> (2 * q)) & 3u;
-        code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
+    // Main region: every cell has a stored word and is counted; the run of
+    // the current centroid position closes when the path leaves its column
+    // (or its row, when the centroid is on the rows).
+    if (i >= rmin && i < top && j < abcol) {
+      const int c = flip ? i : j;
+      if (c != curc) {
+        if (lane == 0) R[curc] = rr.out();
+        curc = c;
+        rr.reset();
       }
     }
-    if (P && lane == 0) P[1 + np] = pack_step(i, j, code);
-    np++;
-    const int c = J.flip ? i : j;
-    const int sidx = J.flip ? j : i;
+    while (i >= rmin && i < top && j < abcol) {
+      // The walk state is the same in every lane; saying so keeps it in
+      // scalar registers and the loop's branches scalar.
+      i = wave_uniform(i); j = wave_uniform(j); curc = wave_uniform(curc);
+      cc.k = wave_uniform(cc.k); cc.l = wave_uniform(cc.l); cc.q = wave_uniform(cc.q); cc.C = wave_uniform(cc.C);
+      tile.i0 = wave_uniform(tile.i0); tile.l0 = wave_uniform(tile.l0); tile.k0 = wave_uniform(tile.k0);
+      rr.lo1 = wave_uniform(rr.lo1); rr.hi1 = wave_uniform(rr.hi1); rr.lo2 = wave_uniform(rr.lo2);
+      rr.hi2 = wave_uniform(rr.hi2); rr.gap = wave_uniform(rr.gap);
+      const uint32_t c2 = (tile.get(M, g.S64, g.r0, cc.k, i, cc.l, lane) >> (2 * cc.q)) & 3u;  // 0 DIAG 1 RIGHT 2 UP
+      if (REC) {
+        if (P && lane == 0) P[1 + np] = pack_step(i, j, c2 == 0 ? ODBA_DIAGONAL : (c2 == 2 ? ODBA_UP : ODBA_RIGHT));
+        np++;
+      }
+      if (!rr.add(true, flip ? j : i)) e |= 4;
+      const bool up = c2 != 1u, lf = c2 != 2u;
+      if (flip ? up : lf) {  // leaving the centroid position
+        if (lane == 0) R[curc] = rr.out();
+        rr.reset();
+        curc = flip ? i - 1 : j - 1;
+      }
+      i -= up ? 1 : 0;
+      if (lf) { j--; cc.left(); }
+      if (i < 0 || j < 0) { e |= 1; break; }
+    }
+    if (e & 1) break;
+    uint32_t code;
+    if (i == 0 && j == 0) {
+      code = g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
+    } else if (j >= abcol) {
+      code = (i == g.ly - 1) ? ODBA_OPEN_RIGHT : 0;
+    } else if (i == 0) {
+      code = g.os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
+    } else {  // open-end top row
+      if (cc.k != topk) {
+        topk = cc.k;
+        topw = TP[(long long)cc.k * 64 + lane];
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      }
+      const uint32_t c2 = ((uint32_t)__builtin_amdgcn_readlane((int)topw, cc.l) >> (2 * cc.q)) & 3u;
+      code = c2 == 0 ? ODBA_DIAGONAL : (c2 == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
+    }
+    if (REC) {
+      if (P && lane == 0) P[1 + np] = pack_step(i, j, code);
+      np++;
+    }
+    const int c = flip ? i : j;
     if (c != curc) {
       if (lane == 0) R[curc] = rr.out();
       curc = c;
@@ -171,17 +243,17 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
     }
     const bool corner = (code == ODBA_NIL || code == ODBA_NIL_OPEN_RIGHT);
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);  // dba.hpp:395-415
-    if (!rr.add(counted, sidx)) e |= 4;
+    if (!rr.add(counted, flip ? j : i)) e |= 4;
     if (corner) break;
-    if (code == ODBA_DIAGONAL) { i--; j--; }
+    if (code == ODBA_DIAGONAL) { i--; j--; cc.left(); }
     else if (code == ODBA_UP) { i--; }
-    else if (code == ODBA_RIGHT || code == ODBA_OPEN_RIGHT) { j--; }
+    else if (code == ODBA_RIGHT || code == ODBA_OPEN_RIGHT) { j--; cc.left(); }
     else { e |= 1; break; }
     if (i < 0 || j < 0) { e |= 1; break; }
   }
   if (lane == 0) {
     R[curc] = rr.out();
-    if (P) P[0] = (unsigned long long)np;
+    if (REC && P) P[0] = (unsigned long long)np;
     if (e) atomicOr(err, e);
   }
 }
@@ -221,16 +293,21 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
   WordTile tile;
   tile.init();
   const uint32_t topw = TP[lane];
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   bool done = false;
   int e = 0;
+  // word-lane l and column q of j inside this stripe, stepped left
+  int l = 0, q = 0;
+  if (j >= left) {
+    l = (j - left) / C;
+    q = (j - left) - l * C;
+  }
   while (j >= left) {
     uint8_t code;
     if (i == 0 && j == 0) code = os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
     else if (S.sab && i == h - 1) code = ODBA_OPEN_RIGHT;
     else if (i == 0) code = os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
     else {
-      const int t = j - left;
-      const int l = t / C, q = t - l * C;
       if (oe && h >= 2 && i == h - 1) {
         const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
@@ -252,9 +329,14 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
     const bool counted = corner ? (code == ODBA_NIL) : (code != ODBA_OPEN_RIGHT);
     if (!rr.add(counted, sidx)) e |= 4;
     if (corner) { done = true; break; }
+    const bool left_move = (code != ODBA_UP);
     if (code == ODBA_DIAGONAL) { i--; j--; }
     else if (code == ODBA_UP) { i--; }
     else { j--; }
+    if (left_move) {
+      if (q > 0) q--;
+      else { l--; q = C - 1; }
+    }
     if (i < 0) { S.err = 1; e |= 1; break; }
   }
   if (done) {
@@ -462,8 +544,12 @@ __global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* 
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
                             const int* abr, int4* runs, int lc, int os, int oe, int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_traceback, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe,
-                     err, pbuf);
+  if (pbuf)
+    hipLaunchKernelGGL(k_traceback<true>, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, mv, mtop, abr, runs, lc,
+                       os, oe, err, pbuf);
+  else
+    hipLaunchKernelGGL(k_traceback<false>, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, mv, mtop, abr, runs, lc,
+                       os, oe, err, pbuf);
   return hipGetLastError();
 }
 

@@ -75,6 +75,11 @@ hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, 
                                   const void* data, const long long* offs, const int* lens, int n, long long npairs,
                                   unsigned long long* counter, void* dist, void* esc, long long stride, int* flags);
 
+// Float path jobs with a workgroup of `waves` waves per job (strips = swaths).
+hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                              unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
+                              long long stride);
+
 hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
                          const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
                          uint32_t* mtop, StripeState* ss);

@@ -523,6 +523,18 @@ __device__ SwathRes<T> swath_dispatch(int C, const SwathIO<T>& io, const WaveWin
   }
 }
 
+template <typename T, bool OS, bool OE, bool PATH, class Pipe>
+__device__ SwathRes<T> swath_dispatch_pipe(int C, const SwathIO<T>& io, const WaveWin<T>* W, int lane, const Pipe& pp) {
+  switch (C) {
+    case 1: return run_swath<T, 1, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+    case 2: return run_swath<T, 2, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+    case 4: return run_swath<T, 4, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+    case 8: return run_swath<T, 8, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+    case 12: return run_swath<T, 12, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+    default: return run_swath<T, 16, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+  }
+}
+
 __host__ __device__ inline int last_swath_c(int wcols) {
   if (wcols <= 64) return 1;
   if (wcols <= 128) return 2;
@@ -800,6 +812,90 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Path jobs with a workgroup of waves per job (few long jobs: the DBA round
+// of a cluster with tens to hundreds of members). The strips are the
+// reference's 1024-column swaths, so the step words land exactly where the
+// one-wave kernel puts them; the edge hand-off and the after-the-fact
+// open-end stop are those of k_all_pairs_strips (the words of swaths past
+// the stop are written but never read: the traceback treats those cells
+// positionally).
+// ---------------------------------------------------------------------------
+template <typename T, bool OS, bool OE>
+__global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restrict__ jobs, int njobs,
+                                                      unsigned int* __restrict__ counter, uint32_t* __restrict__ mv,
+                                                      uint32_t* __restrict__ mtop, int* __restrict__ abrogated,
+                                                      T* __restrict__ escratch, long long stride) {
+  __shared__ T s_y[kMaxStripWaves][128];
+  __shared__ T s_e[kMaxStripWaves][64];
+  __shared__ T s_o[kMaxStripWaves][128];
+  __shared__ int s_owner[2 * kMaxStripWaves];
+  __shared__ int s_cnt[2 * kMaxStripWaves];
+  __shared__ int s_free[2 * kMaxStripWaves];
+  __shared__ int s_job;
+  __shared__ int s_abr;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int nw = uniform_i32((int)(blockDim.x >> 6));
+  const int nslot = 2 * nw;
+  T* ebase = escratch + (long long)blockIdx.x * nslot * stride;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_job = (int)atomicAdd(counter, 1u);
+      s_abr = INT_MAX;
+    }
+    if (threadIdx.x < 2 * kMaxStripWaves) {
+      s_owner[threadIdx.x] = -1;
+      s_cnt[threadIdx.x] = 0;
+      s_free[threadIdx.x] = 1;
+    }
+    __syncthreads();
+    const int jix = uniform_i32(s_job);
+    if (jix >= njobs) break;
+    const PathJob<T> jb = jobs[jix];
+    const int ly = jb.ly, lx = jb.lx;
+    const int K = (lx + ODBA_W - 1) / ODBA_W;
+    const int nrows_main = ((OE && ly >= 2) ? ly - 2 : ly - 1) - (OS ? 1 : 0) + 1;
+    const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
+    for (int s = wid; s < K; s += nw) {
+      const int off = s * ODBA_W;
+      const int so = s % nslot, si = (s + nslot - 1) % nslot;
+      const bool has_out = (s < K - 1);
+      if (has_out) {
+        for (;;) {
+          const int f = uniform_i32(__hip_atomic_load(&s_free[so], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+          if (f) break;
+          __builtin_amdgcn_s_sleep(2);
+        }
+        if (lane == 0) {
+          __hip_atomic_store(&s_free[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(&s_cnt[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(&s_owner[so], s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+      StripPipe pp{&s_owner[si], &s_cnt[si], s - 1, has_out ? &s_cnt[so] : nullptr, lane};
+      SwathIO<T> io;
+      io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = off;
+      io.ein = s > 0 ? ebase + (long long)si * stride : nullptr;
+      io.eout = has_out ? ebase + (long long)so * stride : nullptr;
+      io.mv = mv + jb.mv_off + (size_t)s * swath_words;
+      io.mtop = mtop + jb.top_off + (size_t)s * 64;
+      io.ref_left = 1;
+      const int C = has_out ? 16 : last_swath_c(lx - off);
+      SwathRes<T> res = swath_dispatch_pipe<T, OS, OE, true, StripPipe>(C, io, &win, lane, pp);
+      if (s > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&s_free[si], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (OE && !OS && has_out && off + ODBA_W > ly && res.top == res.colmin && lane == 0) atomicMin(&s_abr, s + 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) abrogated[jix] = (s_abr != INT_MAX) ? s_abr : -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stripe mode (dba.hpp:523-548, 646-763). Forward: every swath's right edge
 // column goes to its own slot (slots + k*H), no step codes. Backward: the
 // host steps m = 0, 1, ...; job j recomputes its stripe k = K_j-1-m with the
@@ -986,6 +1082,19 @@ hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, 
     case ODBA_U32: return launch_strips_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                              unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
+                              long long stride) {
+  if (waves < 1 || waves > kMaxStripWaves) return hipErrorInvalidValue;
+  dim3 g(nblocks), b(64 * waves);
+  const PathJob<float>* J = (const PathJob<float>*)jobs;
+  float* E = (float*)esc;
+#define ODBA_PSTRIPS(A, B) hipLaunchKernelGGL((k_path_strips<float, A, B>), g, b, 0, st, J, njobs, counter, mv, mtop, abr, E, stride)
+  if (os && oe) ODBA_PSTRIPS(true, true); else if (os) ODBA_PSTRIPS(true, false); else if (oe) ODBA_PSTRIPS(false, true); else ODBA_PSTRIPS(false, false);
+#undef ODBA_PSTRIPS
+  return hipGetLastError();
 }
 
 hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,

@@ -22,8 +22,11 @@ EXPORTS = [
     "odba_dtw_all_pairs", "odba_seqset_create", "odba_seqset_destroy", "odba_dtw_all_pairs_resident",
     "odba_dtw_path", "odba_dba_round", "odba_znorm", "odba_perform_dba", "odba_kseg", "odba_kseg_capacity",
     "odba_dba_create", "odba_dba_destroy", "odba_dba_set_centroid", "odba_dba_step",
-    "odba_dba_record_paths", "odba_dba_get_path",
+    "odba_dba_record_paths", "odba_dba_get_path", "odba_ctx_set_mem_cap",
+    "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
+    "odba_server_release", "odba_server_stats",
 ]
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 
 class OdbaError(RuntimeError):
@@ -66,6 +69,15 @@ def load():
     L.odba_dba_step.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
     L.odba_dba_record_paths.argtypes = [vp, i32]
     L.odba_dba_get_path.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
+    L.odba_ctx_set_mem_cap.argtypes = [vp, u64]
+    L.odba_server_create.argtypes = [i32, u64, i32, C.POINTER(vp)]
+    L.odba_server_destroy.argtypes = [vp]
+    L.odba_server_submit.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, C.POINTER(u64)]
+    L.odba_server_wait.argtypes = [vp, u64, C.c_double, C.POINTER(i32)]
+    L.odba_server_result.argtypes = [vp, u64, vp, u64, C.POINTER(u64), C.POINTER(i32), C.POINTER(i32),
+                                     C.POINTER(C.c_double), C.POINTER(i32)]
+    L.odba_server_release.argtypes = [vp, u64]
+    L.odba_server_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)]
     _lib = L
     return L
 

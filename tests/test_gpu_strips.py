@@ -116,3 +116,46 @@ def test_small_workload_both_schedules_match(ctx):
     for sched in ("wave", "strip"):
         got, _ = _run(ctx, seqs, 0, 0, sched)
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), sched
+
+
+@contextlib.contextmanager
+def path_schedule(name):
+    old = os.environ.get("ODBA_PATH_SCHEDULE")
+    os.environ["ODBA_PATH_SCHEDULE"] = name
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["ODBA_PATH_SCHEDULE"]
+        else:
+            os.environ["ODBA_PATH_SCHEDULE"] = old
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+@pytest.mark.parametrize("ly,lx", [(300, 2100), (2500, 4100), (1030, 1025), (5000, 3300)])
+def test_path_strips_step_matrix(ctx, os_, oe, ly, lx):
+    """Path jobs swept by a workgroup of waves (one 1024-column swath each)
+    give the oracle's step matrix, open-end stop included."""
+    rng = np.random.default_rng(ly + lx + 2 * os_ + oe)
+    y, x = O.znorm(_walks(rng, [ly, lx]))[0]
+    print(f"path strips os={os_} oe={oe} {ly}x{lx}", flush=True)
+    with path_schedule("strip"):
+        d, M = ctx.dtw_path(y, x, os_, oe)
+    ed, EM, _, _ = O.dtw_pair(y, x, os_=os_, oe=oe, moves=True)
+    assert np.array_equal(M, EM)
+    assert d == ed
+
+
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_path_strips_dba_round(ctx, os_, oe):
+    rng = np.random.default_rng(70 + 2 * os_ + oe)
+    seqs = O.znorm(_walks(rng, [2100, 3600, 1500, 4200, 2900]))[0]
+    cen = seqs[1].copy()
+    print(f"dba round path strips os={os_} oe={oe}", flush=True)
+    with path_schedule("strip"):
+        got, counts, delta = ctx.dba_round(cen, seqs, os_, oe)
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_=os_, oe=oe)
+    assert rc == 0
+    assert np.array_equal(counts, ecounts)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert delta == edelta
