@@ -167,7 +167,7 @@ def run_serve(args, N, WL):
                                           file=sys.stderr, flush=True))
     plain = S.run_trace(jobs, "plain", max_rounds=args.max_rounds, progress=log("plain"))
     served = S.run_trace(jobs, "served", slots=args.slots, max_rounds=args.max_rounds, progress=log("served"))
-    same = all(np.array_equal(a["centroid"], b["centroid"]) and a["medoid"] == b["medoid"]
+    same = all(a["centroid"].tobytes() == b["centroid"].tobytes() and a["medoid"] == b["medoid"]
                for a, b in zip(plain["results"], served["results"]))
     return {
         "metric": METRIC, "value": round(served["jobs_per_s"], 4), "unit": "jobs/s", "n_gpus": 1,
@@ -186,13 +186,101 @@ def run_serve(args, N, WL):
     }
 
 
+def run_clusters(args, N, WL):
+    """Segment-then-cluster: int16 squiggles (levels N(90,12), dwell geom mean
+    8, noise sd 2), lengths log-uniform 4k-40k, as reads of K molecules
+    (n/K reads each, every read with its own dwell times and noise),
+    k-segmentation (minimum segment length 8) to event medians, z-norm,
+    all-vs-all on the event series, complete-linkage clustering into
+    K clusters of two or more members, then DBA to convergence for every
+    cluster: once with the plain per-cluster path (one DBA session after
+    another) and once with all clusters' rounds packed into shared
+    launches. Mode "open" (open start and open end) throughout: on these
+    event series the global distances cluster by length alone."""
+    reads_per = max(2, args.n_series // args.clusters)
+    raw16, _ = WL.squiggle_reads(args.clusters, reads_per, 4000, 40000, seed=5)
+    raw = WL.short_to_float(raw16)
+    n = len(raw)
+    samples = sum(len(s) for s in raw)
+    stage = {}
+    with N.Context(0) as ctx:
+        t0 = time.perf_counter()
+        ev = ctx.kseg(raw, 8)
+        stage["kseg_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ev, mu, sg = ctx.znorm(ev)
+        order = sorted(range(n), key=lambda i: len(ev[i]))
+        ev = [ev[i] for i in order]
+        lens = np.array([len(e) for e in ev], dtype=np.uint64)
+        sset = ctx.seqset(ev)
+        d, _, kms = sset.all_pairs(1, 1)
+        sset.close()
+        stage["allpairs_s"] = time.perf_counter() - t0
+        cells = WL.pair_cells(lens)
+        t0 = time.perf_counter()
+        memb, med = N.cluster(d, lens, float(args.clusters))
+        stage["cluster_s"] = time.perf_counter() - t0
+        K = len(med)
+        multi = [k for k in range(K) if (memb == k).sum() > 1]
+        sizes = np.bincount(memb)
+        print(f"clusters: {K} ({len(multi)} with 2+ members, largest {sizes.max()}), events {int(lens.sum())}, "
+              f"all-pairs {kms:.1f} ms", file=sys.stderr, flush=True)
+        R = args.max_rounds
+        t0 = time.perf_counter()
+        plain = {}
+        for k in multi:  # the plain path: one cluster after another (dba.hpp:990-1110)
+            members = [ev[i] for i in np.nonzero(memb == k)[0]]
+            with N.DbaSession(ctx, members, 1, 1) as ses:
+                ses.set_centroid(ev[med[k]])
+                prev = np.zeros(len(ev[med[k]]), np.float32)
+                prev2 = prev.copy()
+                for r in range(R):
+                    new, _, delta = ses.step()
+                    rounds_k = r + 1
+                    if delta == 0:
+                        break
+                    if r >= 1 and new.tobytes() == prev2.tobytes():
+                        break
+                    if r > 1:
+                        prev2 = prev.copy()
+                    if r > 0:
+                        prev = new.copy()
+                plain[k] = (new, rounds_k)
+        plain_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cents, rounds = ctx.dba_clusters(ev, memb, med, 1, 1, max_rounds=R)
+        packed_s = time.perf_counter() - t0
+    # bitwise: a centroid position no member maps to is NaN (0/0, dba.hpp:820-822)
+    same = all(plain[k][0].tobytes() == cents[k].tobytes() and plain[k][1] == rounds[k] for k in multi)
+    return {
+        "metric": METRIC, "value": round(len(multi) / packed_s, 3), "unit": "jobs/s (cluster DBA)", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(packed_s * 1000.0, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"segment-then-cluster: {n} int16 squiggles as {reads_per} reads of each of "
+                               f"{args.clusters} molecules, lengths 4k-40k ({samples} samples), "
+                               f"k-seg minseg 8, z-norm, open (open start + open end) all-vs-all on events, "
+                               f"complete linkage into "
+                               f"K={args.clusters} clusters of 2+ members, per-cluster DBA (max {R} rounds)",
+                   "events": int(lens.sum()), "pairs": n * (n - 1) // 2, "all_pairs_cells": cells,
+                   "all_pairs_gcups": round(cells / (kms / 1000.0) / 1e9, 1), "clusters": K,
+                   "multi_member_clusters": len(multi), "rounds_total": int(sum(rounds[k] for k in multi)),
+                   **{k: round(v, 3) for k, v in stage.items()}},
+        "packed": {"jobs_per_s": round(len(multi) / packed_s, 3), "seconds": round(packed_s, 3)},
+        "plain": {"jobs_per_s": round(len(multi) / plain_s, 3), "seconds": round(plain_s, 3)},
+        "speedup_jobs_per_s": round(plain_s / packed_s, 3),
+        "results_identical": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--workload", choices=["squiggles", "small32", "serve"], default="squiggles")
+    ap.add_argument("--workload", choices=["squiggles", "small32", "serve", "clusters"], default="squiggles")
+    ap.add_argument("--n-series", type=int, default=5000, help="clusters: squiggles")
+    ap.add_argument("--clusters", type=int, default=250, help="clusters: K (clusters of 2+ members)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
     ap.add_argument("--scale", type=float, default=0.5, help="serve: series-count and length scale")
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent job slots")
@@ -212,6 +300,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args, N, WL)))
+        return
+    if args.workload == "clusters":
+        print(json.dumps(run_clusters(args, N, WL)))
         return
     raw = WL.squiggles(args.n, 2000, 20000, seed=1)
     seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))

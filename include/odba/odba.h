@@ -123,6 +123,42 @@ int odba_dba_record_paths(odba_dba* dba, int on);
 int odba_dba_get_path(odba_dba* dba, int seq, uint64_t* steps, uint64_t capacity, uint64_t* n_steps);
 
 /*
+ * performDBA's per-cluster consensus (dba.hpp:990-1110) for all clusters of
+ * a clustering at once: sequences (already sorted and normalised as
+ * performDBA has them), memberships[n] in 0..num_clusters-1, medoids[k] the
+ * medoid index of cluster k (odba_cluster). Each round packs the path jobs
+ * of every cluster that has not stopped into one launch per kernel; each
+ * cluster stops on its own rule (delta == 0, the open-mode flip-flop test,
+ * max_rounds). Singletons come back as they are (dba.hpp:1004-1041).
+ * mu / sigma (optional, per sequence) rescale each centroid by its
+ * medoid's (dba.hpp:1117-1125). centroids_out holds the centroids
+ * back to back at centroid_offs[k] (num_clusters + 1 offsets, lengths =
+ * the medoids' lengths); rounds_out[k] = rounds run for cluster k.
+ */
+int odba_dba_clusters(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                      const int* memberships, const int* medoids, int num_clusters, int use_open_start,
+                      int use_open_end, int max_rounds, const double* mu, const double* sigma, void* centroids_out,
+                      uint64_t* centroid_offs, int* rounds_out);
+
+/*
+ * Clustering and medoids of approximateMedoidIndices (dba.hpp:219-351) on
+ * the all-vs-all distances (row-major upper triangle, value type dtype):
+ * distances divided by their maximum, complete-linkage hierarchical
+ * clustering (hclust_fast), then cdist > 1: K clusters with K raised until
+ * cdist of them have two or more members (cutree_k); cdist == 1: one
+ * cluster; 0 <= cdist < 1: cut at that dendrogram height (cutree_cdist).
+ * memberships[n]; medoids[] (capacity n) per cluster with the reference's
+ * rules; *num_clusters. Runs on the host (no device needed).
+ */
+int odba_cluster(int dtype, const void* dists, int n, const uint64_t* lens, double cdist, int* memberships,
+                 int* medoids, int* num_clusters);
+/* The dendrogram alone: complete linkage of a condensed double distance
+ * matrix, R-convention merge ((n-1) x 2, column-major) and heights, and the
+ * cut into nclust clusters. */
+int odba_hclust_complete(int n, const double* condensed, int* merge, double* height);
+int odba_cutree_k(int n, const int* merge, int nclust, int* labels);
+
+/*
  * Z-normalise sequences in place (normalizeSequences, gpu_utils.hpp:131-170):
  * population sigma, sigma == 0 treated as 1; mu / sigma (optional) receive
  * each sequence's mean and standard deviation before that substitution.

@@ -322,28 +322,18 @@ int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* seque
   mats << sequence_names[n - 1];
   for (size_t pad = 0; pad < n; ++pad) mats << "\t";
   mats << "0" << std::endl;
-  if (*cdist != 1) {
-    std::cerr << "Clustering with a distance cutoff or K (cdist=" << *cdist
-              << ") is not supported by this library; all sequences form one cluster" << std::endl;
+  // clusters (cdist > 1: K, == 1: one, [0,1): height cut) and their medoids
+  // by the reference's rules (dba.hpp:219-351)
+  std::vector<int> med(n > 0 ? n : 1);
+  int k = 0;
+  odba_host::check(odba_cluster(odba_host::DType<T>::v, d.data(), (int)n, l.data(), *cdist, memberships, med.data(), &k),
+                   "approximateMedoidIndices");
+  std::cerr << "There are " << k << " clusters" << std::endl;
+  int* medoids = new int[k];
+  for (int c = 0; c < k; c++) {
+    medoids[c] = med[c];
+    std::cerr << "medoid is " << med[c] << std::endl;
   }
-  for (size_t i = 0; i < n; i++) memberships[i] = 0;
-  int* medoids = new int[1];
-  int med = -1;
-  if (n > 2) {  // dba.hpp:327-334
-    T low = std::numeric_limits<T>::max();
-    for (size_t i = 0; i < n; ++i)
-      if (sos[i] < low) { med = (int)i; low = sos[i]; }
-  } else if (n == 2) {
-    med = sequence_lengths[0] > sequence_lengths[1] ? 0 : 1;
-  } else {
-    med = 0;
-  }
-  if (med == -1) {
-    std::cerr << "Logic error in medoid finding routine" << std::endl;
-    std::exit(MEDOID_FINDING_ERROR);
-  }
-  medoids[0] = med;
-  std::cerr << "There are 1 clusters" << std::endl << "medoid is " << med << std::endl;
   return medoids;
 }
 
@@ -425,57 +415,91 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
     std::cerr << "Call to performDBA included an unsupported algorithm mode " << algo_mode << std::endl;
     std::exit(UNKNOWN_ALGO);
   }
-  if (algo_mode == CLUSTER_ONLY) { delete[] medoids; return; }
   const std::string prefix(output_prefix);
+  int nclusters = 1;
+  for (int i = 0; i < n; i++) nclusters = std::max(nclusters, memb[i] + 1);
+  if (cdist != 1) {  // dba.hpp:951-963
+    std::ofstream mf((prefix + ".cluster_membership.txt").c_str());
+    if (!mf.is_open()) {
+      std::cerr << "Cannot open sequence cluster membership file " << prefix << ".cluster_membership.txt for writing"
+                << std::endl;
+      std::exit(CANNOT_WRITE_MEMBERSHIP);
+    }
+    mf << "## cluster distance threshold was " << cdist << std::endl;
+    for (int i = 0; i < n; i++) mf << sequence_names[i] << "\t" << memb[i] << "\t" << sequence_names[medoids[memb[i]]] << std::endl;
+    std::cerr << "Found " << nclusters << " clusters using complete linkage and cluster distance cutoff " << cdist << std::endl;
+  }
+  if (algo_mode == CLUSTER_ONLY) { delete[] medoids; return; }
+  // resume after the clusters whose averages are already in avg.txt (dba.hpp:983-988)
+  int first = 0;
+  {
+    std::ifstream prev((prefix + ".avg.txt").c_str());
+    for (std::string line; std::getline(prev, line);)
+      if (!line.empty()) first++;
+    if (first) std::cerr << "Restarting convergence with cluster " << (first + 1) << "/" << nclusters << std::endl;
+  }
   std::ofstream avgs((prefix + ".avg.txt").c_str(), std::ios::app);
   if (!avgs.is_open()) {
     std::cerr << "Cannot open sequence averages file " << prefix << ".avg.txt for writing" << std::endl;
     std::exit(CANNOT_WRITE_DBA_AVG);
   }
-  const int med = medoids[0];
-  const size_t lc = sequence_lengths[med];
-  std::vector<T> cen(sequences[med], sequences[med] + lc), nb(lc), prev(lc, T(0)), prev2(lc, T(0));
-  const std::string ck = prefix + ".0.evolving_centroid.txt";
-  readCentroidCheckpointFromFile<T>(ck.c_str(), cen.data(), lc);
-  std::vector<uint64_t> ul(sequence_lengths, sequence_lengths + n);
-  odba_dba* dba = nullptr;
-  odba_host::check(odba_dba_create(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)sequences, ul.data(), n,
-                                   use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
-                   "DBA session");
-  odba_host::check(odba_dba_set_centroid(dba, cen.data(), lc), "DBA centroid");
-  odba_host::check(odba_dba_record_paths(dba, 1), "DBA paths");
-  std::cerr << "Processing cluster 1 of 1, " << n << " members, medoid " << sequence_names[med] << " has length " << lc
-            << std::endl;
-  // Every round rewrites <prefix>.0.path<k>.txt against that round's input
-  // centroid; only the last round's files remain, so they are written once,
-  // from the last round's recorded tracebacks.
-  std::vector<T> round_in(cen);
-  for (int i = 0; i < 250; i++) {  // dba.hpp:1078-1110
-    double delta = 0;
-    if (i > 0) round_in = nb;
-    odba_host::check(odba_dba_step(dba, nb.data(), nullptr, &delta), "DBAUpdate");
-    std::cerr << "New delta is " << delta << std::endl;
-    if (delta == 0) break;
-    if (use_open_start || use_open_end) {
-      if (i >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * lc)) {
-        std::cerr << "Detected a flip-flop between two alternative converged centroids, keeping the first one calculated" << std::endl;
-        break;
+  for (int cc = first; cc < nclusters; cc++) {
+    const int med = medoids[cc];
+    const size_t lc = sequence_lengths[med];
+    std::vector<T*> cs;
+    std::vector<char*> cn;
+    std::vector<size_t> cl;
+    for (int i = 0; i < n; i++)
+      if (memb[i] == cc) { cs.push_back(sequences[i]); cn.push_back(sequence_names[i]); cl.push_back(sequence_lengths[i]); }
+    std::vector<T> nb(sequences[med], sequences[med] + lc);
+    if (cs.size() > 1) {
+      std::vector<T> cen(nb), prev(lc, T(0)), prev2(lc, T(0));
+      const std::string ck = prefix + "." + std::to_string(cc) + ".evolving_centroid.txt";
+      readCentroidCheckpointFromFile<T>(ck.c_str(), cen.data(), lc);
+      std::vector<uint64_t> ul(cl.begin(), cl.end());
+      odba_dba* dba = nullptr;
+      odba_host::check(odba_dba_create(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)cs.data(), ul.data(),
+                                       (int)cs.size(), use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
+                       "DBA session");
+      odba_host::check(odba_dba_set_centroid(dba, cen.data(), lc), "DBA centroid");
+      odba_host::check(odba_dba_record_paths(dba, 1), "DBA paths");
+      std::cerr << "Processing cluster " << (cc + 1) << " of " << nclusters << ", " << cs.size() << " members, medoid "
+                << sequence_names[med] << " has length " << lc << std::endl;
+      // Every round rewrites <prefix>.<cluster>.path<k>.txt against that
+      // round's input centroid; only the last round's files remain, so they
+      // are written once, from the last round's recorded tracebacks.
+      std::vector<T> round_in(cen);
+      for (int i = 0; i < 250; i++) {  // dba.hpp:1078-1110
+        double delta = 0;
+        if (i > 0) round_in = nb;
+        odba_host::check(odba_dba_step(dba, nb.data(), nullptr, &delta), "DBAUpdate");
+        std::cerr << "New delta is " << delta << std::endl;
+        if (delta == 0) break;
+        if (use_open_start || use_open_end) {
+          if (i >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * lc)) {
+            std::cerr << "Detected a flip-flop between two alternative converged centroids, keeping the first one calculated" << std::endl;
+            break;
+          }
+          if (i > 1) prev2 = prev;
+          if (i > 0) prev = nb;
+        }
+        writeCentroidCheckpointToFile<T>(ck.c_str(), nb.data(), lc);
       }
-      if (i > 1) prev2 = prev;
-      if (i > 0) prev = nb;
+      odba_host::write_dba_paths<T>(dba, prefix + "." + std::to_string(cc), cs.data(), cn.data(), cl.data(), cs.size(),
+                                    round_in.data(), lc, use_open_end);
+      odba_dba_destroy(dba);
+      std::remove(ck.c_str());
+    } else {
+      std::cerr << "Outputting singleton sequence " << sequence_names[med] << " as-is (a.k.a. cluster " << (cc + 1) << "/"
+                << nclusters << ")." << std::endl;
     }
-    writeCentroidCheckpointToFile<T>(ck.c_str(), nb.data(), lc);
+    if (norm_sequences)  // dba.hpp:1017-1023, 1117-1125
+      for (size_t i = 0; i < lc; i++) nb[i] = (T)(mu[med] + nb[i] * sg[med]);
+    avgs << sequence_names[med];
+    for (size_t i = 0; i < lc; ++i) avgs << "\t" << nb[i];
+    avgs << std::endl;
+    avgs.flush();
   }
-  odba_host::write_dba_paths<T>(dba, prefix + ".0", sequences, sequence_names, sequence_lengths, (size_t)n,
-                                round_in.data(), lc, use_open_end);
-  odba_dba_destroy(dba);
-  if (norm_sequences)
-    for (size_t i = 0; i < lc; i++) nb[i] = (T)(mu[med] + nb[i] * sg[med]);
-  avgs << sequence_names[med];
-  for (size_t i = 0; i < lc; ++i) avgs << "\t" << nb[i];
-  avgs << std::endl;
-  avgs.flush();
-  std::remove(ck.c_str());
   delete[] medoids;
 }
 

@@ -629,6 +629,7 @@ template <typename T> struct DbaState {
       sj[q].top_off = (long long)q * 64;
       tb[q].ly = H; tb[q].lx = LX; tb[q].flip = flip; tb[q].seq = b - b0; tb[q].mv_off = words; tb[q].top_off = (long long)q * 64;
       tb[q].path_off = rec ? h_poff[b] : -1;
+      tb[q].runs_off = (long long)(b - b0) * lc;
       slot_elems += (long long)K * (H + 256);
       words += std::max(1LL, swath_words(H, os, oe));
       maxK = std::max(maxK, K);
@@ -708,6 +709,7 @@ template <typename T> struct DbaState {
           TbJob t;
           t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
           t.path_off = rec ? h_poff[b1] : -1;
+          t.runs_off = (long long)(b1 - b0) * lc;
           tb.push_back(t);
           maxly = std::max(maxly, ly);
           maxlx = std::max(maxlx, lx);
@@ -957,6 +959,224 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   return ODBA_OK;
 }
 
+// Device allocations freed together when the owner goes out of scope.
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() { for (void* x : p) if (x) hipFree(x); }
+  template <typename P> int get(odba_ctx* c, P*& out, size_t elems) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(elems, 1) * sizeof(P)) != hipSuccess)
+      return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes", elems * sizeof(P));
+    p.push_back(q);
+    out = (P*)q;
+    return ODBA_OK;
+  }
+};
+
+// performDBA's per-cluster consensus loop (dba.hpp:990-1110) for every
+// cluster at once. Each round packs the path jobs of all clusters that have
+// not stopped into one path launch and one traceback launch, then one
+// accumulation and one mean/delta launch over all their centroid
+// positions; each cluster applies its own stop rule (delta == 0, the
+// open-mode flip-flop test, max_rounds). Singletons are returned as they
+// are (dba.hpp:1004-1041). With mu/sigma given, centroids are rescaled by
+// their medoid's (dba.hpp:1117-1125).
+template <typename T>
+int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int n, const int* memb,
+                   const int* medoids, int nc, int os, int oe, int max_rounds, const double* mu, const double* sg,
+                   T* cen_out, uint64_t* cen_offs, int* rounds_out) {
+  if (n < 1 || nc < 1) return fail(c, ODBA_E_INVALID, "no sequences or clusters");
+  std::vector<long long> ho(n);
+  std::vector<int> hl(n);
+  long long total = 0;
+  for (int i = 0; i < n; i++) {
+    if (lens[i] == 0 || lens[i] > (uint64_t)INT32_MAX / 2) return fail(c, ODBA_E_INVALID, "sequence %d has length %llu", i, (unsigned long long)lens[i]);
+    ho[i] = total;
+    hl[i] = (int)lens[i];
+    total += padded_len((long long)lens[i]);
+  }
+  std::vector<std::vector<int>> mem(nc);
+  for (int i = 0; i < n; i++) {
+    if (memb[i] < 0 || memb[i] >= nc) return fail(c, ODBA_E_INVALID, "membership %d of sequence %d out of range", memb[i], i);
+    mem[memb[i]].push_back(i);
+  }
+  std::vector<int> lcs(nc);
+  uint64_t off = 0;
+  for (int k = 0; k < nc; k++) {
+    if (medoids[k] < 0 || medoids[k] >= n || memb[medoids[k]] != k) return fail(c, ODBA_E_MEDOID, "medoid %d of cluster %d is not a member", medoids[k], k);
+    lcs[k] = hl[medoids[k]];
+    cen_offs[k] = off;
+    off += (uint64_t)lcs[k];
+  }
+  cen_offs[nc] = off;
+  const bool no_wrap = int_cannot_wrap<T>(seqs, lens, n);
+  // resident data, member offsets per cluster, centroid pools
+  std::vector<T> hb((size_t)total, T(0));
+  for (int i = 0; i < n; i++) std::memcpy(&hb[ho[i]], seqs[i], lens[i] * sizeof(T));
+  std::vector<long long> moffs, mem_off(nc), cen_off(nc), runs_off(nc);
+  long long coff = 0, roff = 0;
+  for (int k = 0; k < nc; k++) {
+    mem_off[k] = (long long)moffs.size();
+    for (int b : mem[k]) moffs.push_back(ho[b]);
+    cen_off[k] = coff;
+    coff += padded_len(lcs[k]);
+    runs_off[k] = roff;
+    roff += (long long)mem[k].size() * lcs[k];
+  }
+  DevBufs B;
+  T *data, *cen, *newc, *acc;
+  long long* dmoffs;
+  uint32_t* counts;
+  int4* runs;
+  MultiSlot* dslots;
+  unsigned long long* dbits;
+  int* derr;
+  int* abr;
+  TbJob* tbj;
+  void* jobs;
+  int rc;
+  if ((rc = B.get(c, data, hb.size())) || (rc = B.get(c, dmoffs, moffs.size())) || (rc = B.get(c, cen, coff)) ||
+      (rc = B.get(c, newc, coff)) || (rc = B.get(c, acc, off)) || (rc = B.get(c, counts, off)) ||
+      (rc = B.get(c, runs, roff)) || (rc = B.get(c, dslots, nc)) || (rc = B.get(c, dbits, nc)) ||
+      (rc = B.get(c, derr, 1)) || (rc = B.get(c, abr, n)) || (rc = B.get(c, tbj, n)))
+    return rc;
+  {
+    unsigned char* q;
+    if ((rc = B.get(c, q, (size_t)n * sizeof(PJ<T>)))) return rc;
+    jobs = q;
+  }
+  HIPC(c, hipMemcpyAsync(data, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dmoffs, moffs.data(), std::max<size_t>(moffs.size(), 1) * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(cen, 0, coff * sizeof(T), c->stream));
+  HIPC(c, hipMemsetAsync(newc, 0, coff * sizeof(T), c->stream));
+  for (int k = 0; k < nc; k++)
+    HIPC(c, hipMemcpyAsync(cen + cen_off[k], data + ho[medoids[k]], lcs[k] * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+  // host state per cluster
+  std::vector<std::vector<T>> nb(nc), prev(nc), prev2(nc);
+  std::vector<char> active(nc, 0);
+  for (int k = 0; k < nc; k++) {
+    rounds_out[k] = 0;
+    nb[k].assign(seqs[medoids[k]], seqs[medoids[k]] + lcs[k]);
+    if (mem[k].size() > 1) {
+      active[k] = 1;
+      prev[k].assign(lcs[k], T(0));
+      prev2[k].assign(lcs[k], T(0));
+    }
+  }
+  const long long budget_words = (long long)(c->mem_cap / 4 / 4);
+  std::vector<unsigned long long> bits(nc);
+  uint32_t* mv = nullptr;
+  uint32_t* mtop = nullptr;
+  size_t mv_cap = 0, mtop_cap = 0;
+  struct FreeWords {
+    uint32_t*& a; uint32_t*& b;
+    ~FreeWords() { if (a) hipFree(a); if (b) hipFree(b); }
+  } free_words{mv, mtop};
+  for (int r = 0; r < max_rounds; r++) {
+    std::vector<int> act;
+    for (int k = 0; k < nc; k++)
+      if (active[k]) act.push_back(k);
+    if (act.empty()) break;
+    // groups of whole clusters whose step words fit the budget
+    size_t a0 = 0;
+    while (a0 < act.size()) {
+      std::vector<PJ<T>> pj;
+      std::vector<TbJob> tb;
+      std::vector<MultiSlot> slots;
+      long long mvw = 0, tw = 0, pos = 0;
+      int maxly = 1, maxlx = 0, minly = INT32_MAX;
+      size_t a1 = a0;
+      for (; a1 < act.size(); a1++) {
+        const int k = act[a1];
+        long long need = 0;
+        for (int b : mem[k]) {
+          const int flip = (oe && lcs[k] < hl[b]) ? 1 : 0;
+          const int ly = flip ? lcs[k] : hl[b], lx = flip ? hl[b] : lcs[k];
+          need += (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        }
+        if (a1 > a0 && mvw + need > budget_words) break;
+        MultiSlot sl;
+        sl.pos0 = pos; sl.runs_off = runs_off[k]; sl.cen_off = cen_off[k]; sl.mem_off = mem_off[k];
+        sl.lc = lcs[k]; sl.nmem = (int)mem[k].size(); sl.slot = k; sl.pad = 0;
+        slots.push_back(sl);
+        pos += lcs[k];
+        for (size_t q = 0; q < mem[k].size(); q++) {
+          const int b = mem[k][q];
+          const int flip = (oe && lcs[k] < hl[b]) ? 1 : 0;
+          const int ly = flip ? lcs[k] : hl[b], lx = flip ? hl[b] : lcs[k];
+          long long mo, to;
+          plan_job(ly, lx, os, oe, mvw, tw, mo, to);
+          const T* sq = data + ho[b];
+          const T* cp = cen + cen_off[k];
+          pj.push_back(PJ<T>{flip ? cp : sq, flip ? sq : cp, ly, lx, mo, to});
+          TbJob t;
+          t.ly = ly; t.lx = lx; t.flip = flip; t.seq = (int)q; t.mv_off = mo; t.top_off = to; t.path_off = -1;
+          t.runs_off = runs_off[k] + (long long)q * lcs[k];
+          tb.push_back(t);
+          maxly = std::max(maxly, ly);
+          maxlx = std::max(maxlx, lx);
+          minly = std::min(minly, ly);
+        }
+      }
+      const int nj = (int)pj.size();
+      if ((size_t)mvw > mv_cap || (size_t)tw > mtop_cap) {  // grow-only step-word buffers
+        if (mv) hipFree(mv);
+        if (mtop) hipFree(mtop);
+        mv = nullptr;
+        mtop = nullptr;
+        mv_cap = std::max<size_t>(mv_cap, (size_t)std::max(1LL, mvw));
+        mtop_cap = std::max<size_t>(mtop_cap, (size_t)std::max(1LL, tw));
+        if (hipMalloc((void**)&mv, mv_cap * 4) != hipSuccess || hipMalloc((void**)&mtop, mtop_cap * 4) != hipSuccess)
+          return fail(c, ODBA_E_NOMEM, "cannot allocate %zu step words", mv_cap);
+      }
+      HIPC(c, hipMemcpyAsync(jobs, pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(tbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(dslots, slots.data(), slots.size() * sizeof(MultiSlot), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemsetAsync(dbits, 0, nc * sizeof(unsigned long long), c->stream));
+      HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
+      rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly);
+      if (rc) return rc;
+      HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, 0, os, oe, derr, nullptr));
+      HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, data, dmoffs, runs, dslots, (int)slots.size(), pos, acc, counts,
+                                cen, newc, dbits, derr));
+      int herr = 0;
+      HIPC(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipMemcpyAsync(bits.data(), dbits, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+      for (size_t q = a0; q < a1; q++) {
+        const int k = act[q];
+        HIPC(c, hipMemcpyAsync(nb[k].data(), newc + cen_off[k], lcs[k] * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+      }
+      HIPC(c, hipStreamSynchronize(c->stream));
+      if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+      if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
+      if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
+      for (size_t q = a0; q < a1; q++) {  // dba.hpp:1084-1110, per cluster
+        const int k = act[q];
+        rounds_out[k] = r + 1;
+        double d;
+        std::memcpy(&d, &bits[k], sizeof(d));
+        // the next round starts from the new centroid
+        HIPC(c, hipMemcpyAsync(cen + cen_off[k], newc + cen_off[k], lcs[k] * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        if (d == 0) { active[k] = 0; continue; }
+        if (os || oe) {
+          if (r >= 1 && !std::memcmp(nb[k].data(), prev2[k].data(), sizeof(T) * lcs[k])) { active[k] = 0; continue; }
+          if (r > 1) prev2[k] = prev[k];
+          if (r > 0) prev[k] = nb[k];
+        }
+      }
+      a0 = a1;
+    }
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nc; k++) {
+    T* out = cen_out + cen_offs[k];
+    const int m = medoids[k];
+    for (int i = 0; i < lcs[k]; i++)
+      out[i] = (mu && sg) ? host_d2t<T>(mu[m] + nb[k][i] * sg[m]) : nb[k][i];
+  }
+  return ODBA_OK;
+}
+
 }  // namespace
 
 #define ODBA_DISPATCH(dtype, CALL)                        \
@@ -986,6 +1206,19 @@ int odba_dba_round(odba_ctx* c, int dtype, const void* centroid, uint64_t lc, co
   if (path_policy < ODBA_PATH_AUTO || path_policy > ODBA_PATH_STRIPE) return fail(c, ODBA_E_INVALID, "unknown path policy %d", path_policy);
   ODBA_DISPATCH(dtype, dba_round_t<T>(c, (const T*)centroid, (int)lc, (const T* const*)seqs, lens, n, os, oe,
                                       path_policy, ref_free_mem, (T*)new_centroid, counts_out, max_delta))
+}
+
+int odba_dba_clusters(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                      const int* memberships, const int* medoids, int num_clusters, int use_open_start,
+                      int use_open_end, int max_rounds, const double* mu, const double* sigma, void* centroids_out,
+                      uint64_t* centroid_offs, int* rounds_out) {
+  if (!c || !seqs || !lens || !memberships || !medoids || !centroids_out || !centroid_offs || !rounds_out || n <= 0 ||
+      num_clusters <= 0 || max_rounds < 0)
+    return fail(c, ODBA_E_INVALID, "invalid cluster DBA arguments");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(dtype, dba_clusters_t<T>(c, (const T* const*)seqs, lens, n, memberships, medoids, num_clusters,
+                                         use_open_start, use_open_end, max_rounds, mu, sigma, (T*)centroids_out,
+                                         centroid_offs, rounds_out))
 }
 
 int odba_znorm(odba_ctx* c, int dtype, void* const* seqs, const uint64_t* lens, int n, double* mu, double* sigma) {

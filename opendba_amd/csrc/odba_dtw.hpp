@@ -47,6 +47,7 @@ struct TbJob {
   long long mv_off;    // word offset of the pair's step words
   long long top_off;   // word offset of the open-end top-row words
   long long path_off;  // -1, or the slot in the path buffer: [count][steps...]
+  long long runs_off;  // element offset of this job's run records (one per centroid position)
 };
 
 // One visited cell of a recorded traceback: row i (31 bits), column j
@@ -93,6 +94,19 @@ hipError_t launch_accumulate(int dtype, hipStream_t st, const void* data, const 
                              int nseq, int lc, void* acc, uint32_t* counts, int first_batch);
 hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const uint32_t* counts, const void* old_c,
                              void* new_c, int lc, unsigned long long* delta_bits, int* err);
+// One active cluster of a packed multi-cluster DBA round.
+struct MultiSlot {
+  long long pos0;      // first global centroid position of the cluster
+  long long runs_off;  // its run records: [member][position]
+  long long cen_off;   // its centroid in the centroid pools
+  long long mem_off;   // its members' data offsets in the member-offset list
+  int lc, nmem, slot;  // centroid length, members, index of its delta word
+  int pad;
+};
+hipError_t launch_multi_mean(int dtype, hipStream_t st, const void* data, const long long* moffs, const int4* runs,
+                             const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts,
+                             const void* oldc, void* newc, unsigned long long* delta_bits, int* err);
+
 hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* offs, const int* lens, int n,
                         double* mu, double* sigma);
 hipError_t launch_decode_moves(hipStream_t st, const uint32_t* mv, const uint32_t* mtop, int ab, int ly, int lx, int os,

@@ -24,7 +24,8 @@ EXPORTS = [
     "odba_dba_create", "odba_dba_destroy", "odba_dba_set_centroid", "odba_dba_step",
     "odba_dba_record_paths", "odba_dba_get_path", "odba_ctx_set_mem_cap",
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
-    "odba_server_release", "odba_server_stats",
+    "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
+    "odba_cutree_k",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
@@ -78,6 +79,10 @@ def load():
                                      C.POINTER(C.c_double), C.POINTER(i32)]
     L.odba_server_release.argtypes = [vp, u64]
     L.odba_server_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32), C.POINTER(i32)]
+    L.odba_dba_clusters.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp]
+    L.odba_cluster.argtypes = [i32, vp, i32, vp, C.c_double, vp, vp, C.POINTER(i32)]
+    L.odba_hclust_complete.argtypes = [i32, vp, vp, vp]
+    L.odba_cutree_k.argtypes = [i32, vp, i32, vp]
     _lib = L
     return L
 
@@ -164,6 +169,26 @@ class Context:
                                       _p(sg)))
         return seqs, mu, sg
 
+    def dba_clusters(self, seqs, memberships, medoids, open_start=0, open_end=0, max_rounds=250, mu=None,
+                     sigma=None):
+        """All clusters' DBA with packed rounds (odba_dba_clusters); returns
+        (list of centroids, rounds per cluster)."""
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        n = len(seqs)
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        memb = np.ascontiguousarray(memberships, dtype=np.int32)
+        med = np.ascontiguousarray(medoids, dtype=np.int32)
+        k = len(med)
+        offs = np.zeros(k + 1, dtype=np.uint64)
+        out = np.zeros(int(sum(len(seqs[m]) for m in med)), dtype=seqs[0].dtype)
+        rounds = np.zeros(k, dtype=np.int32)
+        mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        self._check(self.L.odba_dba_clusters(self.h, CODE[seqs[0].dtype], _ptrs(seqs), _p(lens), n, _p(memb), _p(med),
+                                             k, open_start, open_end, max_rounds, _p(mu), _p(sigma), _p(out),
+                                             _p(offs), _p(rounds)))
+        return [out[int(offs[i]):int(offs[i + 1])] for i in range(k)], rounds
+
     def perform_dba(self, seqs, open_start=0, open_end=0, norm=0, max_rounds=250, path_policy=PATH_FULL,
                     ref_free_mem=0):
         seqs = [np.ascontiguousarray(s) for s in seqs]
@@ -179,6 +204,44 @@ class Context:
                                             open_end, norm, max_rounds, path_policy, ref_free_mem, _p(cen),
                                             C.byref(clen), C.byref(med), _p(perm), C.byref(rounds), _p(dists)))
         return dict(centroid=cen[: clen.value], medoid=med.value, perm=perm, rounds=rounds.value, dists=dists)
+
+
+def cluster(dists, lens, cdist):
+    """Clusters and medoids from all-vs-all distances (odba_cluster; host
+    only). Returns (memberships, medoids)."""
+    L = load()
+    dists = np.ascontiguousarray(dists)
+    n = len(lens)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    memb = np.zeros(n, dtype=np.int32)
+    med = np.zeros(max(n, 1), dtype=np.int32)
+    k = C.c_int()
+    rc = L.odba_cluster(CODE[dists.dtype], _p(dists), n, _p(lens), float(cdist), _p(memb), _p(med), C.byref(k))
+    if rc:
+        raise OdbaError(rc, "odba_cluster failed")
+    return memb, med[: k.value]
+
+
+def hclust_complete(condensed, n):
+    L = load()
+    d = np.ascontiguousarray(condensed, dtype=np.float64)
+    merge = np.zeros(2 * (n - 1), dtype=np.int32)
+    height = np.zeros(n - 1, dtype=np.float64)
+    rc = L.odba_hclust_complete(n, _p(d), _p(merge), _p(height))
+    if rc:
+        raise OdbaError(rc, "odba_hclust_complete failed")
+    return merge.reshape(2, n - 1).T.copy(), height
+
+
+def cutree_k(merge, nclust):
+    L = load()
+    n = merge.shape[0] + 1
+    m = np.ascontiguousarray(merge.T.reshape(-1), dtype=np.int32)
+    labels = np.zeros(n, dtype=np.int32)
+    rc = L.odba_cutree_k(n, _p(m), nclust, _p(labels))
+    if rc:
+        raise OdbaError(rc, "odba_cutree_k failed")
+    return labels
 
 
 def unpack_steps(packed):

@@ -27,6 +27,28 @@ def squiggle(L, rng, level_mu=90.0, level_sd=12.0, dwell_mean=8.0, noise_sd=2.0)
     return np.clip(np.rint(sig), -32768, 32767).astype(np.int16)
 
 
+def squiggle_reads(n_templates=250, reads_per=20, lmin=4000, lmax=40000, seed=5, level_mu=90.0, level_sd=12.0,
+                   dwell_mean=8.0, noise_sd=2.0):
+    """Reads of shared molecules, as nanopore data has them: each template is
+    a sequence of current levels ~N(level_mu, level_sd) pA sized for a
+    log-uniform length in [lmin, lmax]; each read of it walks the same levels
+    with its own geometric dwell times (mean dwell_mean samples) and
+    Gaussian noise, as int16. Returns (reads, template index per read), reads
+    in a seeded shuffled order."""
+    rng = np.random.default_rng(seed)
+    lens = log_uniform_lengths(n_templates, lmin, lmax, rng)
+    reads, label = [], []
+    for t, L in enumerate(lens):
+        lv = rng.normal(level_mu, level_sd, size=int(L / dwell_mean) + 1)
+        for _ in range(reads_per):
+            dw = rng.geometric(1.0 / dwell_mean, size=len(lv))
+            sig = np.repeat(lv, dw) + rng.normal(0.0, noise_sd, size=int(dw.sum()))
+            reads.append(np.clip(np.rint(sig), -32768, 32767).astype(np.int16))
+            label.append(t)
+    perm = rng.permutation(len(reads))
+    return [reads[i] for i in perm], np.asarray(label)[perm]
+
+
 def log_uniform_lengths(n, lmin, lmax, rng):
     return np.rint(np.exp(rng.uniform(np.log(lmin), np.log(lmax), size=n))).astype(np.int64)
 

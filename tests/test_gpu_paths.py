@@ -150,3 +150,53 @@ def test_cli_kat_end_to_end(tmp_path):
     r = subprocess.run([CLI, "-n", "text", "float", "global", prefix + "_n", "0", "/dev/null", "1.0"] + files,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "First argument (-n)" in r.stderr
+
+
+def test_cli_clusters_end_to_end(tmp_path):
+    """openDBA with cdist = 2 (two clusters of two or more members):
+    <prefix>.cluster_membership.txt, one avg.txt line per cluster, each
+    equal to the oracle's DBA loop for that cluster rescaled by its medoid,
+    and the per-cluster path files."""
+    from opendba_amd import native as N
+    rng = np.random.default_rng(77)
+    files, raw = [], []
+    for g in range(2):
+        base = np.cumsum(rng.normal(size=700)) * (1 + 2 * g)
+        for r in range(3):
+            L = int(rng.integers(400, 700))
+            x = (base[:L] + rng.normal(0, 0.2, size=L)).astype(np.float32)
+            f = tmp_path / f"g{g}r{r}.txt"
+            f.write_text("\n".join("%.6g" % v for v in x) + "\n")
+            files.append(str(f))
+    prefix = str(tmp_path / "cl")
+    print("cli: openDBA text float global cdist=2", flush=True)
+    r = subprocess.run([CLI, "text", "float", "global", prefix, "0", "/dev/null", "2"] + files,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    seqs = [O.read_text_series(f, np.float32) for f in files]
+    order = sorted(range(len(seqs)), key=lambda k: len(seqs[k]))
+    names = [files[k][: files[k].index(".")] for k in order]
+    z, mu, sg = O.znorm([seqs[k] for k in order])
+    d = O.all_pairs(z, 0, 0)
+    memb, med = N.cluster(d, [len(x) for x in z], 2.0)
+    lines = open(prefix + ".cluster_membership.txt").read().rstrip("\n").split("\n")
+    assert lines[0] == "## cluster distance threshold was 2"
+    for i, ln in enumerate(lines[1:]):
+        nm, c, mname = ln.split("\t")
+        assert nm == names[i] and int(c) == memb[i] and mname == names[med[memb[i]]]
+    avg = [ln.split("\t") for ln in open(prefix + ".avg.txt").read().rstrip("\n").split("\n")]
+    assert len(avg) == len(med)
+    for c, m in enumerate(med):
+        members = [z[i] for i in range(len(z)) if memb[i] == c]
+        cen = z[m].copy()
+        if len(members) > 1:
+            for _ in range(250):
+                rc, new, _, delta = O.dba_update(cen, members, os_=0, oe=0)
+                cen = new
+                if delta == 0:
+                    break
+            for k in range(len(members)):
+                assert os.path.exists(f"{prefix}.{c}.path{k}.txt")
+        exp = (mu[m] + cen.astype(np.float64) * sg[m]).astype(np.float32)
+        assert avg[c][0] == names[m]
+        assert avg[c][1:] == [_g6(v) for v in exp], c
