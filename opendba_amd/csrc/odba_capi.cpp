@@ -108,6 +108,20 @@ bool int_cannot_wrap(const T* const* seqs, const uint64_t* lens, int n) {
   return bound < (long double)std::numeric_limits<T>::max() / 2;
 }
 
+// Device allocations freed together when the owner goes out of scope.
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() { for (void* x : p) if (x) hipFree(x); }
+  template <typename P> int get(odba_ctx* c, P*& out, size_t elems) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(elems, 1) * sizeof(P)) != hipSuccess)
+      return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes", elems * sizeof(P));
+    p.push_back(q);
+    out = (P*)q;
+    return ODBA_OK;
+  }
+};
+
 int grid_blocks(odba_ctx* c, long long work_items) {
   long long want = (work_items + 3) / 4;
   long long cap = (long long)c->cus * 8;  // 8 workgroups of 4 waves per CU
@@ -222,13 +236,20 @@ int odba_seqset_destroy(odba_seqset* s) {
 
 namespace {
 
-// Pair schedule of the all-vs-all: one wave per pair (many pairs), or a
-// workgroup of waves per pair over column strips (too few pairs to give
-// every SIMD work). ODBA_PAIR_SCHEDULE=wave|strip overrides the choice.
+// Pair schedule of the all-vs-all: one wave per pair (many pairs), or bands
+// of column strips swept by workgroups (too few pairs to give every SIMD
+// work). ODBA_PAIR_SCHEDULE=wave|strip overrides the choice;
+// ODBA_BAND_WAVES and ODBA_BAND_RING set the band width in strips and the
+// in-workgroup edge ring (rows).
 struct PairSchedule {
   bool strips = false;
-  int C = 16, waves = 4, nblocks = 1;
+  int C = 16, waves = 4, nblocks = 1, ring = 2048;
 };
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
 
 PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, long long npairs) {
   PairSchedule ps;
@@ -248,10 +269,65 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, long long npairs
     return ps;
   }
   ps.C = s->maxlen <= 16 * 64 ? 1 : (s->maxlen <= 16 * 256 ? 4 : 16);
-  ps.waves = (int)std::min<long long>(16, std::max<long long>(1, (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C)));
-  const long long per_cu = std::max(1, 16 / ps.waves);  // 16 waves of 128 VGPRs per CU
-  ps.nblocks = (int)std::max<long long>(1, std::min<long long>(npairs, per_cu * c->cus));
+  const long long smax = (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C);
+  ps.waves = (int)std::min<long long>(16, std::max<long long>(1, smax));
+  ps.waves = std::max(1, std::min(16, env_int("ODBA_BAND_WAVES", ps.waves)));
+  ps.ring = env_int("ODBA_BAND_RING", 2048);
   return ps;
+}
+
+// Bands of strips (k_all_pairs_bands): claim table in the one-wave kernel's
+// pair order, scratch, launch, hand-off check.
+int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, int oe, long long npairs, void* dist,
+                    int* flags) {
+  const int n = s->n;
+  const long long SW = 64LL * ps.C;
+  struct Tab { int r, c; long long band0; };
+  static_assert(sizeof(Tab) == 16, "claim table entry");
+  if (band_pair_size() != sizeof(Tab)) return fail(c, ODBA_E_INVALID, "claim table layout mismatch");
+  std::vector<Tab> tab;
+  tab.reserve((size_t)npairs);
+  long long nclaims = 0;
+  for (int cc = n - 1; cc >= 1; cc--)
+    for (int r = cc - 1; r >= 0; r--) {
+      tab.push_back(Tab{r, cc, nclaims});
+      const long long S = ((long long)s->h_lens[cc] + SW - 1) / SW;
+      nclaims += (S + ps.waves - 1) / ps.waves;
+    }
+  const long long per_cu = std::max(1, 16 / ps.waves);
+  const int nblocks = (int)std::max<long long>(1, std::min<long long>(nclaims, per_cu * c->cus));
+  const int R = nblocks + 2;
+  const size_t es = dtype_size(s->dtype);
+  const long long gstride = s->maxlen + 256;
+  const size_t ring_elems = (size_t)nblocks * ps.waves * ps.ring;
+  int rc = ensure_scratch(c, (ring_elems + (size_t)R * gstride) * es);
+  if (rc) return rc;
+  void* rings = c->esc;
+  void* gslots = (char*)c->esc + ring_elems * es;
+  DevBufs B;
+  int *gsync, *abr, *err;
+  unsigned char* dtab;
+  if ((rc = B.get(c, gsync, 3 * (size_t)R)) || (rc = B.get(c, abr, tab.size())) || (rc = B.get(c, err, 1)) ||
+      (rc = B.get(c, dtab, tab.size() * sizeof(Tab))))
+    return rc;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+  const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;  // per wait
+  HIPC(c, hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(Tab), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+  unsigned long long* counter = (unsigned long long*)c->small;
+  HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+  HIPC(c, hipEventRecord(c->ev0, c->stream));
+  hipError_t e = launch_all_pairs_bands(s->dtype, ps.C, ps.waves, os, oe, nblocks, c->stream, s->data, s->offs, s->lens,
+                                        n, dtab, (int)tab.size(), nclaims, counter, dist, rings, ps.ring, gslots, gstride,
+                                        gsync, R, abr, err, limit, flags);
+  if (e != hipSuccess) return fail(c, ODBA_E_HIP, "band launch: %s", hipGetErrorString(e));
+  HIPC(c, hipEventRecord(c->ev1, c->stream));
+  int herr = 0;
+  HIPC(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (herr & 8) return fail(c, ODBA_E_HIP, "band edge hand-off timed out");
+  return ODBA_OK;
 }
 
 }  // namespace
@@ -275,9 +351,10 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   const PairSchedule ps = choose_schedule(c, s, npairs);
   const int nblocks = ps.nblocks;
   const long long stride = s->maxlen + 256;
-  const size_t scratch = ps.strips ? (size_t)nblocks * 2 * ps.waves * stride * es : (size_t)nblocks * 4 * 2 * stride * es;
-  int rc = ensure_scratch(c, scratch);
-  if (rc) return rc;
+  if (!ps.strips) {
+    int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
+    if (rc) return rc;
+  }
   void* dist = nullptr;
   int* flags = nullptr;
   if (hipMalloc(&dist, npairs * es) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
@@ -285,16 +362,18 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
     hipFree(dist);
     return fail(c, ODBA_E_NOMEM, "cannot allocate flags");
   }
-  unsigned long long* counter = (unsigned long long*)c->small;
-  HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
-  HIPC(c, hipEventRecord(c->ev0, c->stream));
-  hipError_t e = ps.strips
-                     ? launch_all_pairs_strips(s->dtype, ps.C, ps.waves, os, oe, nblocks, c->stream, s->data, s->offs,
-                                               s->lens, n, npairs, counter, dist, c->esc, stride, flags)
-                     : launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs,
-                                        s->lens, n, npairs, counter, dist, c->esc, stride, flags);
-  if (e != hipSuccess) { hipFree(dist); if (flags) hipFree(flags); return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e)); }
-  HIPC(c, hipEventRecord(c->ev1, c->stream));
+  if (ps.strips) {
+    const int rc = all_pairs_bands(c, s, ps, os, oe, npairs, dist, flags);
+    if (rc) { hipFree(dist); if (flags) hipFree(flags); return rc; }
+  } else {
+    unsigned long long* counter = (unsigned long long*)c->small;
+    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    HIPC(c, hipEventRecord(c->ev0, c->stream));
+    hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs, s->lens, n,
+                                    npairs, counter, dist, c->esc, stride, flags);
+    if (e != hipSuccess) { hipFree(dist); if (flags) hipFree(flags); return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e)); }
+    HIPC(c, hipEventRecord(c->ev1, c->stream));
+  }
   if (dist_out) HIPC(c, hipMemcpyAsync(dist_out, dist, npairs * es, hipMemcpyDeviceToHost, c->stream));
   if (flags_out) HIPC(c, hipMemcpyAsync(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -630,6 +709,7 @@ template <typename T> struct DbaState {
       tb[q].ly = H; tb[q].lx = LX; tb[q].flip = flip; tb[q].seq = b - b0; tb[q].mv_off = words; tb[q].top_off = (long long)q * 64;
       tb[q].path_off = rec ? h_poff[b] : -1;
       tb[q].runs_off = (long long)(b - b0) * lc;
+      tb[q].rp = nullptr;
       slot_elems += (long long)K * (H + 256);
       words += std::max(1LL, swath_words(H, os, oe));
       maxK = std::max(maxK, K);
@@ -710,6 +790,7 @@ template <typename T> struct DbaState {
           t.ly = ly; t.lx = lx; t.flip = flip; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to;
           t.path_off = rec ? h_poff[b1] : -1;
           t.runs_off = (long long)(b1 - b0) * lc;
+          t.rp = nullptr;
           tb.push_back(t);
           maxly = std::max(maxly, ly);
           maxlx = std::max(maxlx, lx);
@@ -959,19 +1040,6 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   return ODBA_OK;
 }
 
-// Device allocations freed together when the owner goes out of scope.
-struct DevBufs {
-  std::vector<void*> p;
-  ~DevBufs() { for (void* x : p) if (x) hipFree(x); }
-  template <typename P> int get(odba_ctx* c, P*& out, size_t elems) {
-    void* q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(elems, 1) * sizeof(P)) != hipSuccess)
-      return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes", elems * sizeof(P));
-    p.push_back(q);
-    out = (P*)q;
-    return ODBA_OK;
-  }
-};
 
 // performDBA's per-cluster consensus loop (dba.hpp:990-1110) for every
 // cluster at once. Each round packs the path jobs of all clusters that have
@@ -1096,7 +1164,8 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
         }
         if (a1 > a0 && mvw + need > budget_words) break;
         MultiSlot sl;
-        sl.pos0 = pos; sl.runs_off = runs_off[k]; sl.cen_off = cen_off[k]; sl.mem_off = mem_off[k];
+        sl.pos0 = pos; sl.data = data; sl.moffs = dmoffs + mem_off[k]; sl.runs = runs + runs_off[k];
+        sl.cen = cen + cen_off[k]; sl.newc = newc + cen_off[k];
         sl.lc = lcs[k]; sl.nmem = (int)mem[k].size(); sl.slot = k; sl.pad = 0;
         slots.push_back(sl);
         pos += lcs[k];
@@ -1112,6 +1181,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
           TbJob t;
           t.ly = ly; t.lx = lx; t.flip = flip; t.seq = (int)q; t.mv_off = mo; t.top_off = to; t.path_off = -1;
           t.runs_off = runs_off[k] + (long long)q * lcs[k];
+          t.rp = nullptr;
           tb.push_back(t);
           maxly = std::max(maxly, ly);
           maxlx = std::max(maxlx, lx);
@@ -1137,8 +1207,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
       rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly);
       if (rc) return rc;
       HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, 0, os, oe, derr, nullptr));
-      HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, data, dmoffs, runs, dslots, (int)slots.size(), pos, acc, counts,
-                                cen, newc, dbits, derr));
+      HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, dslots, (int)slots.size(), pos, acc, counts, dbits, derr));
       int herr = 0;
       HIPC(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       HIPC(c, hipMemcpyAsync(bits.data(), dbits, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
   const Geo g = make_geo(J.ly, J.lx, os, oe, abr[jid]);
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
-  int4* R = runs + J.runs_off;
+  int4* R = J.rp ? J.rp : runs + J.runs_off;
   unsigned long long* P = (REC && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
   const bool flip = J.flip != 0;
   const int top = (g.oe && g.ly >= 2) ? g.ly - 1 : INT32_MAX;  // open-end top row (its own words)
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
   const int r0 = os ? 1 : 0;
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
-  int4* R = runs + J.runs_off;
+  int4* R = J.rp ? J.rp : runs + J.runs_off;
   unsigned long long* P = (pbuf && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
   int i = wave_uniform(S.i), j = wave_uniform(S.j);
   RunRec rr;
@@ -454,18 +454,18 @@ __device__ __forceinline__ int slot_of(const MultiSlot* __restrict__ sl, int ns,
 }
 
 template <typename T>
-__global__ void k_accumulate_multi(const T* __restrict__ data, const long long* __restrict__ moffs,
-                                   const int4* __restrict__ runs, const MultiSlot* __restrict__ sl, int ns,
-                                   long long total, T* __restrict__ acc, uint32_t* __restrict__ counts) {
+__global__ void k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total, T* __restrict__ acc,
+                                   uint32_t* __restrict__ counts) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const MultiSlot S = sl[slot_of<T>(sl, ns, g)];
   const int c = (int)(g - S.pos0);
+  const T* data = (const T*)S.data;
   T a = T(0);
   uint32_t n = 0;
   for (int s = 0; s < S.nmem; s++) {
-    const int4 r = runs[S.runs_off + (long long)s * S.lc + c];
-    const T* p = data + moffs[S.mem_off + s];
+    const int4 r = S.runs[(long long)s * S.lc + c];
+    const T* p = data + S.moffs[s];
     if (r.y >= r.x) {
       for (int k = r.y; k >= r.x; k--) a = (T)(a + p[k]);
       n += (uint32_t)(r.y - r.x + 1);
@@ -481,9 +481,8 @@ __global__ void k_accumulate_multi(const T* __restrict__ data, const long long* 
 
 template <typename T>
 __global__ void k_mean_delta_multi(const T* __restrict__ acc, const uint32_t* __restrict__ counts,
-                                   const T* __restrict__ oldc, T* __restrict__ newc, const MultiSlot* __restrict__ sl,
-                                   int ns, long long total, unsigned long long* __restrict__ delta_bits,
-                                   int* __restrict__ err) {
+                                   const MultiSlot* __restrict__ sl, int ns, long long total,
+                                   unsigned long long* __restrict__ delta_bits, int* __restrict__ err) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const MultiSlot S = sl[slot_of<T>(sl, ns, g)];
@@ -495,8 +494,8 @@ __global__ void k_mean_delta_multi(const T* __restrict__ acc, const uint32_t* __
   } else {
     m = div_count<T>(acc[g], counts[g]);
   }
-  newc[S.cen_off + c] = m;
-  const double d = fabs((double)(T)(oldc[S.cen_off + c] - m));  // dba.hpp:830-836
+  ((T*)S.newc)[c] = m;
+  const double d = fabs((double)(T)(((const T*)S.cen)[c] - m));  // dba.hpp:830-836
   if (d == d) atomicMax(&delta_bits[S.slot], (unsigned long long)__double_as_longlong(d));
 }
 
@@ -654,27 +653,24 @@ static hipError_t md_t(hipStream_t st, const void* acc, const uint32_t* counts, 
 }
 
 template <typename T>
-static hipError_t multi_t(hipStream_t st, const void* data, const long long* moffs, const int4* runs,
-                          const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts, const void* oldc,
-                          void* newc, unsigned long long* delta_bits, int* err) {
+static hipError_t multi_t(hipStream_t st, const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts,
+                          unsigned long long* delta_bits, int* err) {
   const unsigned nb = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(k_accumulate_multi<T>, dim3(nb), dim3(256), 0, st, (const T*)data, moffs, runs, sl, ns, total,
-                     (T*)acc, counts);
-  hipLaunchKernelGGL(k_mean_delta_multi<T>, dim3(nb), dim3(256), 0, st, (const T*)acc, counts, (const T*)oldc, (T*)newc,
-                     sl, ns, total, delta_bits, err);
+  hipLaunchKernelGGL(k_accumulate_multi<T>, dim3(nb), dim3(256), 0, st, sl, ns, total, (T*)acc, counts);
+  hipLaunchKernelGGL(k_mean_delta_multi<T>, dim3(nb), dim3(256), 0, st, (const T*)acc, counts, sl, ns, total,
+                     delta_bits, err);
   return hipGetLastError();
 }
 
-hipError_t launch_multi_mean(int dtype, hipStream_t st, const void* data, const long long* moffs, const int4* runs,
-                             const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts,
-                             const void* oldc, void* newc, unsigned long long* delta_bits, int* err) {
+hipError_t launch_multi_mean(int dtype, hipStream_t st, const MultiSlot* sl, int ns, long long total, void* acc,
+                             uint32_t* counts, unsigned long long* delta_bits, int* err) {
   if (ns <= 0 || total <= 0) return hipSuccess;
   switch (dtype) {
-    case ODBA_F32: return multi_t<float>(st, data, moffs, runs, sl, ns, total, acc, counts, oldc, newc, delta_bits, err);
-    case ODBA_F64: return multi_t<double>(st, data, moffs, runs, sl, ns, total, acc, counts, oldc, newc, delta_bits, err);
-    case ODBA_I32: return multi_t<int32_t>(st, data, moffs, runs, sl, ns, total, acc, counts, oldc, newc, delta_bits, err);
-    case ODBA_U32: return multi_t<uint32_t>(st, data, moffs, runs, sl, ns, total, acc, counts, oldc, newc, delta_bits, err);
-    case ODBA_U64: return multi_t<uint64_t>(st, data, moffs, runs, sl, ns, total, acc, counts, oldc, newc, delta_bits, err);
+    case ODBA_F32: return multi_t<float>(st, sl, ns, total, acc, counts, delta_bits, err);
+    case ODBA_F64: return multi_t<double>(st, sl, ns, total, acc, counts, delta_bits, err);
+    case ODBA_I32: return multi_t<int32_t>(st, sl, ns, total, acc, counts, delta_bits, err);
+    case ODBA_U32: return multi_t<uint32_t>(st, sl, ns, total, acc, counts, delta_bits, err);
+    case ODBA_U64: return multi_t<uint64_t>(st, sl, ns, total, acc, counts, delta_bits, err);
   }
   return hipErrorInvalidValue;
 }

@@ -48,6 +48,7 @@ struct TbJob {
   long long top_off;   // word offset of the open-end top-row words
   long long path_off;  // -1, or the slot in the path buffer: [count][steps...]
   long long runs_off;  // element offset of this job's run records (one per centroid position)
+  int4* rp;            // when set: this job's run records (runs_off unused)
 };
 
 // One visited cell of a recorded traceback: row i (31 bits), column j
@@ -69,12 +70,19 @@ struct StripeState {
   int done, err;
 };
 
-// Several waves per pair (column strips 64*C wide, C in {1, 4, 16}; float,
-// int32 and uint32 whose costs cannot wrap). `waves` per workgroup <= 16;
-// esc holds 2*waves edge slots of `stride` elements per workgroup.
-hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
-                                  const void* data, const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, void* dist, void* esc, long long stride, int* flags);
+// A few very long pairs: bands of `waves` strips (64*C columns each) claimed
+// by workgroups in pair order. tab: one {int r, c; long long band0} per pair
+// in claim order (band0 = first claim index of the pair), nclaims bands in
+// all. rings: nblocks*waves*ring elements (ring a power of two >= 128);
+// gslots: R slots of gstride elements (>= the longest Y + 256); gsync: 3*R
+// ints; abr: ntab ints; err: bit 8 set when a hand-off wait timed out after
+// tick_limit wall-clock ticks. flags (optional) as for launch_all_pairs.
+hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
+                                  const void* data, const long long* offs, const int* lens, int n, const void* tab,
+                                  int ntab, long long nclaims, unsigned long long* counter, void* dist, void* rings,
+                                  int ring, void* gslots, long long gstride, int* gsync, int R, int* abr, int* err,
+                                  long long tick_limit, int* flags);
+size_t band_pair_size();
 
 // Float path jobs with a workgroup of `waves` waves per job (strips = swaths).
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
@@ -96,16 +104,17 @@ hipError_t launch_mean_delta(int dtype, hipStream_t st, const void* acc, const u
                              void* new_c, int lc, unsigned long long* delta_bits, int* err);
 // One active cluster of a packed multi-cluster DBA round.
 struct MultiSlot {
-  long long pos0;      // first global centroid position of the cluster
-  long long runs_off;  // its run records: [member][position]
-  long long cen_off;   // its centroid in the centroid pools
-  long long mem_off;   // its members' data offsets in the member-offset list
-  int lc, nmem, slot;  // centroid length, members, index of its delta word
+  long long pos0;            // first global centroid position of the cluster
+  const void* data;          // its sequence data (padded, packed)
+  const long long* moffs;    // its members' offsets into data, in cluster order
+  const int4* runs;          // its run records: [member][position]
+  const void* cen;           // its current centroid
+  void* newc;                // its next centroid
+  int lc, nmem, slot;        // centroid length, members, index of its delta word
   int pad;
 };
-hipError_t launch_multi_mean(int dtype, hipStream_t st, const void* data, const long long* moffs, const int4* runs,
-                             const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts,
-                             const void* oldc, void* newc, unsigned long long* delta_bits, int* err);
+hipError_t launch_multi_mean(int dtype, hipStream_t st, const MultiSlot* sl, int ns, long long total, void* acc,
+                             uint32_t* counts, unsigned long long* delta_bits, int* err);
 
 hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* offs, const int* lens, int n,
                         double* mu, double* sigma);

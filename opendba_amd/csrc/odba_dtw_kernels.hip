@@ -34,6 +34,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include <float.h>
 #include <limits.h>
 
@@ -231,6 +232,8 @@ template <typename T> struct SwathIO {
   uint32_t* mv;    // path words for this swath (null: distance only)
   uint32_t* mtop;  // path words of the open-end top row (64 per swath)
   int ref_left;    // 1: `off` is a reference swath start (a multiple of 1024)
+  int imask = -1;  // row index mask of ein: -1 (whole column) or ring rows - 1
+  int omask = -1;  // row index mask of eout
 };
 
 // Pipeline hooks of run_swath. A wave that sweeps a whole pair needs none.
@@ -241,6 +244,8 @@ struct NoPipe {
   static constexpr bool kPrefetch = true;  // load the next block's edge values one block early
   __device__ void wait(int) const {}
   __device__ void publish(int) const {}
+  __device__ void consumed(int) const {}  // input rows below the argument have been read
+  __device__ void reserve(int) const {}   // output rows below the argument may be written now
 };
 
 template <typename T> struct SwathRes {
@@ -289,7 +294,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
   auto load_e = [&](int b) -> T {
     if (first_swath) return MAXV;
     int r = r0 + 64 * b + lane;
-    return a.ein[r];  // edge buffers are padded by 128 entries
+    return a.ein[r & a.imask];  // edge buffers are padded by 128 entries (rings wrap)
   };
   auto rows_of_block = [&](int b) -> int { return min(r0 + 64 * b + 64, r1 + 1); };
   T ynext = nblk > 0 ? load_y(0) : T(0);
@@ -308,6 +313,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     W->y[64 + lane] = ynext;
     W->e[lane] = enext;
     wave_sync();
+    if (!Pipe::kPrefetch && !first_swath) pipe.consumed(rows_of_block(b));
     if (b + 1 < nblk) {
       ynext = load_y(b + 1);
       if (Pipe::kPrefetch) enext = load_e(b + 1);
@@ -392,8 +398,9 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     if (edge_out) {
       const int row = r0 + sbase - 64 + lane;
       T v = W->o[lane];
+      pipe.reserve(min(r0 + sbase, r1 + 1));
       if (row >= r0 && row <= r1) {
-        a.eout[row] = v;
+        a.eout[row & a.omask] = v;
         runmin = Ar<T>::mn(runmin, v);
       }
       pipe.publish(min(r0 + sbase, r1 + 1));
@@ -406,13 +413,14 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
   if (edge_out && nblk > 0) {
     const int row = r0 + 64 * nblk - 64 + lane;
     T v = W->o[lane];
+    pipe.reserve(r1 + 1);
     if (row >= r0 && row <= r1) {
-      a.eout[row] = v;
+      a.eout[row & a.omask] = v;
       runmin = Ar<T>::mn(runmin, v);
     }
   }
   if (edge_out) pipe.publish(r1 + 1);
-  if (OS && edge_out && lane == 0) a.eout[0] = T(0);
+  if (OS && edge_out && lane == 0 && a.omask == -1) a.eout[0] = T(0);  // never read back from a ring
   if (OS) runmin = Ar<T>::mn(runmin, T(0));
 
   // --- open-end top row: D(t,j) = min(min(U,G), D(t,j-1)) with U,G carrying
@@ -421,8 +429,8 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     const int t = ly - 1;
     const T yv = a.y[t];
     if (!first_swath) pipe.wait(ly);
-    const T lane0_diag = first_swath ? MAXV : a.ein[t - 1];
-    const T left0 = first_swath ? MAXV : a.ein[t];
+    const T lane0_diag = first_swath ? MAXV : a.ein[(t - 1) & a.imask];
+    const T left0 = first_swath ? MAXV : a.ein[t & a.imask];
     const T nb = Ar<T>::shr(lane0_diag, D[C - 1]);
     // pass 1: m[q] = running min over the lane of min(U, G); U = up + dd,
     // G = diag + dd except at a reference swath-left column where the
@@ -459,7 +467,8 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     if (PATH) a.mtop[lane] = word;
     if (edge_out) {
       T top = __shfl(D[C - 1], 63, 64);
-      if (lane == 0) a.eout[t] = top;
+      pipe.reserve(ly);
+      if (lane == 0) a.eout[t & a.omask] = top;
       runmin = Ar<T>::mn(runmin, top);
       pipe.publish(ly);
     }
@@ -643,25 +652,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesP
 }
 
 // ---------------------------------------------------------------------------
-// All-vs-all with a workgroup of NW waves per pair, for workloads with too
-// few pairs to give every SIMD its own waves (a few hundred short pairs, or
-// a few very long ones). The pair's columns are cut into strips 64*C wide;
-// wave w sweeps strips w, w+NW, w+2NW, ... top to bottom, one strip a few
-// 64-row blocks behind its left neighbour. Strip s reads the right-edge
-// column of strip s-1 from a global edge slot; the slot's owner and the
-// number of rows published so far are LDS words, so the hand-off costs one
-// LDS poll per 64-row block. Slots are reused round-robin (2*NW of them); a
-// producer claims a slot only after the previous owner's consumer released
-// it. Waiting only ever goes to a lower strip index, which is already
-// running or done, so the pipeline cannot deadlock.
+// Pipeline hooks for a workgroup of waves sweeping one pair in column
+// strips (k_path_strips): strip s reads the right-edge column of strip s-1
+// from a global edge slot; the slot's owner and the number of rows published
+// so far are LDS words, so the hand-off costs one LDS poll per 64-row block.
+// Slots are reused round-robin (2*NW of them); a producer claims a slot only
+// after the previous owner's consumer released it. Waiting only ever goes to
+// a lower strip index, which is already running or done, so the pipeline
+// cannot deadlock.
 //
-// The open-end early stop (dtw.hpp:74-129) is not taken: every strip is
-// swept, and the stop is evaluated afterwards at each reference swath
-// start. This gives the same distance: when the column minimum at a swath
-// boundary equals its top value, every later cell is at least that minimum
-// (costs only grow by non-negative terms; min-first types only), while the
-// free right moves of the open-end top row carry exactly that value to the
-// last column. The flags report the swath the reference stops at.
+// The open-end early stop (dtw.hpp:74-129) is not taken by strip schedules:
+// every strip is swept, and the stop is evaluated afterwards at each
+// reference swath start. This gives the same distance: when the column
+// minimum at a swath boundary equals its top value, every later cell is at
+// least that minimum (costs only grow by non-negative terms; min-first types
+// only), while the free right moves of the open-end top row carry exactly
+// that value to the last column. The flags report the swath the reference
+// stops at.
 // ---------------------------------------------------------------------------
 struct StripPipe {
   static constexpr bool kPrefetch = false;
@@ -684,93 +691,260 @@ struct StripPipe {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0) __hip_atomic_store(out_cnt, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
+  __device__ void consumed(int) const {}
+  __device__ void reserve(int) const {}
 };
 
 constexpr int kMaxStripWaves = 16;
 
+// ---------------------------------------------------------------------------
+// All-vs-all for a few very long pairs: bands of strips across workgroups.
+// A pair's columns are cut into strips 64*C wide and the strips into bands
+// of NW consecutive strips. Workgroups (NW waves) claim bands from a global
+// counter in pair order, band order; wave w of a band sweeps its strip w.
+//
+//   * Inside a band, the edge column between strips w and w+1 goes through
+//     a ring of `ring` rows in the workgroup's scratch: the producer waits
+//     before overwriting rows its consumer has not read (LDS counters).
+//   * Between bands, the band's last strip writes the whole edge column to
+//     a global slot (q mod R for claim q) and publishes its progress in
+//     global words, 256 rows at a time; the next band's first strip (the
+//     next claim) reads it. Slot q mod R is reused by claim q+R only after
+//     claim q+1 has read it, or at once when claim q hands nothing over.
+//
+// No deadlock: every wait targets a lower claim or a lower strip of the
+// same claim, except the ring wait, which targets the next strip of the
+// same band; the band's last strip never waits on a ring (its output slot
+// is a whole column), and a claim is only taken by a running workgroup.
+// Every wait also ends after `tick_limit` wall-clock ticks, setting bit 8
+// of *err, so a broken hand-off cannot hang the device.
+//
+// Distances are written by the pair's last strip. The open-end early stop
+// is reconstructed after the fact as for every strip schedule: the per-pair
+// stop swath is an atomic minimum in `abr`, turned into flags by
+// k_band_flags once every band has finished.
+// ---------------------------------------------------------------------------
+struct BandPair {
+  int r, c;         // Y = sequence r, X = sequence c
+  long long band0;  // claim index of the pair's first band
+};
+
+constexpr int kGlobalGrain = 256;  // rows per global publish
+
+__device__ __forceinline__ bool band_timed_out(long long t0, long long limit, int* err, int lane) {
+  if (wall_clock64() - t0 <= limit) return false;
+  if (lane == 0) atomicOr(err, 8);
+  return true;
+}
+
+// After one wait has timed out every other wait returns at once, so the
+// launch drains quickly (with garbage results, reported through *err).
+__device__ __forceinline__ bool band_failed(int* err) {
+  return (uniform_i32(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 8) != 0;
+}
+
+struct BandPipe {
+  static constexpr bool kPrefetch = false;
+  const int* in_cnt;    // rows published into the input (LDS, or a global word)
+  const int* in_owner;  // global input: claim index that owns the slot
+  int in_tag;           // the left band's claim index
+  int in_global;
+  int* in_used;         // ring input: LDS word of rows read so far
+  int* out_cnt;         // rows published of the output (null: none)
+  int out_global;
+  const int* out_used;  // ring output: the consumer's rows read
+  int ring;
+  int main_end;         // rows of the main loop (r1 + 1): always published
+  int lane;
+  int* err;
+  long long limit;
+  mutable int owned = 0;
+  mutable int last_pub = 0;
+
+  __device__ void wait(int rows) const {
+    if (band_failed(err)) return;
+    const long long t0 = wall_clock64();
+    if (in_global) {
+      while (!owned) {
+        const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (o == in_tag) owned = 1;
+        else {
+          if (band_timed_out(t0, limit, err, lane)) return;
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      for (;;) {
+        const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (c >= rows) break;
+        if (band_timed_out(t0, limit, err, lane)) return;
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    } else {
+      for (;;) {
+        const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (c >= rows) break;
+        if (band_timed_out(t0, limit, err, lane)) return;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
+  __device__ void publish(int rows) const {
+    if (!out_cnt) return;
+    if (out_global) {
+      if (rows < main_end && rows - last_pub < kGlobalGrain) return;
+      last_pub = rows;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0) __hip_atomic_store(out_cnt, rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) __hip_atomic_store(out_cnt, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __device__ void consumed(int rows) const {
+    if (!in_used) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(in_used, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ void reserve(int rows) const {
+    if (!out_used) return;
+    const int need = rows - ring;
+    if (need <= 0) return;
+    if (band_failed(err)) return;
+    const long long t0 = wall_clock64();
+    for (;;) {
+      const int u = uniform_i32(__hip_atomic_load(out_used, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (u >= need) break;
+      if (band_timed_out(t0, limit, err, lane)) return;
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+};
+
 template <typename T, int C, bool OS, bool OE>
-__global__ void __launch_bounds__(1024) k_all_pairs_strips(const T* __restrict__ data, const long long* __restrict__ offs,
-                                                           const int* __restrict__ lens, int n, long long npairs,
-                                                           unsigned long long* __restrict__ counter,
-                                                           T* __restrict__ dist, T* __restrict__ escratch,
-                                                           long long stride, int* __restrict__ flags) {
+__global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ data, const long long* __restrict__ offs,
+                                                          const int* __restrict__ lens, int n,
+                                                          const BandPair* __restrict__ tab, int ntab, long long nclaims,
+                                                          unsigned long long* __restrict__ counter, T* __restrict__ dist,
+                                                          T* __restrict__ rings, int ring, T* __restrict__ gslots,
+                                                          long long gstride, int* __restrict__ gsync, int R,
+                                                          int* __restrict__ abr, int* __restrict__ err, long long limit) {
   __shared__ T s_y[kMaxStripWaves][128];
   __shared__ T s_e[kMaxStripWaves][64];
   __shared__ T s_o[kMaxStripWaves][128];
-  __shared__ int s_owner[2 * kMaxStripWaves];
-  __shared__ int s_cnt[2 * kMaxStripWaves];
-  __shared__ int s_free[2 * kMaxStripWaves];
-  __shared__ unsigned long long s_p;
-  __shared__ int s_abr;
-  __shared__ T s_dist;
+  __shared__ int s_cnt[kMaxStripWaves];   // rows published into ring w
+  __shared__ int s_used[kMaxStripWaves];  // rows of ring w read by wave w+1
+  __shared__ unsigned long long s_q;
   constexpr int SW = 64 * C;
   const int lane = threadIdx.x & 63;
   const int wid = uniform_i32((int)(threadIdx.x >> 6));
   const int nw = uniform_i32((int)(blockDim.x >> 6));
-  const int nslot = 2 * nw;
-  T* ebase = escratch + (long long)blockIdx.x * nslot * stride;
+  T* rbase = rings + (long long)blockIdx.x * nw * ring;
+  int* g_owner = gsync;
+  int* g_cnt = gsync + R;
+  int* g_rel = gsync + 2 * R;
   const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
   for (;;) {
-    __syncthreads();  // every wave is done with the previous pair
-    if (threadIdx.x == 0) {
-      s_p = atomicAdd(counter, 1ull);
-      s_abr = INT_MAX;
-    }
-    if (threadIdx.x < 2 * kMaxStripWaves) {
-      s_owner[threadIdx.x] = -1;
+    __syncthreads();  // every wave is done with the previous claim
+    if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
+    if (threadIdx.x < kMaxStripWaves) {
       s_cnt[threadIdx.x] = 0;
-      s_free[threadIdx.x] = 1;
+      s_used[threadIdx.x] = 0;
     }
     __syncthreads();
-    const long long p = (long long)uniform_u64(s_p);
-    if (p >= npairs) break;
-    int r, c;
-    decode_pair(p, npairs, r, c);
-    const T* y = data + offs[r];
-    const T* x = data + offs[c];
+    const long long q = (long long)uniform_u64(s_q);
+    if (q >= nclaims) break;
+    int lo = 0, hi = ntab - 1;  // the pair: last table entry with band0 <= q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tab[mid].band0 <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const int t = uniform_i32(lo);
+    const int r = uniform_i32(tab[t].r), c = uniform_i32(tab[t].c);
+    const int b = uniform_i32((int)(q - tab[t].band0));
     const int ly = lens[r], lx = lens[c];
     const int S = (lx + SW - 1) / SW;
-    for (int s = wid; s < S; s += nw) {
-      const int off = s * SW;
-      const int so = s % nslot, si = (s + nslot - 1) % nslot;
-      const bool has_out = (s < S - 1);
-      if (has_out) {  // claim the output slot
-        for (;;) {
-          const int f = uniform_i32(__hip_atomic_load(&s_free[so], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-          if (f) break;
-          __builtin_amdgcn_s_sleep(2);
+    const int s0 = b * nw;
+    const int slast = min(s0 + nw, S) - 1;
+    const int s = s0 + wid;
+    if (s > slast) continue;  // no strip for this wave in the pair's last band
+    const int off = s * SW;
+    const bool has_out = s < S - 1;
+    const bool in_global = (wid == 0) && s > 0;
+    const bool out_global = (s == slast) && has_out;
+    const int gin = (int)((q + R - 1) % R), gout = (int)(q % R);
+    if (s == slast && !band_failed(err)) {  // this claim's turn on global slot q mod R
+      const long long t0 = wall_clock64();
+      for (;;) {
+        const int rel = uniform_i32(__hip_atomic_load(&g_rel[gout], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if ((long long)rel >= q - R) break;
+        if (band_timed_out(t0, limit, err, lane)) break;
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (lane == 0) {
+        if (out_global) {
+          __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&g_owner[gout], (int)q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          __hip_atomic_store(&g_rel[gout], (int)q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (lane == 0) {
-          __hip_atomic_store(&s_free[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_store(&s_cnt[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_store(&s_owner[so], s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
       }
-      StripPipe pp{&s_owner[si], &s_cnt[si], s - 1, has_out ? &s_cnt[so] : nullptr, lane};
-      SwathIO<T> io;
-      io.y = y; io.ly = ly; io.x = x; io.lx = lx; io.off = off;
-      io.ein = s > 0 ? ebase + (long long)si * stride : nullptr;
-      io.eout = has_out ? ebase + (long long)so * stride : nullptr;
-      io.mv = nullptr; io.mtop = nullptr;
-      io.ref_left = (off % ODBA_W) == 0;
-      SwathRes<T> res = run_swath<T, C, OS, OE, true, false, StripPipe>(io, &win, lane, pp);
-      if (s > 0) {  // the input slot is consumed
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (lane == 0) __hip_atomic_store(&s_free[si], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      if (OE && !OS && has_out) {
-        const int next = off + SW;
-        if ((next % ODBA_W) == 0 && next > ly && res.top == res.colmin && lane == 0) atomicMin(&s_abr, next / ODBA_W);
-      }
-      if (!has_out && lane == 0) s_dist = pair_distance<T>(res.last, ly, OE != OS);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    BandPipe pp;
+    pp.in_global = in_global;
+    pp.in_cnt = in_global ? &g_cnt[gin] : (s > 0 ? &s_cnt[wid - 1] : nullptr);
+    pp.in_owner = &g_owner[gin];
+    pp.in_tag = (int)(q - 1);
+    pp.in_used = (s > 0 && !in_global) ? &s_used[wid - 1] : nullptr;
+    pp.out_cnt = has_out ? (out_global ? &g_cnt[gout] : &s_cnt[wid]) : nullptr;
+    pp.out_global = out_global;
+    pp.out_used = (has_out && !out_global) ? &s_used[wid] : nullptr;
+    pp.ring = ring;
+    pp.main_end = (OE && ly >= 2) ? ly - 1 : ly;
+    pp.lane = lane;
+    pp.err = err;
+    pp.limit = limit;
+    SwathIO<T> io;
+    io.y = data + offs[r]; io.ly = ly; io.x = data + offs[c]; io.lx = lx; io.off = off;
+    io.ein = s == 0 ? nullptr : (in_global ? gslots + (long long)gin * gstride : rbase + (long long)(wid - 1) * ring);
+    io.imask = in_global ? -1 : ring - 1;
+    io.eout = has_out ? (out_global ? gslots + (long long)gout * gstride : rbase + (long long)wid * ring) : nullptr;
+    io.omask = out_global ? -1 : ring - 1;
+    io.mv = nullptr; io.mtop = nullptr;
+    io.ref_left = (off % ODBA_W) == 0;
+    SwathRes<T> res = run_swath<T, C, OS, OE, true, false, BandPipe>(io, &win, lane, pp);
+    if (in_global && lane == 0)  // the left band's slot is free for claim q-1+R
+      __hip_atomic_store(&g_rel[gin], (int)(q - 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (OE && !OS && has_out) {
+      const int next = off + SW;
+      if ((next % ODBA_W) == 0 && next > ly && res.top == res.colmin && lane == 0) atomicMin(&abr[t], next / ODBA_W);
+    }
+    if (s == S - 1 && lane == 0) {
       const long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
-      dist[idx] = s_dist;
-      if (flags) flags[idx] = (s_abr != INT_MAX) ? (1 | (s_abr << 8)) : 0;
+      dist[idx] = pair_distance<T>(res.last, ly, OE != OS);
     }
   }
+}
+
+__global__ void k_band_init(int* __restrict__ gsync, int R, int* __restrict__ abr, int ntab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < R) {
+    gsync[i] = -1;          // owner
+    gsync[R + i] = 0;       // rows published
+    gsync[2 * R + i] = i - R;  // last claim done with the slot (a virtual one before the first use)
+  }
+  if (i < ntab) abr[i] = INT_MAX;
+}
+
+__global__ void k_band_flags(const BandPair* __restrict__ tab, int ntab, int n, const int* __restrict__ abr,
+                             int* __restrict__ flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntab) return;
+  const int r = tab[t].r, c = tab[t].c;
+  const long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
+  flags[idx] = abr[t] != INT_MAX ? (1 | (abr[t] << 8)) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +990,7 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
 // of a cluster with tens to hundreds of members). The strips are the
 // reference's 1024-column swaths, so the step words land exactly where the
 // one-wave kernel puts them; the edge hand-off and the after-the-fact
-// open-end stop are those of k_all_pairs_strips (the words of swaths past
+// open-end stop are those of the strip schedules (the words of swaths past
 // the stop are written but never read: the traceback treats those cells
 // positionally).
 // ---------------------------------------------------------------------------
@@ -1017,28 +1191,6 @@ static hipError_t launch_all_pairs_t(int os, int oe, dim3 grid, hipStream_t st, 
   return hipGetLastError();
 }
 
-template <typename T, int C>
-static hipError_t launch_strips_c(int os, int oe, dim3 grid, dim3 block, hipStream_t st, const T* data,
-                                  const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, T* dist, T* esc, long long stride, int* flags) {
-#define ODBA_STRIPS(A, B) hipLaunchKernelGGL((k_all_pairs_strips<T, C, A, B>), grid, block, 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags)
-  if (os && oe) ODBA_STRIPS(true, true); else if (os) ODBA_STRIPS(true, false); else if (oe) ODBA_STRIPS(false, true); else ODBA_STRIPS(false, false);
-#undef ODBA_STRIPS
-  return hipGetLastError();
-}
-
-template <typename T>
-static hipError_t launch_strips_t(int C, int os, int oe, dim3 grid, dim3 block, hipStream_t st, const T* data,
-                                  const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, T* dist, T* esc, long long stride, int* flags) {
-  switch (C) {
-    case 1: return launch_strips_c<T, 1>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-    case 4: return launch_strips_c<T, 4>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-    case 16: return launch_strips_c<T, 16>(os, oe, grid, block, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-  }
-  return hipErrorInvalidValue;
-}
-
 template <typename T, bool MINFIRST>
 static hipError_t launch_path_jobs_t(int os, int oe, dim3 grid, hipStream_t st, const void* jobs, int njobs,
                                      unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, T* esc,
@@ -1071,18 +1223,51 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_all_pairs_strips(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
-                                  const void* data, const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, void* dist, void* esc, long long stride, int* flags) {
-  if (waves < 1 || waves > kMaxStripWaves) return hipErrorInvalidValue;
-  dim3 g(nblocks), b(64 * waves);
-  switch (dtype) {
-    case ODBA_F32: return launch_strips_t<float>(C, os, oe, g, b, st, (const float*)data, offs, lens, n, npairs, counter, (float*)dist, (float*)esc, stride, flags);
-    case ODBA_I32: return launch_strips_t<int32_t>(C, os, oe, g, b, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
-    case ODBA_U32: return launch_strips_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
-  }
-  return hipErrorInvalidValue;
+template <typename T, int C>
+static void launch_bands_c(int os, int oe, dim3 g, dim3 b, hipStream_t st, const T* data, const long long* offs,
+                           const int* lens, int n, const BandPair* tab, int ntab, long long nclaims,
+                           unsigned long long* counter, T* dist, T* rings, int ring, T* gslots, long long gstride,
+                           int* gsync, int R, int* abr, int* err, long long limit) {
+#define ODBA_BANDS(A, B) hipLaunchKernelGGL((k_all_pairs_bands<T, C, A, B>), g, b, 0, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit)
+  if (os && oe) ODBA_BANDS(true, true); else if (os) ODBA_BANDS(true, false); else if (oe) ODBA_BANDS(false, true); else ODBA_BANDS(false, false);
+#undef ODBA_BANDS
 }
+
+template <typename T>
+static hipError_t launch_bands_t(int C, int os, int oe, dim3 g, dim3 b, hipStream_t st, const T* data,
+                                 const long long* offs, const int* lens, int n, const BandPair* tab, int ntab,
+                                 long long nclaims, unsigned long long* counter, T* dist, T* rings, int ring, T* gslots,
+                                 long long gstride, int* gsync, int R, int* abr, int* err, long long limit) {
+  switch (C) {
+    case 1: launch_bands_c<T, 1>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
+    case 4: launch_bands_c<T, 4>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
+    case 16: launch_bands_c<T, 16>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
+                                  const void* data, const long long* offs, const int* lens, int n, const void* tab,
+                                  int ntab, long long nclaims, unsigned long long* counter, void* dist, void* rings,
+                                  int ring, void* gslots, long long gstride, int* gsync, int R, int* abr, int* err,
+                                  long long tick_limit, int* flags) {
+  if (waves < 1 || waves > kMaxStripWaves || ring < 128 || (ring & (ring - 1)) || R < 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_band_init, dim3((std::max(R, ntab) + 255) / 256), dim3(256), 0, st, gsync, R, abr, ntab);
+  dim3 g(nblocks), b(64 * waves);
+  const BandPair* T_ = (const BandPair*)tab;
+  hipError_t e = hipErrorInvalidValue;
+  switch (dtype) {
+    case ODBA_F32: e = launch_bands_t<float>(C, os, oe, g, b, st, (const float*)data, offs, lens, n, T_, ntab, nclaims, counter, (float*)dist, (float*)rings, ring, (float*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
+    case ODBA_I32: e = launch_bands_t<int32_t>(C, os, oe, g, b, st, (const int32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (int32_t*)dist, (int32_t*)rings, ring, (int32_t*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
+    case ODBA_U32: e = launch_bands_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (uint32_t*)dist, (uint32_t*)rings, ring, (uint32_t*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
+  }
+  if (e != hipSuccess) return e;
+  if (flags) hipLaunchKernelGGL(k_band_flags, dim3((ntab + 255) / 256), dim3(256), 0, st, T_, ntab, n, abr, flags);
+  return hipGetLastError();
+}
+
+size_t band_pair_size() { return sizeof(BandPair); }
 
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,

@@ -28,6 +28,11 @@
  *   stripe traceback          equals the full-matrix path in global mode
  *                             (brute-force comparison); the open-end height
  *                             quirk has no fixture: parity unpinned there
+ *   prefix_chop               closed forms: an exact prefix match (chop at
+ *                             P-1, one cell per prefix row) and a prefix
+ *                             whose values each occur twice (hand-derived
+ *                             chop and histogram); histogram sum = path
+ *                             length minus open moves
  */
 #include <algorithm>
 #include <atomic>
@@ -282,6 +287,45 @@ int traceback_full(const uint8_t* M, size_t rows, size_t cols, int flip, const T
     counts[0]++;
   }
   if (path_len) *path_len = steps + 1;
+  return ODBA_OK;
+}
+
+/*
+ * chopPrefixFromSequences<T> for one (already normalised) sequence
+ * (dba.hpp:1192-1395): open-end DTW with the prefix on Y (rows) and the
+ * sequence on X, W = 1024 (dba.hpp:1219-1224, 1264-1280), then the host walk
+ * of dba.hpp:1317-1336: from the top-right corner, step left over the top
+ * row's OPEN_RIGHT run; the column reached is the chop position. hist[i] then
+ * counts the path cells in prefix row i from that cell down to (0,0); the
+ * first cell is set to 1 (dba.hpp:1329). hist starts at zero (the reference's
+ * buffer is fresh pinned pages).
+ */
+template <typename T>
+int prefix_chop(const T* prefix, size_t P, const T* seq, size_t L, uint64_t* chop, int32_t* hist) {
+  if (P == 0 || L == 0) return ODBA_E_INVALID;
+  std::vector<uint8_t> M(P * L, 0);
+  T dist;
+  int rc = dtw_pair<T>(prefix, P, seq, L, 1024, 0, 1, M.data(), &dist, nullptr, nullptr);
+  if (rc) return rc;
+  long i = (long)P - 1, j = (long)L - 1;
+  uint8_t mv = M[i * L + j];
+  while (mv == ODBA_OPEN_RIGHT) {
+    i += kMoveI[mv];
+    j += kMoveJ[mv];
+    if (j < 0) return ODBA_E_TRACEBACK;
+    mv = M[i * L + j];
+  }
+  *chop = (uint64_t)j;
+  for (size_t k = 0; k < P; k++) hist[k] = 0;
+  hist[i] = 1;
+  while (mv != ODBA_NIL) {
+    if (mv == 0 || mv > 5) return ODBA_E_TRACEBACK;  // an unwritten cell: never reached
+    i += kMoveI[mv];
+    j += kMoveJ[mv];
+    if (i < 0 || j < 0) return ODBA_E_TRACEBACK;
+    hist[i]++;
+    mv = M[i * L + j];
+  }
   return ODBA_OK;
 }
 
@@ -905,6 +949,11 @@ int orc_perform_dba(int dtype, void* const* seqs, const uint64_t* lens, uint64_t
                     uint64_t* centroid_len, int* medoid_sorted, int* perm_out, int* rounds_out, void* dists_out) {
   ORC_DISPATCH(dtype, perform_dba<T>((T* const*)seqs, lens, n, os, oe, norm, max_rounds, medoid_w, path_policy, free_mem,
                                      (T*)centroid_out, centroid_len, medoid_sorted, perm_out, rounds_out, (T*)dists_out))
+}
+
+int orc_prefix_chop(int dtype, const void* prefix, uint64_t P, const void* seq, uint64_t L, uint64_t* chop,
+                    int32_t* hist) {
+  ORC_DISPATCH(dtype, prefix_chop<T>((const T*)prefix, P, (const T*)seq, L, chop, hist))
 }
 
 /* out must hold sum over sequences of mk * ceil(L / (256 * ds)) values. */

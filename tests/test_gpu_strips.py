@@ -1,6 +1,7 @@
-"""The workgroup-per-pair all-vs-all schedule (column strips 64*C wide swept by
-several waves, edges handed over through LDS-signalled slots) against the
-oracle, in every mode and strip width, and against the one-wave-per-pair
+"""The banded all-vs-all schedule (column strips 64*C wide, bands of strips
+claimed by workgroups, edges handed over through rings inside a workgroup and
+whole-column global slots between bands) against the oracle, in every mode,
+strip width, band width and ring size, and against the one-wave-per-pair
 schedule for the open-end abrogation flags it reconstructs after the fact.
 """
 import contextlib
@@ -13,6 +14,8 @@ import oracle_lib as O
 from opendba_amd import native as N
 
 pytestmark = pytest.mark.gpu
+# a broken hand-off would end each wait after 10 s and report an error
+os.environ.setdefault("ODBA_BAND_WAIT_S", "10")
 MODES = [(0, 0), (0, 1), (1, 0), (1, 1)]
 
 
@@ -24,16 +27,31 @@ def ctx():
 
 
 @contextlib.contextmanager
-def schedule(name):
-    old = os.environ.get("ODBA_PAIR_SCHEDULE")
-    os.environ["ODBA_PAIR_SCHEDULE"] = name
+def env(**kv):
+    old = {k: os.environ.get(k) for k in kv}
+    for k, v in kv.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = str(v)
     try:
         yield
     finally:
-        if old is None:
-            del os.environ["ODBA_PAIR_SCHEDULE"]
-        else:
-            os.environ["ODBA_PAIR_SCHEDULE"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def schedule(name):
+    return env(ODBA_PAIR_SCHEDULE=name)
+
+
+# (strips per band, ring rows): the default (up to 16 strips per band, 2048
+# rows), and narrow bands with small rings so that bands hand over between
+# workgroups and the rings wrap many times.
+BANDS = {"default": (None, None), "w2r128": (2, 128), "w3r256": (3, 256)}
 
 
 def _walks(rng, lens, dtype=np.float32):
@@ -54,31 +72,51 @@ def _run(ctx, seqs, os_, oe, sched):
 CASES = {"c1": (200, 400, 12), "c4": (1100, 3000, 8), "c16": (4200, 9000, 6)}
 
 
+@pytest.mark.parametrize("band", list(BANDS))
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("os_,oe", MODES)
-def test_strip_schedule_bitexact(ctx, case, os_, oe):
+def test_strip_schedule_bitexact(ctx, case, os_, oe, band):
     lo, hi, n = CASES[case]
     rng = np.random.default_rng(10 * list(CASES).index(case) + 2 * os_ + oe)
     lens = rng.integers(lo, hi + 1, size=n)
     seqs = O.znorm(_walks(rng, lens))[0]
-    print(f"strips {case} os={os_} oe={oe} lens {lens.min()}..{lens.max()}", flush=True)
-    got, flags = _run(ctx, seqs, os_, oe, "strip")
+    print(f"strips {case} os={os_} oe={oe} band {band} lens {lens.min()}..{lens.max()}", flush=True)
+    w, r = BANDS[band]
+    with env(ODBA_BAND_WAVES=w, ODBA_BAND_RING=r):
+        got, flags = _run(ctx, seqs, os_, oe, "strip")
     exp = O.all_pairs(seqs, os_, oe)
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
     _, flags_w = _run(ctx, seqs, os_, oe, "wave")
     assert np.array_equal(flags, flags_w)
 
 
-def test_strip_schedule_open_end_abrogation(ctx):
+@pytest.mark.parametrize("band", list(BANDS))
+def test_strip_schedule_open_end_abrogation(ctx, band):
     rng = np.random.default_rng(7)
     seqs = O.znorm(_walks(rng, [150, 300, 900, 5000, 7000, 9000]))[0]
-    print("strips abrogation", flush=True)
-    got, flags = _run(ctx, seqs, 0, 1, "strip")
+    print(f"strips abrogation band {band}", flush=True)
+    w, r = BANDS[band]
+    with env(ODBA_BAND_WAVES=w, ODBA_BAND_RING=r):
+        got, flags = _run(ctx, seqs, 0, 1, "strip")
     exp = O.all_pairs(seqs, 0, 1)
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
     _, flags_w = _run(ctx, seqs, 0, 1, "wave")
     assert (flags_w & 1).sum() > 0
     assert np.array_equal(flags, flags_w)
+
+
+def test_band_schedule_one_strip_per_band(ctx):
+    """Every strip its own band: each edge column between strips crosses
+    workgroups through a global slot, and slots are reused many times over
+    (far more claims than workgroups)."""
+    rng = np.random.default_rng(12)
+    seqs = O.znorm(_walks(rng, [5000, 6100, 7000, 8300]))[0]
+    for os_, oe in MODES:
+        print(f"bands of one strip os={os_} oe={oe}", flush=True)
+        with env(ODBA_BAND_WAVES=1, ODBA_BAND_RING=128):
+            got, _ = _run(ctx, seqs, os_, oe, "strip")
+        exp = O.all_pairs(seqs, os_, oe)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (os_, oe)
 
 
 def test_strip_schedule_uneven_lengths(ctx):
@@ -88,10 +126,13 @@ def test_strip_schedule_uneven_lengths(ctx):
     lens = [64, 65, 700, 1023, 1024, 1025, 2048, 3000, 5000]
     seqs = O.znorm(_walks(rng, lens))[0]
     for os_, oe in MODES:
-        print(f"strips uneven os={os_} oe={oe}", flush=True)
-        got, _ = _run(ctx, seqs, os_, oe, "strip")
-        exp = O.all_pairs(seqs, os_, oe)
-        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (os_, oe)
+        for band in ("default", "w2r128"):
+            print(f"strips uneven os={os_} oe={oe} band {band}", flush=True)
+            w, r = BANDS[band]
+            with env(ODBA_BAND_WAVES=w, ODBA_BAND_RING=r):
+                got, _ = _run(ctx, seqs, os_, oe, "strip")
+            exp = O.all_pairs(seqs, os_, oe)
+            assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (os_, oe, band)
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.uint32])
@@ -101,7 +142,8 @@ def test_strip_schedule_integers(ctx, dtype):
     seqs = [rng.integers(lo, 60, size=L).astype(dtype) for L in (300, 700, 1500, 2600)]
     for os_, oe in MODES:
         print(f"strips {np.dtype(dtype).name} os={os_} oe={oe}", flush=True)
-        got, _ = _run(ctx, seqs, os_, oe, "strip")
+        with env(ODBA_BAND_WAVES=2, ODBA_BAND_RING=128):
+            got, _ = _run(ctx, seqs, os_, oe, "strip")
         exp = O.all_pairs(seqs, os_, oe)
         assert np.array_equal(got, exp), (os_, oe)
 
