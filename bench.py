@@ -152,6 +152,61 @@ def run_small32(args, N, WL):
     return line
 
 
+def run_stripes(args, N, WL):
+    """Few very long pairs: 8 float32 sequences of length 1M-4M (a shared
+    random-walk motif, time-warped per sequence by a seeded monotone warp,
+    plus N(0, 0.1) noise), global full unbanded all-vs-all on the 28 pairs
+    with the banded strip schedule. The result is checked against the same
+    run on the reversed sequence order (Y and X swapped in every pair): the
+    global distance is symmetric under transposition, bit for bit."""
+    n, lmin, lmax = args.long_n, args.long_min, args.long_max
+    seqs = WL.warped_motif_set(n, lmin, lmax, seed=2)
+    print(f"stripes: {n} sequences, {sum(len(s) for s in seqs):,} samples", file=sys.stderr, flush=True)
+    lens = [len(s) for s in seqs]
+    nominal = WL.pair_cells(lens)
+    with N.Context(0) as ctx:
+        info = ctx.device_info()
+        sset = ctx.seqset(seqs)
+        kms = []
+        d = None
+        t0 = time.perf_counter()
+        for step in range(args.steps):
+            d, _, ms = sset.all_pairs(0, 0, copy_out=True)
+            kms.append(ms)
+            print(f"step {step}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        sset.close()
+        rev = ctx.seqset(seqs[::-1])
+        dr, _, rms = rev.all_pairs(0, 0, copy_out=True)
+        rev.close()
+        print(f"reversed order: kernel {rms:.1f} ms", file=sys.stderr, flush=True)
+    # pair (r, c) of the reversed set is pair (n-1-c, n-1-r) of the forward one, transposed
+    iu = np.triu_indices(n, 1)
+    F = np.zeros((n, n), dtype=np.float32)
+    F[iu] = d
+    R = np.zeros((n, n), dtype=np.float32)
+    R[iu] = dr
+    back = R[::-1, ::-1].T[iu]
+    same = bool(np.array_equal(back.view(np.uint32), d.view(np.uint32)))
+    kmed = float(np.median(kms))
+    value = nominal / (kmed / 1000.0) / 1e9
+    achieved = nominal * VALU_OPS_PER_CELL / (kmed / 1000.0) / 1e12
+    peak = FP32_VECTOR_TFLOPS / 2.0
+    return {
+        "metric": METRIC, "value": round(value, 1), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"few long pairs: {n} float32 warped-motif sequences, lengths U[{lmin},{lmax}], "
+                               f"global full unbanded all-vs-all ({n * (n - 1) // 2} pairs), banded strip schedule",
+                   "lengths": lens, "full_matrix_cells": nominal, "device": info["name"],
+                   "kernel_ms_median": round(kmed, 1), "transposed_run_bit_identical": same},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "TOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_all_pairs_bands<float, C=16, os=0, oe=0>", "kernel_ms": round(kmed, 1),
+                     "ops_per_cell": VALU_OPS_PER_CELL},
+    }
+
+
 def run_serve(args, N, WL):
     """Serving: a seeded Poisson trace (mean 20 jobs/s) of DBA jobs with
     16-512 series (scaled by --scale), lengths log-uniform 500-50k (scaled),
@@ -278,7 +333,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--workload", choices=["squiggles", "small32", "serve", "clusters"], default="squiggles")
+    ap.add_argument("--workload", choices=["squiggles", "small32", "serve", "clusters", "stripes"], default="squiggles")
+    ap.add_argument("--long-n", type=int, default=8, help="stripes: sequences")
+    ap.add_argument("--long-min", type=int, default=1_000_000, help="stripes: shortest length")
+    ap.add_argument("--long-max", type=int, default=4_000_000, help="stripes: longest length")
     ap.add_argument("--n-series", type=int, default=5000, help="clusters: squiggles")
     ap.add_argument("--clusters", type=int, default=250, help="clusters: K (clusters of 2+ members)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
@@ -303,6 +361,9 @@ def main():
         return
     if args.workload == "clusters":
         print(json.dumps(run_clusters(args, N, WL)))
+        return
+    if args.workload == "stripes":
+        print(json.dumps(run_stripes(args, N, WL)))
         return
     raw = WL.squiggles(args.n, 2000, 20000, seed=1)
     seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))

@@ -201,3 +201,21 @@ def test_path_strips_dba_round(ctx, os_, oe):
     assert np.array_equal(counts, ecounts)
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
     assert delta == edelta
+
+
+def test_band_schedule_long_pairs_default_bands(ctx):
+    """Sequences long enough for the automatic choice (bands of 16 strips of
+    1024 columns, several bands per pair, 2048-row rings) against the oracle,
+    and the global distance's transposition symmetry on the reversed order."""
+    from opendba_amd import workloads as WL
+    seqs = WL.warped_motif_set(5, 30_000, 60_000, seed=3)
+    print("long pairs, default bands", flush=True)
+    got, _ = _run(ctx, seqs, 0, 0, "auto")
+    exp = O.all_pairs(seqs, 0, 0)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    rev, _ = _run(ctx, seqs[::-1], 0, 0, "auto")
+    n = len(seqs)
+    iu = np.triu_indices(n, 1)
+    R = np.zeros((n, n), dtype=np.float32)
+    R[iu] = rev
+    assert np.array_equal(R[::-1, ::-1].T[iu].view(np.uint32), got.view(np.uint32))
