@@ -180,6 +180,21 @@ int odba_perform_dba(odba_ctx* ctx, int dtype, const void* const* seqs, const ui
                      int* perm_out, int* rounds_out, void* dists_out);
 
 /*
+ * Prefix location (chopPrefixFromSequences<T>, dba.hpp:1192-1395): each
+ * sequence is aligned against the prefix (prefix on Y, open end, swath width
+ * 1024, path kept), and the path is walked back from the top-right corner.
+ * chop_pos[i] = the column where the walk leaves the top row's open-right
+ * run; hist[i * prefix_len + r] = the number of path cells in prefix row r
+ * from there down to (0,0) (the reference's leader-path histogram). With
+ * norm, the sequences are z-normalised in place first and the prefix is
+ * z-normalised in a private copy, as the reference does. The caller cuts
+ * each sequence at chop_pos (the reference keeps seq[chop_pos:]).
+ * prefix_len >= 2.
+ */
+int odba_prefix_chop(odba_ctx* ctx, int dtype, const void* prefix, uint64_t prefix_len, void* const* seqs,
+                     const uint64_t* lens, int n, int norm, uint64_t* chop_pos, int32_t* hist);
+
+/*
  * Adaptive k-segmentation (adaptive_segmentation<T>, segmentation.hpp:339-583;
  * kernel adaptive_device_segmentation<T>, segmentation.hpp:48-337): each
  * sequence is cut into chunks of 256*ceil(minseg/3) samples, every chunk is

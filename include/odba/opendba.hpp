@@ -9,6 +9,7 @@
  *   approximateMedoidIndices<T>       dba.hpp:36-365
  *   DBAUpdate<T>                      dba.hpp:432-848
  *   performDBA<T>                     dba.hpp:863-1189
+ *   chopPrefixFromSequences<T>        dba.hpp:1192-1395
  *   adaptive_segmentation<T>          segmentation.hpp:339-583
  *   setupAndRun<T>                    openDBA.cuh:13-200
  *
@@ -46,6 +47,7 @@
 #define MEDOID_FINDING_ERROR 36
 #define UNKNOWN_ALGO 42
 #define CANNOT_READ_SEQUENCE_PREFIX_FILE 24
+#define CANNOT_COPY_PREFIX_CHOPPED_SEQ 25
 #define CANNOT_WRITE_DTW_PATH_MATRIX 26
 
 namespace odba_host {
@@ -231,6 +233,54 @@ void normalizeSequences(T** sequences, size_t num_sequences, size_t* sequence_le
                    "normalizeSequences");
 }
 
+/* chopPrefixFromSequences<T> (dba.hpp:1192-1395): locates the prefix in
+ * every sequence (open-end DTW, prefix on Y), writes
+ * <output_prefix>.prefix_chop.txt (name, chop position, original length,
+ * then the per-prefix-position path histogram, tab-separated;
+ * dba.hpp:1349-1358) and replaces each sequence by its part from the chop
+ * position on. With norm_sequences the sequences are z-normalised in place
+ * first, as the reference does, and stay so. A sequence left empty would be
+ * dropped and *num_sequences lowered (dba.hpp:1362-1374); the chop position
+ * is at most length - 1, so none is. */
+template <typename T>
+void chopPrefixFromSequences(T* sequence_prefix, size_t sequence_prefix_length, T** sequences, int* num_sequences,
+                             size_t* sequence_lengths, char** sequence_names, char* output_prefix,
+                             int norm_sequences, const void* stream = nullptr) {
+  (void)stream;
+  const int n = *num_sequences;
+  std::vector<uint64_t> l(sequence_lengths, sequence_lengths + n);
+  std::vector<uint64_t> chop(n);
+  std::vector<int32_t> hist((size_t)n * sequence_prefix_length);
+  odba_host::check(odba_prefix_chop(odba_host::ctx(), odba_host::DType<T>::v, sequence_prefix, sequence_prefix_length,
+                                    (void* const*)sequences, l.data(), n, norm_sequences, chop.data(), hist.data()),
+                   "chopPrefixFromSequences");
+  std::ofstream out((std::string(output_prefix) + ".prefix_chop.txt").c_str());
+  int kept = 0;
+  for (int i = 0; i < n; i++) {
+    out << sequence_names[i] << "\t" << chop[i] << "\t" << sequence_lengths[i];
+    for (size_t j = 0; j < sequence_prefix_length; j++) out << "\t" << hist[(size_t)i * sequence_prefix_length + j];
+    out << std::endl;
+    const size_t left = sequence_lengths[i] - (size_t)chop[i];
+    if (left == 0) {
+      std::cerr << "Skipping " << sequence_names[i] << " due to zero-length after prefix chopping" << std::endl;
+      std::free(sequences[i]);
+      continue;
+    }
+    T* cut = (T*)std::malloc(sizeof(T) * left);
+    if (!cut) {
+      std::cerr << "Running memcpy to copy prefix chopped sequence failed";
+      std::exit(CANNOT_COPY_PREFIX_CHOPPED_SEQ);
+    }
+    std::memcpy(cut, sequences[i] + chop[i], sizeof(T) * left);
+    std::free(sequences[i]);
+    sequences[kept] = cut;
+    sequence_lengths[kept] = left;
+    sequence_names[kept] = sequence_names[i];
+    kept++;
+  }
+  *num_sequences = kept;
+}
+
 namespace odba_host {
 
 inline const char* move_name(int code) {  // io_utils.hpp:303-308
@@ -384,9 +434,11 @@ int readCentroidCheckpointFromFile(const char* name, T* c, size_t len) {  // io_
   return 1;
 }
 
-/* performDBA<T> (dba.hpp:863-1189), one cluster. Reorders sequences /
- * names / lengths in place by stable length sort and z-normalises in place
- * when norm_sequences, as the reference does. */
+/* performDBA<T> (dba.hpp:863-1189): medoids and clusters by cdist, then
+ * every cluster's consensus (packed rounds on the device), written to
+ * <prefix>.avg.txt with the membership file and per-cluster path files.
+ * Reorders sequences / names / lengths in place by stable length sort and
+ * z-normalises in place when norm_sequences, as the reference does. */
 template <typename T>
 void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char** sequence_names, int use_open_start,
                 int use_open_end, char* output_prefix, int norm_sequences, double cdist, char** series_file_names,
@@ -549,9 +601,24 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
     std::exit(NOT_ENOUGH_SEQUENCES);
   }
   for (int i = 0; i < n; i++) { char* z = std::strchr(names[i], '.'); if (z) *z = '\0'; }  // openDBA.cuh:59
-  if (seqprefix_file_name != nullptr) {
-    std::cerr << "Prefix chopping (open_prefix mode) is not supported by this library" << std::endl;
-    std::exit(CANNOT_READ_SEQUENCE_PREFIX_FILE);
+  if (seqprefix_file_name != nullptr) {  // openDBA.cuh:62-83
+    T** pre = nullptr;
+    char** pre_name = nullptr;
+    size_t* pre_len = nullptr;
+    int got;
+    if (read_mode == BINARY_READ_MODE) got = readSequenceBinaryFiles<T>(&seqprefix_file_name, 1, &pre, &pre_name, &pre_len);
+    else got = readSequenceTextFiles<T>(&seqprefix_file_name, 1, &pre, &pre_name, &pre_len);
+    if (got < 1 || pre_len[0] == 0) {
+      std::cerr << "Cannot read prefix " << (read_mode == BINARY_READ_MODE ? "binary" : "text") << " data from "
+                << seqprefix_file_name << ", aborting" << std::endl;
+      std::exit(CANNOT_READ_SEQUENCE_PREFIX_FILE);
+    }
+    chopPrefixFromSequences<T>(pre[0], pre_len[0], seqs, &n, lens, names, output_prefix, norm_sequences);
+    std::free(pre[0]);
+    std::free(pre);
+    std::free(pre_name[0]);
+    std::free(pre_name);
+    std::free(pre_len);
   }
   if (minseg > 0) {  // openDBA.cuh:85-177
     T** seg = nullptr;

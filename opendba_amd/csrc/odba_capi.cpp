@@ -296,7 +296,7 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
     }
   const long long per_cu = std::max(1, 16 / ps.waves);
   const int nblocks = (int)std::max<long long>(1, std::min<long long>(nclaims, per_cu * c->cus));
-  const int R = nblocks + 2;
+  const int R = 2 * nblocks + 2;
   const size_t es = dtype_size(s->dtype);
   const long long gstride = s->maxlen + 256;
   const size_t ring_elems = (size_t)nblocks * ps.waves * ps.ring;
@@ -305,10 +305,10 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
   void* rings = c->esc;
   void* gslots = (char*)c->esc + ring_elems * es;
   DevBufs B;
-  int *gsync, *abr, *err;
-  unsigned char* dtab;
-  if ((rc = B.get(c, gsync, 3 * (size_t)R)) || (rc = B.get(c, abr, tab.size())) || (rc = B.get(c, err, 1)) ||
-      (rc = B.get(c, dtab, tab.size() * sizeof(Tab))))
+  int* err;
+  unsigned char *dtab, *sync;
+  if ((rc = B.get(c, err, 1)) || (rc = B.get(c, dtab, tab.size() * sizeof(Tab))) ||
+      (rc = B.get(c, sync, band_sync_bytes(R, nclaims, (int)tab.size()))))
     return rc;
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
@@ -320,7 +320,7 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
   HIPC(c, hipEventRecord(c->ev0, c->stream));
   hipError_t e = launch_all_pairs_bands(s->dtype, ps.C, ps.waves, os, oe, nblocks, c->stream, s->data, s->offs, s->lens,
                                         n, dtab, (int)tab.size(), nclaims, counter, dist, rings, ps.ring, gslots, gstride,
-                                        gsync, R, abr, err, limit, flags);
+                                        sync, R, err, limit, flags);
   if (e != hipSuccess) return fail(c, ODBA_E_HIP, "band launch: %s", hipGetErrorString(e));
   HIPC(c, hipEventRecord(c->ev1, c->stream));
   int herr = 0;
@@ -1246,6 +1246,125 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
   return ODBA_OK;
 }
 
+// chopPrefixFromSequences (dba.hpp:1192-1395): every sequence is aligned
+// against the prefix (prefix on Y, open end, W = 1024) as one batch of path
+// jobs, and the DBA traceback with the prefix as the "centroid" on Y gives,
+// per prefix row, the counted sequence columns. Its counts are the
+// reference's leader-path histogram (every path cell except the top row's
+// open-right run, plus (0,0); dba.hpp:1317-1336) and the first counted
+// column of the top row is the chop position. With norm, the prefix (a
+// copy) and the sequences (in place) are z-normalised first
+// (dba.hpp:1208-1216).
+template <typename T>
+int prefix_chop_t(odba_ctx* c, const T* prefix_in, int P, T* const* seqs, const uint64_t* lens, int n, int norm,
+                  uint64_t* chop, int32_t* hist) {
+  if (P < 2) return fail(c, ODBA_E_INVALID, "prefix length %d: at least 2 is needed", P);
+  if (n < 1) return fail(c, ODBA_E_INVALID, "no sequences");
+  for (int i = 0; i < n; i++)
+    if (lens[i] == 0 || lens[i] > (uint64_t)INT32_MAX / 2) return fail(c, ODBA_E_INVALID, "sequence %d has length %llu", i, (unsigned long long)lens[i]);
+  std::vector<T> pv(prefix_in, prefix_in + P);
+  if (norm) {
+    int rc = znorm_t<T>(c, seqs, lens, n, nullptr, nullptr);
+    if (rc) return rc;
+    T* pp = pv.data();
+    const uint64_t pl = (uint64_t)P;
+    rc = znorm_t<T>(c, &pp, &pl, 1, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  bool no_wrap = true;
+  if (!is_float<T>()) {
+    std::vector<const T*> all(seqs, seqs + n);
+    std::vector<uint64_t> al(lens, lens + n);
+    all.push_back(pv.data());
+    al.push_back((uint64_t)P);
+    no_wrap = int_cannot_wrap<T>(all.data(), al.data(), n + 1);
+  }
+  std::vector<long long> ho(n);
+  long long total = padded_len(P);
+  int maxlx = 0;
+  for (int i = 0; i < n; i++) {
+    ho[i] = total;
+    total += padded_len((long long)lens[i]);
+    maxlx = std::max(maxlx, (int)lens[i]);
+  }
+  std::vector<T> hb((size_t)total, T(0));
+  std::memcpy(hb.data(), pv.data(), P * sizeof(T));
+  for (int i = 0; i < n; i++) std::memcpy(&hb[ho[i]], seqs[i], lens[i] * sizeof(T));
+  DevBufs B;
+  T* data;
+  int4* runs;
+  int *abr, *derr;
+  TbJob* tbj;
+  unsigned char* jobs;
+  int rc;
+  if ((rc = B.get(c, data, hb.size())) || (rc = B.get(c, runs, (size_t)n * P)) || (rc = B.get(c, abr, n)) ||
+      (rc = B.get(c, derr, 1)) || (rc = B.get(c, tbj, n)) || (rc = B.get(c, jobs, (size_t)n * sizeof(PJ<T>))))
+    return rc;
+  HIPC(c, hipMemcpyAsync(data, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  const long long budget_words = (long long)(c->mem_cap / 4 / 4);
+  uint32_t* mv = nullptr;
+  uint32_t* mtop = nullptr;
+  size_t mv_cap = 0, mtop_cap = 0;
+  struct FreeWords {
+    uint32_t*& a; uint32_t*& b;
+    ~FreeWords() { if (a) hipFree(a); if (b) hipFree(b); }
+  } free_words{mv, mtop};
+  std::vector<int4> hr((size_t)n * P);
+  int b0 = 0;
+  while (b0 < n) {  // batches whose step words fit the budget
+    std::vector<PJ<T>> pj;
+    std::vector<TbJob> tb;
+    long long mvw = 0, tw = 0;
+    int b1 = b0, bmaxlx = 0;
+    for (; b1 < n; b1++) {
+      const int lx = (int)lens[b1];
+      const long long need = (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(P, 0, 1);
+      if (b1 > b0 && mvw + need > budget_words) break;
+      long long mo, to;
+      plan_job(P, lx, 0, 1, mvw, tw, mo, to);
+      pj.push_back(PJ<T>{data, data + ho[b1], P, lx, mo, to});
+      TbJob t;
+      t.ly = P; t.lx = lx; t.flip = 1; t.seq = b1 - b0; t.mv_off = mo; t.top_off = to; t.path_off = -1;
+      t.runs_off = (long long)(b1 - b0) * P;
+      t.rp = nullptr;
+      tb.push_back(t);
+      bmaxlx = std::max(bmaxlx, lx);
+    }
+    const int nj = b1 - b0;
+    if ((size_t)mvw > mv_cap || (size_t)tw > mtop_cap) {  // grow-only step-word buffers
+      if (mv) hipFree(mv);
+      if (mtop) hipFree(mtop);
+      mv = nullptr;
+      mtop = nullptr;
+      mv_cap = std::max<size_t>(mv_cap, (size_t)std::max(1LL, mvw));
+      mtop_cap = std::max<size_t>(mtop_cap, (size_t)std::max(1LL, tw));
+      if (hipMalloc((void**)&mv, mv_cap * 4) != hipSuccess || hipMalloc((void**)&mtop, mtop_cap * 4) != hipSuccess)
+        return fail(c, ODBA_E_NOMEM, "cannot allocate %zu step words", mv_cap);
+    }
+    HIPC(c, hipMemcpyAsync(jobs, pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(tbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
+    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P);
+    if (rc) return rc;
+    HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, P, 0, 1, derr, nullptr));
+    int herr = 0;
+    HIPC(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(&hr[(size_t)b0 * P], runs, (size_t)nj * P * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+    if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one prefix row");
+    b0 = b1;
+  }
+  for (int b = 0; b < n; b++) {
+    const int4* r = &hr[(size_t)b * P];
+    for (int i = 0; i < P; i++)
+      hist[(size_t)b * P + i] = (r[i].y >= r[i].x ? r[i].y - r[i].x + 1 : 0) + (r[i].w >= r[i].z ? r[i].w - r[i].z + 1 : 0);
+    if (r[P - 1].y < r[P - 1].x) return fail(c, ODBA_E_TRACEBACK, "sequence %d: no counted cell in the prefix's last row", b);
+    chop[b] = (uint64_t)r[P - 1].y;
+  }
+  return ODBA_OK;
+}
+
 }  // namespace
 
 #define ODBA_DISPATCH(dtype, CALL)                        \
@@ -1294,6 +1413,14 @@ int odba_znorm(odba_ctx* c, int dtype, void* const* seqs, const uint64_t* lens, 
   if (!c || !seqs || !lens || n <= 0) return fail(c, ODBA_E_INVALID, "invalid z-norm arguments");
   if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
   ODBA_DISPATCH(dtype, znorm_t<T>(c, (T* const*)seqs, lens, n, mu, sigma))
+}
+
+int odba_prefix_chop(odba_ctx* c, int dtype, const void* prefix, uint64_t prefix_len, void* const* seqs,
+                     const uint64_t* lens, int n, int norm, uint64_t* chop_pos, int32_t* hist) {
+  if (!c || !prefix || !seqs || !lens || !chop_pos || !hist) return fail(c, ODBA_E_INVALID, "invalid prefix-chop arguments");
+  if (prefix_len > (uint64_t)INT32_MAX / 2) return fail(c, ODBA_E_INVALID, "prefix too long");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(dtype, prefix_chop_t<T>(c, (const T*)prefix, (int)prefix_len, (T* const*)seqs, lens, n, norm, chop_pos, hist))
 }
 
 int odba_perform_dba(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int os, int oe,

@@ -74,14 +74,16 @@ struct StripeState {
 // by workgroups in pair order. tab: one {int r, c; long long band0} per pair
 // in claim order (band0 = first claim index of the pair), nclaims bands in
 // all. rings: nblocks*waves*ring elements (ring a power of two >= 128);
-// gslots: R slots of gstride elements (>= the longest Y + 256); gsync: 3*R
-// ints; abr: ntab ints; err: bit 8 set when a hand-off wait timed out after
-// tick_limit wall-clock ticks. flags (optional) as for launch_all_pairs.
+// gslots: R >= 2*nblocks+2 slots of gstride elements (>= the longest Y +
+// 256); sync: band_sync_bytes(R, nclaims, ntab) bytes; err: bit 8 set when a
+// hand-off wait timed out after tick_limit wall-clock ticks. flags
+// (optional) as for launch_all_pairs.
 hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
                                   const void* data, const long long* offs, const int* lens, int n, const void* tab,
                                   int ntab, long long nclaims, unsigned long long* counter, void* dist, void* rings,
-                                  int ring, void* gslots, long long gstride, int* gsync, int R, int* abr, int* err,
+                                  int ring, void* gslots, long long gstride, void* sync, int R, int* err,
                                   long long tick_limit, int* flags);
+size_t band_sync_bytes(int R, long long nclaims, int ntab);
 size_t band_pair_size();
 
 // Float path jobs with a workgroup of `waves` waves per job (strips = swaths).

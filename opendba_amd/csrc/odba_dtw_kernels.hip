@@ -241,7 +241,7 @@ template <typename T> struct SwathIO {
 // waits until its left neighbour has published the edge rows it is about to
 // read, and publishes its own edge rows as they are flushed.
 struct NoPipe {
-  static constexpr bool kPrefetch = true;  // load the next block's edge values one block early
+  __device__ bool ready(int) const { return true; }  // input rows below the argument are published
   __device__ void wait(int) const {}
   __device__ void publish(int) const {}
   __device__ void consumed(int) const {}  // input rows below the argument have been read
@@ -297,14 +297,21 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     return a.ein[r & a.imask];  // edge buffers are padded by 128 entries (rings wrap)
   };
   auto rows_of_block = [&](int b) -> int { return min(r0 + 64 * b + 64, r1 + 1); };
+  // The next block's edge values are loaded one block early whenever the
+  // left neighbour has already published them (always, without a pipe);
+  // otherwise the wave waits for them when it stages that block.
   T ynext = nblk > 0 ? load_y(0) : T(0);
   T enext = T(0);
-  if (Pipe::kPrefetch && nblk > 0) enext = load_e(0);
+  bool e_ready = false;
+  if (nblk > 0 && (first_swath || pipe.ready(rows_of_block(0)))) {
+    enext = load_e(0);
+    e_ready = true;
+  }
 
   for (int b = 0; b < nblk; b++) {
     // --- stage the window for block b
-    if (!Pipe::kPrefetch) {
-      if (!first_swath) pipe.wait(rows_of_block(b));
+    if (!e_ready) {
+      pipe.wait(rows_of_block(b));
       enext = load_e(b);
     }
     T yshift = W->y[64 + lane];
@@ -313,10 +320,14 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     W->y[64 + lane] = ynext;
     W->e[lane] = enext;
     wave_sync();
-    if (!Pipe::kPrefetch && !first_swath) pipe.consumed(rows_of_block(b));
+    if (!first_swath) pipe.consumed(rows_of_block(b));
+    e_ready = false;
     if (b + 1 < nblk) {
       ynext = load_y(b + 1);
-      if (Pipe::kPrefetch) enext = load_e(b + 1);
+      if (first_swath || pipe.ready(rows_of_block(b + 1))) {
+        enext = load_e(b + 1);
+        e_ready = true;
+      }
     }
 
     const int sbase = 64 * b;
@@ -671,12 +682,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesP
 // stops at.
 // ---------------------------------------------------------------------------
 struct StripPipe {
-  static constexpr bool kPrefetch = false;
   int* in_owner;  // LDS: strip whose edge is in the input slot
   int* in_cnt;    // LDS: edge rows published into the input slot
   int in_strip;   // the left neighbour's strip index
   int* out_cnt;   // LDS: rows published of this strip's edge (null: none)
   int lane;
+  __device__ bool ready(int rows) const {
+    const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (o != in_strip || c < rows) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+  }
   __device__ void wait(int rows) const {
     for (;;) {
       const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
@@ -707,17 +724,21 @@ constexpr int kMaxStripWaves = 16;
 //     a ring of `ring` rows in the workgroup's scratch: the producer waits
 //     before overwriting rows its consumer has not read (LDS counters).
 //   * Between bands, the band's last strip writes the whole edge column to
-//     a global slot (q mod R for claim q) and publishes its progress in
-//     global words, 256 rows at a time; the next band's first strip (the
-//     next claim) reads it. Slot q mod R is reused by claim q+R only after
-//     claim q+1 has read it, or at once when claim q hands nothing over.
+//     a global slot taken from a pool of free slots (a bitmap), publishes
+//     the slot in a per-claim word and its progress in a per-slot word, 256
+//     rows at a time; the next band's first strip (the next claim) reads it
+//     and returns the slot to the pool when done. A slot is held from its
+//     producer's start until its consumer ends, so at most twice the number
+//     of running claims plus one are held at once: a pool of
+//     2 * workgroups + 2 slots is never empty.
 //
 // No deadlock: every wait targets a lower claim or a lower strip of the
 // same claim, except the ring wait, which targets the next strip of the
 // same band; the band's last strip never waits on a ring (its output slot
 // is a whole column), and a claim is only taken by a running workgroup.
 // Every wait also ends after `tick_limit` wall-clock ticks, setting bit 8
-// of *err, so a broken hand-off cannot hang the device.
+// of *err, after which every other wait returns at once: a broken hand-off
+// cannot hang the device.
 //
 // Distances are written by the pair's last strip. The open-end early stop
 // is reconstructed after the fact as for every strip schedule: the per-pair
@@ -744,10 +765,7 @@ __device__ __forceinline__ bool band_failed(int* err) {
 }
 
 struct BandPipe {
-  static constexpr bool kPrefetch = false;
   const int* in_cnt;    // rows published into the input (LDS, or a global word)
-  const int* in_owner;  // global input: claim index that owns the slot
-  int in_tag;           // the left band's claim index
   int in_global;
   int* in_used;         // ring input: LDS word of rows read so far
   int* out_cnt;         // rows published of the output (null: none)
@@ -758,36 +776,27 @@ struct BandPipe {
   int lane;
   int* err;
   long long limit;
-  mutable int owned = 0;
   mutable int last_pub = 0;
 
+  __device__ bool ready(int rows) const {
+    if (in_global) {
+      const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (c < rows) return false;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    } else {
+      const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (c < rows) return false;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    return true;
+  }
   __device__ void wait(int rows) const {
     if (band_failed(err)) return;
     const long long t0 = wall_clock64();
-    if (in_global) {
-      while (!owned) {
-        const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-        if (o == in_tag) owned = 1;
-        else {
-          if (band_timed_out(t0, limit, err, lane)) return;
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-      for (;;) {
-        const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (c >= rows) break;
-        if (band_timed_out(t0, limit, err, lane)) return;
-        __builtin_amdgcn_s_sleep(2);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    } else {
-      for (;;) {
-        const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-        if (c >= rows) break;
-        if (band_timed_out(t0, limit, err, lane)) return;
-        __builtin_amdgcn_s_sleep(1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    while (!ready(rows)) {
+      if (band_timed_out(t0, limit, err, lane)) return;
+      if (in_global) __builtin_amdgcn_s_sleep(2);
+      else __builtin_amdgcn_s_sleep(1);
     }
   }
   __device__ void publish(int rows) const {
@@ -822,14 +831,43 @@ struct BandPipe {
   }
 };
 
+// Takes a free global slot (lane 0 works the bitmap; the pool is sized so
+// that one is always free).
+__device__ int band_pop_slot(unsigned long long* bits, int nwords, int* err, long long limit, int lane) {
+  int got = -1;
+  if (lane == 0) {
+    const long long t0 = wall_clock64();
+    while (got < 0) {
+      for (int w = 0; w < nwords && got < 0; w++) {
+        unsigned long long v = __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (v && got < 0) {
+          const int b = __builtin_ctzll(v);
+          const unsigned long long m = 1ull << b;
+          const unsigned long long old = atomicAnd(&bits[w], ~m);
+          if (old & m) got = w * 64 + b;
+          else v = old & ~m;
+        }
+      }
+      if (got < 0 && wall_clock64() - t0 > limit) {
+        atomicOr(err, 8);
+        got = 0;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // after the last reader's release
+  return uniform_i32(got);
+}
+
 template <typename T, int C, bool OS, bool OE>
 __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ data, const long long* __restrict__ offs,
                                                           const int* __restrict__ lens, int n,
                                                           const BandPair* __restrict__ tab, int ntab, long long nclaims,
                                                           unsigned long long* __restrict__ counter, T* __restrict__ dist,
                                                           T* __restrict__ rings, int ring, T* __restrict__ gslots,
-                                                          long long gstride, int* __restrict__ gsync, int R,
-                                                          int* __restrict__ abr, int* __restrict__ err, long long limit) {
+                                                          long long gstride, int* __restrict__ g_cnt,
+                                                          unsigned long long* __restrict__ g_free, int R,
+                                                          int* __restrict__ slot_of, int* __restrict__ abr,
+                                                          int* __restrict__ err, long long limit) {
   __shared__ T s_y[kMaxStripWaves][128];
   __shared__ T s_e[kMaxStripWaves][64];
   __shared__ T s_o[kMaxStripWaves][128];
@@ -841,9 +879,6 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
   const int wid = uniform_i32((int)(threadIdx.x >> 6));
   const int nw = uniform_i32((int)(blockDim.x >> 6));
   T* rbase = rings + (long long)blockIdx.x * nw * ring;
-  int* g_owner = gsync;
-  int* g_cnt = gsync + R;
-  int* g_rel = gsync + 2 * R;
   const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
   for (;;) {
     __syncthreads();  // every wave is done with the previous claim
@@ -874,29 +909,26 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
     const bool has_out = s < S - 1;
     const bool in_global = (wid == 0) && s > 0;
     const bool out_global = (s == slast) && has_out;
-    const int gin = (int)((q + R - 1) % R), gout = (int)(q % R);
-    if (s == slast && !band_failed(err)) {  // this claim's turn on global slot q mod R
+    int gout = 0, gin = 0;
+    if (out_global) {  // take a slot, reset its progress word, then announce it
+      gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
+      if (lane == 0) {
+        __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slot_of[q], gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (in_global && !band_failed(err)) {  // the left band's slot
       const long long t0 = wall_clock64();
       for (;;) {
-        const int rel = uniform_i32(__hip_atomic_load(&g_rel[gout], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-        if ((long long)rel >= q - R) break;
-        if (band_timed_out(t0, limit, err, lane)) break;
+        gin = uniform_i32(__hip_atomic_load(&slot_of[q - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (gin >= 0) break;
+        if (band_timed_out(t0, limit, err, lane)) { gin = 0; break; }
         __builtin_amdgcn_s_sleep(2);
-      }
-      if (lane == 0) {
-        if (out_global) {
-          __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(&g_owner[gout], (int)q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-          __hip_atomic_store(&g_rel[gout], (int)q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
       }
     }
     BandPipe pp;
     pp.in_global = in_global;
     pp.in_cnt = in_global ? &g_cnt[gin] : (s > 0 ? &s_cnt[wid - 1] : nullptr);
-    pp.in_owner = &g_owner[gin];
-    pp.in_tag = (int)(q - 1);
     pp.in_used = (s > 0 && !in_global) ? &s_used[wid - 1] : nullptr;
     pp.out_cnt = has_out ? (out_global ? &g_cnt[gout] : &s_cnt[wid]) : nullptr;
     pp.out_global = out_global;
@@ -915,8 +947,10 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
     io.mv = nullptr; io.mtop = nullptr;
     io.ref_left = (off % ODBA_W) == 0;
     SwathRes<T> res = run_swath<T, C, OS, OE, true, false, BandPipe>(io, &win, lane, pp);
-    if (in_global && lane == 0)  // the left band's slot is free for claim q-1+R
-      __hip_atomic_store(&g_rel[gin], (int)(q - 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (in_global && lane == 0) {  // every read of the slot has returned: back to the pool
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      atomicOr(&g_free[gin >> 6], 1ull << (gin & 63));
+    }
     if (OE && !OS && has_out) {
       const int next = off + SW;
       if ((next % ODBA_W) == 0 && next > ly && res.top == res.colmin && lane == 0) atomicMin(&abr[t], next / ODBA_W);
@@ -928,13 +962,15 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
   }
 }
 
-__global__ void k_band_init(int* __restrict__ gsync, int R, int* __restrict__ abr, int ntab) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < R) {
-    gsync[i] = -1;          // owner
-    gsync[R + i] = 0;       // rows published
-    gsync[2 * R + i] = i - R;  // last claim done with the slot (a virtual one before the first use)
+__global__ void k_band_init(int* __restrict__ g_cnt, unsigned long long* __restrict__ g_free, int R,
+                            int* __restrict__ slot_of, long long nclaims, int* __restrict__ abr, int ntab) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < R) g_cnt[i] = 0;
+  if (i < (R + 63) / 64) {
+    const int rem = R - (int)i * 64;
+    g_free[i] = rem >= 64 ? ~0ull : ((1ull << rem) - 1);
   }
+  if (i < nclaims) slot_of[i] = -1;
   if (i < ntab) abr[i] = INT_MAX;
 }
 
@@ -1227,8 +1263,9 @@ template <typename T, int C>
 static void launch_bands_c(int os, int oe, dim3 g, dim3 b, hipStream_t st, const T* data, const long long* offs,
                            const int* lens, int n, const BandPair* tab, int ntab, long long nclaims,
                            unsigned long long* counter, T* dist, T* rings, int ring, T* gslots, long long gstride,
-                           int* gsync, int R, int* abr, int* err, long long limit) {
-#define ODBA_BANDS(A, B) hipLaunchKernelGGL((k_all_pairs_bands<T, C, A, B>), g, b, 0, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit)
+                           int* g_cnt, unsigned long long* g_free, int R, int* slot_of, int* abr, int* err,
+                           long long limit) {
+#define ODBA_BANDS(A, B) hipLaunchKernelGGL((k_all_pairs_bands<T, C, A, B>), g, b, 0, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, limit)
   if (os && oe) ODBA_BANDS(true, true); else if (os) ODBA_BANDS(true, false); else if (oe) ODBA_BANDS(false, true); else ODBA_BANDS(false, false);
 #undef ODBA_BANDS
 }
@@ -1237,30 +1274,42 @@ template <typename T>
 static hipError_t launch_bands_t(int C, int os, int oe, dim3 g, dim3 b, hipStream_t st, const T* data,
                                  const long long* offs, const int* lens, int n, const BandPair* tab, int ntab,
                                  long long nclaims, unsigned long long* counter, T* dist, T* rings, int ring, T* gslots,
-                                 long long gstride, int* gsync, int R, int* abr, int* err, long long limit) {
+                                 long long gstride, int* g_cnt, unsigned long long* g_free, int R, int* slot_of,
+                                 int* abr, int* err, long long limit) {
   switch (C) {
-    case 1: launch_bands_c<T, 1>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
-    case 4: launch_bands_c<T, 4>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
-    case 16: launch_bands_c<T, 16>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, gsync, R, abr, err, limit); break;
+    case 1: launch_bands_c<T, 1>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, limit); break;
+    case 4: launch_bands_c<T, 4>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, limit); break;
+    case 16: launch_bands_c<T, 16>(os, oe, g, b, st, data, offs, lens, n, tab, ntab, nclaims, counter, dist, rings, ring, gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, limit); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+size_t band_sync_bytes(int R, long long nclaims, int ntab) {
+  return (size_t)R * 4 + (size_t)((R + 63) / 64) * 8 + (size_t)nclaims * 4 + (size_t)ntab * 4 + 64;
+}
+
 hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, int nblocks, hipStream_t st,
                                   const void* data, const long long* offs, const int* lens, int n, const void* tab,
                                   int ntab, long long nclaims, unsigned long long* counter, void* dist, void* rings,
-                                  int ring, void* gslots, long long gstride, int* gsync, int R, int* abr, int* err,
+                                  int ring, void* gslots, long long gstride, void* sync, int R, int* err,
                                   long long tick_limit, int* flags) {
-  if (waves < 1 || waves > kMaxStripWaves || ring < 128 || (ring & (ring - 1)) || R < 2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_band_init, dim3((std::max(R, ntab) + 255) / 256), dim3(256), 0, st, gsync, R, abr, ntab);
+  if (waves < 1 || waves > kMaxStripWaves || ring < 128 || (ring & (ring - 1)) || R < 2 * nblocks + 2) return hipErrorInvalidValue;
+  // sync: g_free (u64 words), g_cnt[R], slot_of[nclaims], abr[ntab]
+  unsigned long long* g_free = (unsigned long long*)sync;
+  int* g_cnt = (int*)(g_free + (R + 63) / 64);
+  int* slot_of = g_cnt + R;
+  int* abr = slot_of + nclaims;
+  const long long ninit = std::max<long long>(std::max<long long>(R, nclaims), ntab);
+  hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
+                     nclaims, abr, ntab);
   dim3 g(nblocks), b(64 * waves);
   const BandPair* T_ = (const BandPair*)tab;
   hipError_t e = hipErrorInvalidValue;
   switch (dtype) {
-    case ODBA_F32: e = launch_bands_t<float>(C, os, oe, g, b, st, (const float*)data, offs, lens, n, T_, ntab, nclaims, counter, (float*)dist, (float*)rings, ring, (float*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
-    case ODBA_I32: e = launch_bands_t<int32_t>(C, os, oe, g, b, st, (const int32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (int32_t*)dist, (int32_t*)rings, ring, (int32_t*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
-    case ODBA_U32: e = launch_bands_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (uint32_t*)dist, (uint32_t*)rings, ring, (uint32_t*)gslots, gstride, gsync, R, abr, err, tick_limit); break;
+    case ODBA_F32: e = launch_bands_t<float>(C, os, oe, g, b, st, (const float*)data, offs, lens, n, T_, ntab, nclaims, counter, (float*)dist, (float*)rings, ring, (float*)gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit); break;
+    case ODBA_I32: e = launch_bands_t<int32_t>(C, os, oe, g, b, st, (const int32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (int32_t*)dist, (int32_t*)rings, ring, (int32_t*)gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit); break;
+    case ODBA_U32: e = launch_bands_t<uint32_t>(C, os, oe, g, b, st, (const uint32_t*)data, offs, lens, n, T_, ntab, nclaims, counter, (uint32_t*)dist, (uint32_t*)rings, ring, (uint32_t*)gslots, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit); break;
   }
   if (e != hipSuccess) return e;
   if (flags) hipLaunchKernelGGL(k_band_flags, dim3((ntab + 255) / 256), dim3(256), 0, st, T_, ntab, n, abr, flags);

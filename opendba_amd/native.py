@@ -25,7 +25,7 @@ EXPORTS = [
     "odba_dba_record_paths", "odba_dba_get_path", "odba_ctx_set_mem_cap",
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
-    "odba_cutree_k",
+    "odba_cutree_k", "odba_prefix_chop",
 ]
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
@@ -71,6 +71,7 @@ def load():
     L.odba_dba_record_paths.argtypes = [vp, i32]
     L.odba_dba_get_path.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
     L.odba_ctx_set_mem_cap.argtypes = [vp, u64]
+    L.odba_prefix_chop.argtypes = [vp, i32, vp, u64, vp, vp, i32, i32, vp, vp]
     L.odba_server_create.argtypes = [i32, u64, i32, C.POINTER(vp)]
     L.odba_server_destroy.argtypes = [vp]
     L.odba_server_submit.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, C.POINTER(u64)]
@@ -168,6 +169,23 @@ class Context:
         self._check(self.L.odba_znorm(self.h, CODE[seqs[0].dtype], _ptrs(seqs), _p(lens), len(seqs), _p(mu),
                                       _p(sg)))
         return seqs, mu, sg
+
+    def set_mem_cap(self, nbytes):
+        """Device memory the context plans its batches within (odba_ctx_set_mem_cap)."""
+        self._check(self.L.odba_ctx_set_mem_cap(self.h, int(nbytes)))
+
+    def prefix_chop(self, prefix, seqs, norm=0):
+        """Prefix location (odba_prefix_chop): returns (chop positions,
+        histograms [n, len(prefix)], the sequences as the library left them
+        (z-normalised copies when norm))."""
+        prefix = np.ascontiguousarray(prefix)
+        seqs = [np.array(s, copy=True) for s in seqs]
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        chop = np.zeros(len(seqs), dtype=np.uint64)
+        hist = np.zeros((len(seqs), len(prefix)), dtype=np.int32)
+        self._check(self.L.odba_prefix_chop(self.h, CODE[prefix.dtype], _p(prefix), len(prefix), _ptrs(seqs), _p(lens),
+                                            len(seqs), norm, _p(chop), _p(hist)))
+        return chop, hist, seqs
 
     def dba_clusters(self, seqs, memberships, medoids, open_start=0, open_end=0, max_rounds=250, mu=None,
                      sigma=None):

@@ -33,6 +33,7 @@ def lib():
         L.orc_znorm.argtypes = [i32, vp, vp, u64, vp, vp]
         L.orc_medoid_single.argtypes = [i32, vp, vp, u64, vp]
         L.orc_perform_dba.argtypes = [i32, vp, vp, u64, i32, i32, i32, i32, u64, i32, u64, vp, vp, vp, vp, vp, vp]
+        L.orc_prefix_chop.argtypes = [i32, vp, u64, vp, u64, vp, vp]
         _lib = L
     return _lib
 
@@ -81,6 +82,19 @@ def all_pairs(seqs, os_=0, oe=0, w=1024, nthreads=8, subset=None):
     rc = lib().orc_all_pairs(dt, _ptrs(seqs), _p(lens), n, os_, oe, w, _p(out), nthreads, sub, nsub)
     assert rc == 0, rc
     return out
+
+
+def prefix_chop(prefix, seq):
+    """chopPrefixFromSequences for one sequence (inputs already normalised):
+    returns (chop position, per-prefix-row path-cell histogram)."""
+    prefix = np.ascontiguousarray(prefix)
+    seq = np.ascontiguousarray(seq)
+    assert prefix.dtype == seq.dtype
+    chop = C.c_uint64(0)
+    hist = np.zeros(len(prefix), dtype=np.int32)
+    rc = lib().orc_prefix_chop(dtype_code(seq), _p(prefix), len(prefix), _p(seq), len(seq), C.byref(chop), _p(hist))
+    assert rc == 0, rc
+    return chop.value, hist
 
 
 def traceback_full(moves, flip, seq, lc):

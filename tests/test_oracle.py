@@ -243,3 +243,44 @@ def test_kseg_events_are_segment_medians_of_input_values():
         # chunk-border average can add quarter steps
         assert np.all(np.abs(e * 4 - np.round(e * 4)) == 0)
         assert e.min() >= r.min() and e.max() <= r.max()
+
+
+# chopPrefixFromSequences (dba.hpp:1192-1395)
+
+def test_prefix_chop_exact_prefix():
+    """The sequence starts with the prefix itself, then far-away values: the
+    path is the diagonal and the top row is open-right after it."""
+    rng = np.random.default_rng(31)
+    y = rng.permutation(50).astype(np.float32)          # distinct values
+    tail = (100.0 + rng.permutation(300)).astype(np.float32)
+    chop, hist = O.prefix_chop(y, np.concatenate([y, tail]))
+    assert chop == len(y) - 1
+    assert np.array_equal(hist, np.ones(len(y), dtype=np.int32))
+
+
+def test_prefix_chop_doubled_values_hand_derived():
+    """prefix [0, 10, 20], sequence [0, 0, 10, 10, 20, 20, 100, 100, 100]:
+    walking back, (2,8)..(2,5) are OPEN_RIGHT, (2,4) is DIAG (chop = 4), then
+    (1,3) RIGHT, (1,2) DIAG, (0,1) RIGHT, (0,0) NIL."""
+    y = np.array([0, 10, 20], dtype=np.float32)
+    x = np.array([0, 0, 10, 10, 20, 20, 100, 100, 100], dtype=np.float32)
+    chop, hist = O.prefix_chop(y, x)
+    assert chop == 4
+    assert hist.tolist() == [2, 2, 1]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_prefix_chop_histogram_is_the_dba_count(seed):
+    """The histogram is the per-row count of the DBA traceback with the
+    prefix on Y (every cell but the top row's open-right run, plus (0,0)),
+    and the chop position is the first counted column of the top row."""
+    rng = np.random.default_rng(seed)
+    y = np.cumsum(rng.normal(size=40)).astype(np.float32)
+    x = np.cumsum(rng.normal(size=int(rng.integers(60, 2600)))).astype(np.float32)
+    chop, hist = O.prefix_chop(y, x)
+    _, M, _, _ = O.dtw_pair(y, x, os_=0, oe=1, moves=True)
+    rc, _, counts, _ = O.traceback_full(M, 1, x, len(y))
+    assert rc == 0
+    assert np.array_equal(hist, counts.astype(np.int32))
+    top = M[len(y) - 1]
+    assert all(top[j] == 4 for j in range(chop + 1, len(x))) and top[chop] != 4
