@@ -70,6 +70,26 @@ int odba_dtw_all_pairs_resident(odba_ctx* ctx, odba_seqset* set, int use_open_st
                                 void* dist_out, int* flags_out, double* kernel_ms);
 
 /*
+ * All-vs-all autotuner (no counterpart in the reference). Times every
+ * candidate schedule on the resident set in this mode: one wave per pair in
+ * both packing orders, and, when the set is eligible (float, or integers
+ * whose costs cannot wrap; every sequence >= 64 samples) and has fewer than
+ * 16 pairs per compute unit, bands of every strip width and band width that
+ * the longest sequence admits. The fastest (median of `reps` kernel times)
+ * is remembered in the context for sets of the same shape (value type,
+ * mode, count, lengths) and used by later all-vs-all calls on them; the
+ * results are the same under every schedule. best / best_ms / tried
+ * (optional) receive the choice, its time and the number of candidates.
+ */
+int odba_tune_all_pairs(odba_ctx* ctx, odba_seqset* set, int use_open_start, int use_open_end, int reps,
+                        odba_schedule* best, double* best_ms, int* tried);
+/* The schedule the next all-vs-all on this set and mode will use. */
+int odba_get_schedule(odba_ctx* ctx, odba_seqset* set, int use_open_start, int use_open_end, odba_schedule* out);
+/* Pins a schedule for sets of this shape and mode (null: forget it). */
+int odba_set_schedule(odba_ctx* ctx, odba_seqset* set, int use_open_start, int use_open_end,
+                      const odba_schedule* sched);
+
+/*
  * One pair with its step matrix: moves_out (optional, ly*lx bytes,
  * row-major) receives the reference step codes of every cell
  * (dtw.hpp:10-16; 0 where the reference leaves a cell unwritten after an

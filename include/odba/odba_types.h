@@ -38,6 +38,20 @@
 #define ODBA_PATH_FULL 1   /* always the full move matrix                        */
 #define ODBA_PATH_STRIPE 2 /* always stripe mode (store swath edges, recompute)  */
 
+/* All-vs-all schedules (no counterpart in the reference, which launches one
+ * threadblock per pair and swath). */
+#define ODBA_SCHED_WAVE 0  /* one wave per pair, persistent grid               */
+#define ODBA_SCHED_BANDS 1 /* bands of column strips, a workgroup per band     */
+
+typedef struct odba_schedule {
+  int kind;       /* ODBA_SCHED_WAVE or ODBA_SCHED_BANDS                               */
+  int order;      /* wave: pair packing order, 0 = second index descending then first   */
+                  /* index descending, 1 = largest cell count first                     */
+  int band_c;     /* bands: columns per lane, 1, 4 or 16 (strips of 64 * band_c columns) */
+  int band_waves; /* bands: strips per band = waves per workgroup, 1..16                 */
+  int ring;       /* bands: rows of the in-workgroup edge ring (power of two >= 128)     */
+} odba_schedule;
+
 /* Return codes. 0 is success; 18..46 are the reference's exit codes
  * (exit_codes.hpp:4-32); values from 100 up are this library's own. */
 #define ODBA_OK 0

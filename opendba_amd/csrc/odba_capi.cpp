@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -32,6 +33,7 @@ struct odba_ctx {
   size_t esc_bytes = 0;
   void* small = nullptr;  // counters, flags (256 bytes)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::map<std::string, odba_schedule> tuned;  // all-vs-all schedules by workload shape
 };
 
 struct odba_seqset {
@@ -45,6 +47,7 @@ struct odba_seqset {
   std::vector<int> h_lens;
   int maxlen = 0;
   bool int_no_wrap = true;     // integer costs provably cannot wrap
+  void* order = nullptr;       // device pair order (int2 per pair), largest cell count first
 };
 
 namespace {
@@ -225,6 +228,7 @@ int odba_seqset_create(odba_ctx* c, int dtype, const void* const* seqs, const ui
 
 int odba_seqset_destroy(odba_seqset* s) {
   if (!s) return ODBA_OK;
+  if (s->order) hipFree(s->order);
   if (s->data) hipFree(s->data);
   if (s->offs) hipFree(s->offs);
   if (s->lens) hipFree(s->lens);
@@ -238,11 +242,12 @@ namespace {
 
 // Pair schedule of the all-vs-all: one wave per pair (many pairs), or bands
 // of column strips swept by workgroups (too few pairs to give every SIMD
-// work). ODBA_PAIR_SCHEDULE=wave|strip overrides the choice;
-// ODBA_BAND_WAVES and ODBA_BAND_RING set the band width in strips and the
-// in-workgroup edge ring (rows).
+// work). A schedule the autotuner chose (or the caller pinned) for sets of
+// this shape wins over the built-in rule. ODBA_PAIR_SCHEDULE=wave|strip,
+// ODBA_BAND_WAVES and ODBA_BAND_RING override both (tests).
 struct PairSchedule {
   bool strips = false;
+  int order = 0;
   int C = 16, waves = 4, nblocks = 1, ring = 2048;
 };
 
@@ -251,29 +256,99 @@ int env_int(const char* name, int dflt) {
   return (v && *v) ? std::atoi(v) : dflt;
 }
 
-PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, long long npairs) {
-  PairSchedule ps;
+int set_minlen(const odba_seqset* s) {
   int minlen = s->h_lens.empty() ? 0 : s->h_lens[0];
   for (int L : s->h_lens) minlen = std::min(minlen, L);
-  const bool eligible = (s->dtype == ODBA_F32 || ((s->dtype == ODBA_I32 || s->dtype == ODBA_U32) && s->int_no_wrap)) &&
-                        minlen >= 64;
-  const char* env = std::getenv("ODBA_PAIR_SCHEDULE");
-  // Fewer pairs than one wave per SIMD, and long enough that the strip
-  // pipeline's fill (about 128 rows per strip) is small against |Y|.
-  bool want = npairs < 4LL * c->cus && minlen >= 4096;
-  if (env && !std::strcmp(env, "wave")) want = false;
-  if (env && !std::strcmp(env, "strip")) want = true;
-  ps.strips = want && eligible;
-  if (!ps.strips) {
-    ps.nblocks = grid_blocks(c, npairs);
-    return ps;
-  }
-  ps.C = s->maxlen <= 16 * 64 ? 1 : (s->maxlen <= 16 * 256 ? 4 : 16);
-  const long long smax = (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C);
-  ps.waves = (int)std::min<long long>(16, std::max<long long>(1, smax));
-  ps.waves = std::max(1, std::min(16, env_int("ODBA_BAND_WAVES", ps.waves)));
-  ps.ring = env_int("ODBA_BAND_RING", 2048);
+  return minlen;
+}
+
+bool bands_eligible(const odba_seqset* s) {
+  return (s->dtype == ODBA_F32 || ((s->dtype == ODBA_I32 || s->dtype == ODBA_U32) && s->int_no_wrap)) &&
+         set_minlen(s) >= 64;
+}
+
+std::string shape_key(const odba_seqset* s, int os, int oe) {
+  long long sum = 0;
+  for (int L : s->h_lens) sum += L;
+  char b[160];
+  std::snprintf(b, sizeof(b), "%d/%d%d/%d/%d/%d/%lld", s->dtype, os, oe, s->n, s->maxlen, set_minlen(s), sum);
+  return b;
+}
+
+PairSchedule from_public(odba_ctx* c, const odba_schedule& u, long long npairs) {
+  PairSchedule ps;
+  ps.strips = u.kind == ODBA_SCHED_BANDS;
+  ps.order = u.order;
+  ps.C = u.band_c;
+  ps.waves = u.band_waves;
+  ps.ring = u.ring;
+  if (!ps.strips) ps.nblocks = grid_blocks(c, npairs);
   return ps;
+}
+
+odba_schedule to_public(const PairSchedule& ps) {
+  odba_schedule u;
+  u.kind = ps.strips ? ODBA_SCHED_BANDS : ODBA_SCHED_WAVE;
+  u.order = ps.order;
+  u.band_c = ps.C;
+  u.band_waves = ps.waves;
+  u.ring = ps.ring;
+  return u;
+}
+
+PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, long long npairs) {
+  PairSchedule ps;
+  const bool eligible = bands_eligible(s);
+  auto t = c->tuned.find(shape_key(s, os, oe));
+  if (t != c->tuned.end() && (t->second.kind == ODBA_SCHED_WAVE || eligible)) {
+    ps = from_public(c, t->second, npairs);
+  } else {
+    // Fewer pairs than one wave per SIMD, and long enough that the strip
+    // pipeline's fill (about 128 rows per strip) is small against |Y|.
+    ps.strips = eligible && npairs < 4LL * c->cus && set_minlen(s) >= 4096;
+    if (ps.strips) {
+      ps.C = s->maxlen <= 16 * 64 ? 1 : (s->maxlen <= 16 * 256 ? 4 : 16);
+      const long long smax = (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C);
+      ps.waves = (int)std::min<long long>(16, std::max<long long>(1, smax));
+    } else {
+      ps.nblocks = grid_blocks(c, npairs);
+    }
+  }
+  const char* env = std::getenv("ODBA_PAIR_SCHEDULE");
+  if (env && !std::strcmp(env, "wave") && ps.strips) {
+    ps = PairSchedule();
+    ps.nblocks = grid_blocks(c, npairs);
+  }
+  if (env && !std::strcmp(env, "strip") && !ps.strips && eligible) {
+    ps.strips = true;
+    ps.C = s->maxlen <= 16 * 64 ? 1 : (s->maxlen <= 16 * 256 ? 4 : 16);
+    const long long smax = (s->maxlen + 64LL * ps.C - 1) / (64LL * ps.C);
+    ps.waves = (int)std::min<long long>(16, std::max<long long>(1, smax));
+  }
+  if (ps.strips) {
+    ps.waves = std::max(1, std::min(16, env_int("ODBA_BAND_WAVES", ps.waves)));
+    ps.ring = env_int("ODBA_BAND_RING", ps.ring);
+  }
+  return ps;
+}
+
+// Pair order with the largest cell count first (ties: the built-in order),
+// built once per set.
+int ensure_pair_order(odba_ctx* c, odba_seqset* s, long long npairs) {
+  if (s->order) return ODBA_OK;
+  std::vector<int2> ord;
+  ord.reserve((size_t)npairs);
+  for (int cc = s->n - 1; cc >= 1; cc--)
+    for (int r = cc - 1; r >= 0; r--) ord.push_back(make_int2(r, cc));
+  std::stable_sort(ord.begin(), ord.end(), [&](const int2& a, const int2& b) {
+    return (long long)s->h_lens[a.x] * s->h_lens[a.y] > (long long)s->h_lens[b.x] * s->h_lens[b.y];
+  });
+  if (hipMalloc(&s->order, ord.size() * sizeof(int2)) != hipSuccess) {
+    s->order = nullptr;
+    return fail(c, ODBA_E_NOMEM, "cannot allocate the pair order");
+  }
+  HIPC(c, hipMemcpy(s->order, ord.data(), ord.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return ODBA_OK;
 }
 
 // Bands of strips (k_all_pairs_bands): claim table in the one-wave kernel's
@@ -330,6 +405,36 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
   return ODBA_OK;
 }
 
+// One all-vs-all launch under a given schedule into device buffers; the
+// kernel time (HIP events on the launch stream) in *ms.
+int run_all_pairs(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, int oe, long long npairs, void* dist,
+                  int* flags, double* ms) {
+  int rc;
+  if (ps.strips) {
+    rc = all_pairs_bands(c, s, ps, os, oe, npairs, dist, flags);
+    if (rc) return rc;
+  } else {
+    const long long stride = s->maxlen + 256;
+    rc = ensure_scratch(c, (size_t)ps.nblocks * 4 * 2 * stride * dtype_size(s->dtype));
+    if (rc) return rc;
+    if (ps.order == 1 && (rc = ensure_pair_order(c, s, npairs))) return rc;
+    unsigned long long* counter = (unsigned long long*)c->small;
+    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    HIPC(c, hipEventRecord(c->ev0, c->stream));
+    hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, ps.nblocks, c->stream, s->data, s->offs, s->lens,
+                                    s->n, npairs, counter, dist, c->esc, stride, flags, ps.order == 1 ? s->order : nullptr);
+    if (e != hipSuccess) return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e));
+    HIPC(c, hipEventRecord(c->ev1, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+  }
+  if (ms) {
+    float f = 0;
+    hipEventElapsedTime(&f, c->ev0, c->ev1);
+    *ms = f;
+  }
+  return ODBA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -348,13 +453,7 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences");
   const long long npairs = (long long)n * (n - 1) / 2;
   const size_t es = dtype_size(s->dtype);
-  const PairSchedule ps = choose_schedule(c, s, npairs);
-  const int nblocks = ps.nblocks;
-  const long long stride = s->maxlen + 256;
-  if (!ps.strips) {
-    int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);
-    if (rc) return rc;
-  }
+  const PairSchedule ps = choose_schedule(c, s, os, oe, npairs);
   void* dist = nullptr;
   int* flags = nullptr;
   if (hipMalloc(&dist, npairs * es) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
@@ -362,28 +461,93 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
     hipFree(dist);
     return fail(c, ODBA_E_NOMEM, "cannot allocate flags");
   }
-  if (ps.strips) {
-    const int rc = all_pairs_bands(c, s, ps, os, oe, npairs, dist, flags);
-    if (rc) { hipFree(dist); if (flags) hipFree(flags); return rc; }
-  } else {
-    unsigned long long* counter = (unsigned long long*)c->small;
-    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
-    HIPC(c, hipEventRecord(c->ev0, c->stream));
-    hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nblocks, c->stream, s->data, s->offs, s->lens, n,
-                                    npairs, counter, dist, c->esc, stride, flags);
-    if (e != hipSuccess) { hipFree(dist); if (flags) hipFree(flags); return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e)); }
-    HIPC(c, hipEventRecord(c->ev1, c->stream));
-  }
-  if (dist_out) HIPC(c, hipMemcpyAsync(dist_out, dist, npairs * es, hipMemcpyDeviceToHost, c->stream));
-  if (flags_out) HIPC(c, hipMemcpyAsync(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
-  if (kernel_ms) {
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    *kernel_ms = ms;
-  }
+  int rc = run_all_pairs(c, s, ps, os, oe, npairs, dist, flags, kernel_ms);
+  if (!rc && dist_out) rc = hipMemcpy(dist_out, dist, npairs * es, hipMemcpyDeviceToHost) == hipSuccess ? ODBA_OK : fail(c, ODBA_E_HIP, "copying distances");
+  if (!rc && flags_out) rc = hipMemcpy(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess ? ODBA_OK : fail(c, ODBA_E_HIP, "copying flags");
   hipFree(dist);
   if (flags) hipFree(flags);
+  return rc;
+}
+
+int odba_tune_all_pairs(odba_ctx* c, odba_seqset* s, int os, int oe, int reps, odba_schedule* best, double* best_ms,
+                        int* tried) {
+  if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
+  if (s->n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences");
+  HIPC(c, hipSetDevice(c->device));
+  reps = std::max(1, reps);
+  const long long npairs = (long long)s->n * (s->n - 1) / 2;
+  std::vector<PairSchedule> cand;
+  for (int order = 0; order < 2; order++) {
+    PairSchedule w;
+    w.order = order;
+    w.nblocks = grid_blocks(c, npairs);
+    cand.push_back(w);
+  }
+  if (bands_eligible(s) && npairs < 16LL * c->cus) {
+    for (int C : {1, 4, 16}) {
+      const long long smax = (s->maxlen + 64LL * C - 1) / (64LL * C);
+      if (C > 1 && s->maxlen < 64 * C) continue;
+      for (int w : {2, 4, 8, 16}) {
+        if (w > std::max<long long>(2, smax)) continue;
+        PairSchedule b;
+        b.strips = true;
+        b.C = C;
+        b.waves = w;
+        cand.push_back(b);
+      }
+    }
+  }
+  void* dist = nullptr;
+  if (hipMalloc(&dist, npairs * dtype_size(s->dtype)) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
+  double bms = 1e300;
+  PairSchedule bps = cand[0];
+  int rc = ODBA_OK;
+  for (const PairSchedule& ps : cand) {
+    std::vector<double> t;
+    for (int k = 0; k < reps && !rc; k++) {
+      double ms = 0;
+      rc = run_all_pairs(c, s, ps, os, oe, npairs, dist, nullptr, &ms);
+      t.push_back(ms);
+    }
+    if (rc) break;
+    std::sort(t.begin(), t.end());
+    const double med = t[t.size() / 2];
+    if (med < bms) { bms = med; bps = ps; }
+  }
+  hipFree(dist);
+  if (rc) return rc;
+  c->tuned[shape_key(s, os, oe)] = to_public(bps);
+  if (best) *best = to_public(bps);
+  if (best_ms) *best_ms = bms;
+  if (tried) *tried = (int)cand.size();
+  return ODBA_OK;
+}
+
+int odba_get_schedule(odba_ctx* c, odba_seqset* s, int os, int oe, odba_schedule* out) {
+  if (!c || !s || s->ctx != c || !out) return fail(c, ODBA_E_INVALID, "invalid arguments");
+  const long long npairs = (long long)s->n * (s->n - 1) / 2;
+  *out = to_public(choose_schedule(c, s, os, oe, npairs));
+  return ODBA_OK;
+}
+
+int odba_set_schedule(odba_ctx* c, odba_seqset* s, int os, int oe, const odba_schedule* u) {
+  if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
+  const std::string key = shape_key(s, os, oe);
+  if (!u) {
+    c->tuned.erase(key);
+    return ODBA_OK;
+  }
+  if (u->kind == ODBA_SCHED_WAVE) {
+    if (u->order != 0 && u->order != 1) return fail(c, ODBA_E_INVALID, "pair order %d", u->order);
+  } else if (u->kind == ODBA_SCHED_BANDS) {
+    if (!bands_eligible(s)) return fail(c, ODBA_E_INVALID, "bands need float (or integers that cannot wrap) and sequences of 64+ samples");
+    if ((u->band_c != 1 && u->band_c != 4 && u->band_c != 16) || u->band_waves < 1 || u->band_waves > 16 ||
+        u->ring < 128 || (u->ring & (u->ring - 1)))
+      return fail(c, ODBA_E_INVALID, "band schedule (C %d, waves %d, ring %d)", u->band_c, u->band_waves, u->ring);
+  } else {
+    return fail(c, ODBA_E_INVALID, "schedule kind %d", u->kind);
+  }
+  c->tuned[key] = *u;
   return ODBA_OK;
 }
 
@@ -438,7 +602,7 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   const size_t es = dtype_size(dtype);
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
-  HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned int), c->stream));
+  HIPC(c, launch_fill_u32(c->stream, counter, 0u, 1));  // a kernel: this may run inside a graph capture
   const char* env = std::getenv("ODBA_PATH_SCHEDULE");
   bool strips = njobs < 4 * c->cus && maxlx >= 2 * ODBA_W && minly >= 64;
   if (env && !std::strcmp(env, "wave")) strips = false;
@@ -575,8 +739,32 @@ template <typename T> struct DbaState {
   unsigned long long* pbuf = nullptr;
   size_t pbuf_cap = 0;
   std::vector<long long> h_poff;
+  // Graph replay of a round (graph_round): the jobs of the last eager round
+  // when it was a single batch of full-matrix paths without recording.
+  bool g_ok = false;
+  std::vector<PJ<T>> g_pj;
+  std::vector<char> g_flip;
+  int g_maxly = 0, g_maxlx = 0, g_minly = 0;
+  T* cbuf[2] = {nullptr, nullptr};         // the two centroid buffers
+  void* gjobs[2] = {nullptr, nullptr};     // job tables reading cbuf[p]
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<void*> gsig[2];              // device pointers baked into each graph
+  int* h_err = nullptr;                    // pinned outputs of a replay
+  unsigned long long* h_bits = nullptr;
+  T* h_cen = nullptr;
+
+  void drop_graphs() {
+    for (int p = 0; p < 2; p++) {
+      if (gexec[p]) hipGraphExecDestroy(gexec[p]);
+      gexec[p] = nullptr;
+      gsig[p].clear();
+    }
+  }
 
   ~DbaState() {
+    drop_graphs();
+    for (void* p : gjobs) if (p) hipFree(p);
+    for (void* p : {(void*)h_err, (void*)h_bits, (void*)h_cen}) if (p) hipHostFree(p);
     if (!own_seqs) seqdata = nullptr, offs = nullptr;
     for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
@@ -636,6 +824,10 @@ template <typename T> struct DbaState {
 
   int set_centroid(const T* h, int len) {
     if (len != lc) {
+      drop_graphs();
+      g_ok = false;
+      if (h_cen) hipHostFree(h_cen);
+      h_cen = nullptr;
       if (cen) hipFree(cen);
       if (newc) hipFree(newc);
       if (acc) hipFree(acc);
@@ -650,6 +842,8 @@ template <typename T> struct DbaState {
       HIPC(c, hipMalloc((void**)&counts, lc * sizeof(uint32_t)));
       HIPC(c, hipMemsetAsync(cen, 0, pl * sizeof(T), c->stream));
       HIPC(c, hipMemsetAsync(newc, 0, pl * sizeof(T), c->stream));
+      cbuf[0] = cen;
+      cbuf[1] = newc;
     }
     if (h) HIPC(c, hipMemcpyAsync(cen, h, lc * sizeof(T), hipMemcpyHostToDevice, c->stream));
     return ODBA_OK;
@@ -743,7 +937,15 @@ template <typename T> struct DbaState {
     return ODBA_OK;
   }
 
-  int round(double* delta) {
+  static bool graphs_enabled() {
+    const char* e = std::getenv("ODBA_DBA_GRAPH");
+    return !(e && !std::strcmp(e, "0"));
+  }
+
+  // One DBA round. With host_out, the new centroid is also copied there.
+  int round(double* delta, T* host_out = nullptr) {
+    if (g_ok && !rec && graphs_enabled()) return graph_round(delta, host_out);
+    g_ok = false;
     HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPC(c, hipMemsetAsync(dbits, 0, sizeof(unsigned long long), c->stream));
     int rc = grow(c, runs, runs_cap, (size_t)n * lc);
@@ -799,6 +1001,13 @@ template <typename T> struct DbaState {
         b1++;
       }
       const int nfull = (int)pj.size();
+      if (b0 == 0 && b1 == n && nfull == n && !rec) {  // replayable as a graph from the next round on
+        g_ok = true;
+        g_pj = pj;
+        g_flip.assign(n, 0);
+        for (int q = 0; q < n; q++) g_flip[q] = (char)(pj[q].y == cen);
+        g_maxly = maxly; g_maxlx = maxlx; g_minly = minly;
+      }
       if (nfull) {
         rc = grow(c, mv, mv_cap, (size_t)std::max(1LL, mvw));
         if (rc) return rc;
@@ -832,6 +1041,78 @@ template <typename T> struct DbaState {
     if (delta) *delta = d;
     std::swap(cen, newc);
     rec_valid = rec;
+    if (host_out) {
+      HIPC(c, hipMemcpyAsync(host_out, cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+    }
+    return ODBA_OK;
+  }
+
+  // The round as one hipGraph launch: clears by fill kernels (no memset
+  // nodes), path jobs, traceback, accumulation, mean/delta, and the error
+  // word, delta and new centroid copied to pinned host memory. One graph per
+  // centroid buffer the round reads; each is captured on first use and
+  // captured again when a device buffer it uses has moved.
+  int graph_round(double* delta, T* host_out) {
+    const int p = (cen == cbuf[0]) ? 0 : 1;
+    const int nj = (int)g_pj.size();
+    std::vector<void*> sig = {c->esc, c->small, (void*)mv, (void*)mtop, (void*)runs, (void*)acc, (void*)counts,
+                              (void*)abr, (void*)tbj, (void*)err, (void*)dbits, (void*)cbuf[0], (void*)cbuf[1],
+                              (void*)seqdata, (void*)offs};
+    if (gexec[p] && gsig[p] != sig) {
+      hipGraphExecDestroy(gexec[p]);
+      gexec[p] = nullptr;
+    }
+    if (!gexec[p]) {
+      if (!h_err) HIPC(c, hipHostMalloc((void**)&h_err, sizeof(int)));
+      if (!h_bits) HIPC(c, hipHostMalloc((void**)&h_bits, sizeof(unsigned long long)));
+      if (!h_cen) HIPC(c, hipHostMalloc((void**)&h_cen, std::max(1, lc) * sizeof(T)));
+      if (!gjobs[p]) HIPC(c, hipMalloc(&gjobs[p], nj * sizeof(PJ<T>)));
+      std::vector<PJ<T>> pj = g_pj;
+      for (int q = 0; q < nj; q++) {
+        if (g_flip[q]) pj[q].y = cbuf[p];
+        else pj[q].x = cbuf[p];
+      }
+      HIPC(c, hipMemcpyAsync(gjobs[p], pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = ODBA_OK;
+      hipError_t e = launch_fill_u32(c->stream, err, 0u, 1);
+      if (e == hipSuccess) e = launch_fill_u32(c->stream, dbits, 0u, 2);
+      if (e != hipSuccess) rc = fail(c, ODBA_E_HIP, "graph capture: %s", hipGetErrorString(e));
+      if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly);
+      if (!rc) {
+        e = launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, lc, os, oe, err, nullptr);
+        if (e == hipSuccess) e = launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs, runs, n, lc, acc, counts, 1);
+        if (e == hipSuccess) e = launch_mean_delta(dtype_of<T>(), c->stream, acc, counts, cbuf[p], cbuf[1 - p], lc, dbits, err);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_bits, dbits, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_cen, cbuf[1 - p], lc * sizeof(T), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) rc = fail(c, ODBA_E_HIP, "graph capture: %s", hipGetErrorString(e));
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+      if (!rc && ee != hipSuccess) rc = fail(c, ODBA_E_HIP, "end of graph capture: %s", hipGetErrorString(ee));
+      if (!rc && hipGraphInstantiate(&gexec[p], g, nullptr, nullptr, 0) != hipSuccess) {
+        gexec[p] = nullptr;
+        rc = fail(c, ODBA_E_HIP, "graph instantiation failed");
+      }
+      if (g) hipGraphDestroy(g);
+      if (rc) return rc;
+      gsig[p] = sig;
+    }
+    HIPC(c, hipGraphLaunch(gexec[p], c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const int herr = *h_err;
+    if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+    if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
+    if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
+    double d;
+    std::memcpy(&d, h_bits, sizeof(d));
+    if (delta) *delta = d;
+    std::swap(cen, newc);
+    rec_valid = false;
+    if (host_out) std::memcpy(host_out, h_cen, lc * sizeof(T));
     return ODBA_OK;
   }
 
@@ -1002,6 +1283,10 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   view.h_offs = ho; view.h_lens = hl; view.maxlen = maxlen; view.int_no_wrap = no_wrap;
   std::vector<T> d((size_t)n * (n - 1) / 2);
   int rc = odba_dtw_all_pairs_resident(c, &view, os, oe, d.data(), nullptr, nullptr);
+  if (view.order) {  // the view owns nothing else
+    hipFree(view.order);
+    view.order = nullptr;
+  }
   if (rc) return rc;
   if (dists_out) std::memcpy(dists_out, d.data(), d.size() * sizeof(T));
   const int med = medoid_single<T>(d.data(), l, n);
@@ -1018,11 +1303,9 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   int rounds = 0;
   for (int r = 0; r < max_rounds; r++) {  // dba.hpp:1084-1110
     double delta = 0;
-    rc = st.round(&delta);
+    rc = st.round(&delta, nb.data());
     if (rc) return rc;
     rounds = r + 1;
-    HIPC(c, hipMemcpyAsync(nb.data(), st.cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
     if (delta == 0) break;
     if (os || oe) {
       if (r >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * lc)) break;

@@ -31,9 +31,12 @@ inline long long swath_words(int ly, int os, int oe) {
   return nr > 0 ? (long long)(nr + 63) * 64 : 0;
 }
 
+// order (optional): npairs int2 {r, c} (r < c) giving the packing order;
+// null: second index descending, then first index descending.
 hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* data,
                             const long long* offs, const int* lens, int n, long long npairs,
-                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags);
+                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags,
+                            const void* order);
 hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,
                             int njobs, unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                             long long stride);
@@ -118,6 +121,7 @@ struct MultiSlot {
 hipError_t launch_multi_mean(int dtype, hipStream_t st, const MultiSlot* sl, int ns, long long total, void* acc,
                              uint32_t* counts, unsigned long long* delta_bits, int* err);
 
+hipError_t launch_fill_u32(hipStream_t st, void* p, uint32_t v, long long n);
 hipError_t launch_znorm(int dtype, hipStream_t st, void* data, const long long* offs, const int* lens, int n,
                         double* mu, double* sigma);
 hipError_t launch_decode_moves(hipStream_t st, const uint32_t* mv, const uint32_t* mtop, int ab, int ly, int lx, int os,

@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesP
                                                    const int* __restrict__ lens, int n, long long npairs,
                                                    unsigned long long* __restrict__ counter, T* __restrict__ dist,
                                                    T* __restrict__ escratch, long long escratch_stride,
-                                                   int* __restrict__ flags) {
+                                                   int* __restrict__ flags, const int2* __restrict__ order) {
   __shared__ T s_y[4][128];
   __shared__ T s_e[4][64];
   __shared__ T s_o[4][128];
@@ -650,7 +650,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesP
     const unsigned long long p = uniform_u64(atomicAdd(counter, lane == 0 ? 1ull : 0ull));
     if ((long long)p >= npairs) break;
     int r, c;
-    decode_pair((long long)p, npairs, r, c);
+    if (order) {  // an explicit packing order (r < c)
+      r = uniform_i32(order[p].x);
+      c = uniform_i32(order[p].y);
+    } else {
+      decode_pair((long long)p, npairs, r, c);
+    }
     PairOut po;
     T d = dtw_one_pair<T, OS, OE, MINFIRST, false>(data + offs[r], lens[r], data + offs[c], lens[c], e0, e1, nullptr,
                                                     nullptr, W, lane, po);
@@ -1219,11 +1224,11 @@ __global__ void __launch_bounds__(256) k_stripe_step(const StripeJob<T>* __restr
 template <typename T, bool MINFIRST>
 static hipError_t launch_all_pairs_t(int os, int oe, dim3 grid, hipStream_t st, const T* data, const long long* offs,
                                      const int* lens, int n, long long npairs, unsigned long long* counter, T* dist,
-                                     T* esc, long long stride, int* flags) {
-  if (os && oe) hipLaunchKernelGGL((k_all_pairs<T, true, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-  else if (os) hipLaunchKernelGGL((k_all_pairs<T, true, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-  else if (oe) hipLaunchKernelGGL((k_all_pairs<T, false, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
-  else hipLaunchKernelGGL((k_all_pairs<T, false, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags);
+                                     T* esc, long long stride, int* flags, const int2* order) {
+  if (os && oe) hipLaunchKernelGGL((k_all_pairs<T, true, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags, order);
+  else if (os) hipLaunchKernelGGL((k_all_pairs<T, true, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags, order);
+  else if (oe) hipLaunchKernelGGL((k_all_pairs<T, false, true, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags, order);
+  else hipLaunchKernelGGL((k_all_pairs<T, false, false, MINFIRST>), grid, dim3(256), 0, st, data, offs, lens, n, npairs, counter, dist, esc, stride, flags, order);
   return hipGetLastError();
 }
 
@@ -1241,20 +1246,22 @@ static hipError_t launch_path_jobs_t(int os, int oe, dim3 grid, hipStream_t st, 
 
 hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* data,
                             const long long* offs, const int* lens, int n, long long npairs,
-                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags) {
+                            unsigned long long* counter, void* dist, void* esc, long long stride, int* flags,
+                            const void* order) {
+  const int2* O = (const int2*)order;
   dim3 g(nblocks);
   switch (dtype) {
-    case ODBA_F32: return launch_all_pairs_t<float, true>(os, oe, g, st, (const float*)data, offs, lens, n, npairs, counter, (float*)dist, (float*)esc, stride, flags);
-    case ODBA_F64: return launch_all_pairs_t<double, true>(os, oe, g, st, (const double*)data, offs, lens, n, npairs, counter, (double*)dist, (double*)esc, stride, flags);
+    case ODBA_F32: return launch_all_pairs_t<float, true>(os, oe, g, st, (const float*)data, offs, lens, n, npairs, counter, (float*)dist, (float*)esc, stride, flags, O);
+    case ODBA_F64: return launch_all_pairs_t<double, true>(os, oe, g, st, (const double*)data, offs, lens, n, npairs, counter, (double*)dist, (double*)esc, stride, flags, O);
     case ODBA_I32:
-      if (minfirst) return launch_all_pairs_t<int32_t, true>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
-      return launch_all_pairs_t<int32_t, false>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags);
+      if (minfirst) return launch_all_pairs_t<int32_t, true>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags, O);
+      return launch_all_pairs_t<int32_t, false>(os, oe, g, st, (const int32_t*)data, offs, lens, n, npairs, counter, (int32_t*)dist, (int32_t*)esc, stride, flags, O);
     case ODBA_U32:
-      if (minfirst) return launch_all_pairs_t<uint32_t, true>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
-      return launch_all_pairs_t<uint32_t, false>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags);
+      if (minfirst) return launch_all_pairs_t<uint32_t, true>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags, O);
+      return launch_all_pairs_t<uint32_t, false>(os, oe, g, st, (const uint32_t*)data, offs, lens, n, npairs, counter, (uint32_t*)dist, (uint32_t*)esc, stride, flags, O);
     case ODBA_U64:
-      if (minfirst) return launch_all_pairs_t<uint64_t, true>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
-      return launch_all_pairs_t<uint64_t, false>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags);
+      if (minfirst) return launch_all_pairs_t<uint64_t, true>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags, O);
+      return launch_all_pairs_t<uint64_t, false>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags, O);
   }
   return hipErrorInvalidValue;
 }

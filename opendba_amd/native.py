@@ -25,8 +25,18 @@ EXPORTS = [
     "odba_dba_record_paths", "odba_dba_get_path", "odba_ctx_set_mem_cap",
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
-    "odba_cutree_k", "odba_prefix_chop",
+    "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
 ]
+SCHED_WAVE, SCHED_BANDS = 0, 1
+
+
+class Schedule(C.Structure):
+    """odba_schedule (odba_types.h)."""
+    _fields_ = [("kind", C.c_int), ("order", C.c_int), ("band_c", C.c_int), ("band_waves", C.c_int),
+                ("ring", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 
@@ -72,6 +82,10 @@ def load():
     L.odba_dba_get_path.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
     L.odba_ctx_set_mem_cap.argtypes = [vp, u64]
     L.odba_prefix_chop.argtypes = [vp, i32, vp, u64, vp, vp, i32, i32, vp, vp]
+    L.odba_tune_all_pairs.argtypes = [vp, vp, i32, i32, i32, C.POINTER(Schedule), C.POINTER(C.c_double),
+                                      C.POINTER(i32)]
+    L.odba_get_schedule.argtypes = [vp, vp, i32, i32, C.POINTER(Schedule)]
+    L.odba_set_schedule.argtypes = [vp, vp, i32, i32, C.POINTER(Schedule)]
     L.odba_server_create.argtypes = [i32, u64, i32, C.POINTER(vp)]
     L.odba_server_destroy.argtypes = [vp]
     L.odba_server_submit.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, C.POINTER(u64)]
@@ -358,6 +372,31 @@ class SeqSet:
         self.ctx._check(self.ctx.L.odba_dtw_all_pairs_resident(self.ctx.h, self.h, open_start, open_end, _p(out),
                                                                _p(fl), C.byref(ms)))
         return out, fl, ms.value
+
+    def tune(self, open_start=0, open_end=0, reps=3):
+        """Autotune the all-vs-all schedule for this set and mode
+        (odba_tune_all_pairs); returns (schedule dict, kernel ms, candidates)."""
+        best = Schedule()
+        ms = C.c_double(0)
+        tried = C.c_int(0)
+        self.ctx._check(self.ctx.L.odba_tune_all_pairs(self.ctx.h, self.h, open_start, open_end, reps, C.byref(best),
+                                                       C.byref(ms), C.byref(tried)))
+        return best.as_dict(), ms.value, tried.value
+
+    def schedule(self, open_start=0, open_end=0):
+        out = Schedule()
+        self.ctx._check(self.ctx.L.odba_get_schedule(self.ctx.h, self.h, open_start, open_end, C.byref(out)))
+        return out.as_dict()
+
+    def set_schedule(self, sched, open_start=0, open_end=0):
+        """Pin a schedule (a dict with odba_schedule's fields) or, with None,
+        go back to the automatic choice."""
+        if sched is None:
+            self.ctx._check(self.ctx.L.odba_set_schedule(self.ctx.h, self.h, open_start, open_end, None))
+            return
+        u = Schedule(**{k: int(sched.get(k, d)) for k, d in
+                        (("kind", 0), ("order", 0), ("band_c", 16), ("band_waves", 4), ("ring", 2048))})
+        self.ctx._check(self.ctx.L.odba_set_schedule(self.ctx.h, self.h, open_start, open_end, C.byref(u)))
 
     def close(self):
         if self.h:
