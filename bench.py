@@ -200,6 +200,30 @@ def run_stripes(args, N, WL):
     R[iu] = dr
     back = R[::-1, ::-1].T[iu]
     same = bool(np.array_equal(back.view(np.uint32), d.view(np.uint32)))
+    # the DP traceback of every sequence against the medoid: one DBA round in
+    # stripe mode (checkpointed block recomputes), with the paths recorded
+    D = F + F.T
+    med = int(np.argmin((D.astype(np.float64) ** 2).sum(axis=1)))
+    tb = {}
+    if not args.no_traceback:
+        with N.Context(0) as ctx2, N.DbaSession(ctx2, seqs, 0, 0, path_policy=N.PATH_STRIPE) as s:
+            s.set_centroid(seqs[med])
+            s.record_paths(True)
+            t1 = time.perf_counter()
+            newc, counts, delta = s.step()
+            tb_s = time.perf_counter() - t1
+            ok, steps = True, 0
+            for k in range(n):
+                pi, pj, pc = s.path(k)
+                steps += len(pi)
+                ok = ok and pi[0] == len(seqs[k]) - 1 and pj[0] == len(seqs[med]) - 1 and pi[-1] == 0 and pj[-1] == 0
+                ok = ok and bool(np.all(np.diff(pi) <= 0) and np.all(np.diff(pj) <= 0))
+                ok = ok and bool(np.all((np.diff(pi) < 0) | (np.diff(pj) < 0)))
+            ok = ok and int(counts.astype(np.int64).sum()) == steps  # global mode: every path cell is counted
+        print(f"traceback round: {tb_s:.1f} s, {steps:,} path cells, checks {ok}", file=sys.stderr, flush=True)
+        fwd = float(sum(len(x) for x in seqs)) * len(seqs[med])
+        tb = {"dba_round_stripe_s": round(tb_s, 2), "medoid": med, "path_cells": steps, "forward_cells": fwd,
+              "paths_monotone_and_counts_match": ok}
     kmed = float(np.median(kms))
     value = nominal / (kmed / 1000.0) / 1e9
     achieved = nominal * VALU_OPS_PER_CELL / (kmed / 1000.0) / 1e12
@@ -211,7 +235,8 @@ def run_stripes(args, N, WL):
         "config": {"workload": f"few long pairs: {n} float32 warped-motif sequences, lengths U[{lmin},{lmax}], "
                                f"global full unbanded all-vs-all ({n * (n - 1) // 2} pairs), banded strip schedule",
                    "lengths": lens, "full_matrix_cells": nominal, "device": info["name"],
-                   "kernel_ms_median": round(kmed, 1), "transposed_run_bit_identical": same},
+                   "kernel_ms_median": round(kmed, 1), "transposed_run_bit_identical": same,
+                   "traceback": tb or "skipped"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "TOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": "k_all_pairs_bands<float, C=16, os=0, oe=0>", "kernel_ms": round(kmed, 1),
@@ -349,6 +374,7 @@ def main():
     ap.add_argument("--long-n", type=int, default=8, help="stripes: sequences")
     ap.add_argument("--long-min", type=int, default=1_000_000, help="stripes: shortest length")
     ap.add_argument("--long-max", type=int, default=4_000_000, help="stripes: longest length")
+    ap.add_argument("--no-traceback", action="store_true", help="stripes: skip the DBA round's tracebacks")
     ap.add_argument("--n-series", type=int, default=5000, help="clusters: squiggles")
     ap.add_argument("--clusters", type=int, default=250, help="clusters: K (clusters of 2+ members)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")

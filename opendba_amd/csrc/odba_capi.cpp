@@ -34,6 +34,10 @@ struct odba_ctx {
   void* small = nullptr;  // counters, flags (256 bytes)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::map<std::string, odba_schedule> tuned;  // all-vs-all schedules by workload shape
+  // Pinned host staging for graph replays, grow-only: pinned allocations
+  // can wait for the whole device, which would stall other contexts' work.
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
 };
 
 struct odba_seqset {
@@ -165,6 +169,7 @@ int odba_ctx_destroy(odba_ctx* c) {
   hipStreamSynchronize(c->stream);
   if (c->esc) hipFree(c->esc);
   if (c->small) hipFree(c->small);
+  if (c->pinned) hipHostFree(c->pinned);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   hipStreamDestroy(c->stream);
@@ -347,7 +352,8 @@ int ensure_pair_order(odba_ctx* c, odba_seqset* s, long long npairs) {
     s->order = nullptr;
     return fail(c, ODBA_E_NOMEM, "cannot allocate the pair order");
   }
-  HIPC(c, hipMemcpy(s->order, ord.data(), ord.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpyAsync(s->order, ord.data(), ord.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
   return ODBA_OK;
 }
 
@@ -462,8 +468,13 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
     return fail(c, ODBA_E_NOMEM, "cannot allocate flags");
   }
   int rc = run_all_pairs(c, s, ps, os, oe, npairs, dist, flags, kernel_ms);
-  if (!rc && dist_out) rc = hipMemcpy(dist_out, dist, npairs * es, hipMemcpyDeviceToHost) == hipSuccess ? ODBA_OK : fail(c, ODBA_E_HIP, "copying distances");
-  if (!rc && flags_out) rc = hipMemcpy(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess ? ODBA_OK : fail(c, ODBA_E_HIP, "copying flags");
+  // copies on the context's stream: a plain hipMemcpy runs on the null
+  // stream and would wait for every other context's work on the device
+  if (!rc && dist_out && hipMemcpyAsync(dist_out, dist, npairs * es, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = fail(c, ODBA_E_HIP, "copying distances");
+  if (!rc && flags_out && hipMemcpyAsync(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = fail(c, ODBA_E_HIP, "copying flags");
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, ODBA_E_HIP, "stream synchronisation");
   hipFree(dist);
   if (flags) hipFree(flags);
   return rc;
@@ -512,6 +523,9 @@ int odba_tune_all_pairs(odba_ctx* c, odba_seqset* s, int os, int oe, int reps, o
     if (rc) break;
     std::sort(t.begin(), t.end());
     const double med = t[t.size() / 2];
+    if (std::getenv("ODBA_TUNE_VERBOSE"))
+      std::fprintf(stderr, "odba tune: %s order %d C %d waves %d: %.4f ms\n", ps.strips ? "bands" : "wave", ps.order,
+                   ps.C, ps.waves, med);
     if (med < bms) { bms = med; bps = ps; }
   }
   hipFree(dist);
@@ -732,6 +746,24 @@ template <typename T> struct DbaState {
   size_t sjobs_cap = 0;
   TbJob* stbj = nullptr;
   size_t stbj_cap = 0;
+  // Checkpointed global paths (float): row and column checkpoints, block
+  // step words, strip slots, rings, jobs and walk states.
+  float* ck_rowck = nullptr;
+  size_t ck_rowck_cap = 0;
+  float* ck_colck = nullptr;
+  size_t ck_colck_cap = 0;
+  int* ck_cnt = nullptr;
+  size_t ck_cnt_cap = 0;
+  uint32_t* ck_mv = nullptr;
+  size_t ck_mv_cap = 0;
+  float* ck_slots = nullptr;
+  size_t ck_slots_cap = 0;
+  float* ck_rings = nullptr;
+  size_t ck_rings_cap = 0;
+  unsigned char* ck_jobs = nullptr;
+  size_t ck_jobs_cap = 0;
+  unsigned char* ck_states = nullptr;
+  size_t ck_states_cap = 0;
   // Traceback recording (the cells writeDTWPath prints): per sequence a
   // count slot then up to |seq| + |centroid| packed steps.
   bool rec = false;
@@ -742,6 +774,7 @@ template <typename T> struct DbaState {
   // Graph replay of a round (graph_round): the jobs of the last eager round
   // when it was a single batch of full-matrix paths without recording.
   bool g_ok = false;
+  int g_eager = 0;                         // consecutive replayable eager rounds
   std::vector<PJ<T>> g_pj;
   std::vector<char> g_flip;
   int g_maxly = 0, g_maxlx = 0, g_minly = 0;
@@ -749,7 +782,7 @@ template <typename T> struct DbaState {
   void* gjobs[2] = {nullptr, nullptr};     // job tables reading cbuf[p]
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<void*> gsig[2];              // device pointers baked into each graph
-  int* h_err = nullptr;                    // pinned outputs of a replay
+  int* h_err = nullptr;                    // pinned outputs of a replay (in the context's staging)
   unsigned long long* h_bits = nullptr;
   T* h_cen = nullptr;
 
@@ -764,11 +797,12 @@ template <typename T> struct DbaState {
   ~DbaState() {
     drop_graphs();
     for (void* p : gjobs) if (p) hipFree(p);
-    for (void* p : {(void*)h_err, (void*)h_bits, (void*)h_cen}) if (p) hipHostFree(p);
     if (!own_seqs) seqdata = nullptr, offs = nullptr;
     for (void* p : {(void*)seqdata, (void*)offs, (void*)cen, (void*)newc, (void*)acc, (void*)counts, (void*)runs,
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
-                    (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf})
+                    (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf,
+                    (void*)ck_rowck, (void*)ck_colck, (void*)ck_cnt, (void*)ck_mv, (void*)ck_slots, (void*)ck_rings,
+                    (void*)ck_jobs, (void*)ck_states})
       if (p) hipFree(p);
   }
 
@@ -826,8 +860,6 @@ template <typename T> struct DbaState {
     if (len != lc) {
       drop_graphs();
       g_ok = false;
-      if (h_cen) hipHostFree(h_cen);
-      h_cen = nullptr;
       if (cen) hipFree(cen);
       if (newc) hipFree(newc);
       if (acc) hipFree(acc);
@@ -877,6 +909,98 @@ template <typename T> struct DbaState {
     const bool flip = oe && (uint64_t)lc < ls;
     const double need = (double)((flip ? (uint64_t)lc : ls) * sizeof(T)) + (double)ls * (double)lc * 1.05;
     return (double)free_mem < need;
+  }
+
+  // In global mode a stripe-mode path equals the full matrix's, so float
+  // jobs that take stripe mode use checkpointed block recomputes instead,
+  // which scale to million-sample sequences. ODBA_CKPT=0 keeps the stripe
+  // recompute; ODBA_CKPT_RSHIFT (rows per band = 2^k) and ODBA_CKPT_EW
+  // (1024-column strips per block, 2 or 16) set the block size.
+  bool use_ckpt() const {
+    const char* e = std::getenv("ODBA_CKPT");
+    return std::is_same<T, float>::value && !os && !oe && !(e && !std::strcmp(e, "0"));
+  }
+
+  int run_ckpt_batch(const std::vector<int>& idx, int b0) {
+    if constexpr (!std::is_same<T, float>::value) {
+      return fail(c, ODBA_E_UNSUPPORTED, "checkpointed paths are float only");
+    } else {
+      const int nj = (int)idx.size();
+      if (!nj) return ODBA_OK;
+      const int rshift = std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 14)));
+      const int ew = env_int("ODBA_CKPT_EW", 16) == 2 ? 2 : 16;
+      const long long R = 1LL << rshift, E = (long long)ew * ODBA_W;
+      struct Job {
+        const float* y; const float* x; int ly, lx;
+        long long rowck_off, colck_off, cnt_off, claim0, runs_off, path_off;
+      };
+      struct State {
+        int i, j, curc, lo1, hi1, lo2, hi2, gap;
+        long long np;
+        int done, err, bi, pad;
+      };
+      if (sizeof(Job) != ckpt_job_size() || sizeof(State) != ckpt_state_size())
+        return fail(c, ODBA_E_INVALID, "checkpoint record layout mismatch");
+      std::vector<Job> jobs(nj);
+      std::vector<State> sts(nj);
+      long long nrow = 0, ncol = 0, ncnt = 0, nclaims = 0;
+      int maxit = 0;
+      for (int q = 0; q < nj; q++) {
+        const int b = idx[q];
+        const int ly = h_lens[b], lx = lc;
+        const long long nb = (ly + R - 1) / R, nm = (lx + E - 1) / E;
+        jobs[q] = Job{seqdata + h_offs[b], cen, ly, lx, nrow, ncol, ncnt, nclaims, (long long)(b - b0) * lc,
+                      rec ? h_poff[b] : -1};
+        nrow += (nb - 1) * lx;
+        ncol += (nm - 1) * ((long long)ly + 256);
+        ncnt += std::max(0LL, nm - 1);
+        nclaims += nm;
+        maxit = std::max<int>(maxit, (int)(nb + nm + 1));
+        sts[q] = State{ly - 1, lx - 1, lx - 1, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0};
+      }
+      const long long job_words = (long long)ew * (R + 63) * 64;
+      const int ring = 2048;
+      const int nblocks = (int)std::max<long long>(1, std::min<long long>(nclaims, (long long)std::max(1, 16 / ew) * c->cus));
+      int rc;
+      if ((rc = grow(c, ck_rowck, ck_rowck_cap, (size_t)std::max(1LL, nrow))) ||
+          (rc = grow(c, ck_colck, ck_colck_cap, (size_t)std::max(1LL, ncol))) ||
+          (rc = grow(c, ck_cnt, ck_cnt_cap, (size_t)std::max(1LL, ncnt))) ||
+          (rc = grow(c, ck_mv, ck_mv_cap, (size_t)nj * job_words)) ||
+          (rc = grow(c, ck_slots, ck_slots_cap, (size_t)nj * ew * (R + 256))) ||
+          (rc = grow(c, ck_rings, ck_rings_cap, (size_t)nblocks * ew * ring)) ||
+          (rc = grow(c, ck_jobs, ck_jobs_cap, (size_t)nj * sizeof(Job))) ||
+          (rc = grow(c, ck_states, ck_states_cap, (size_t)nj * sizeof(State))))
+        return rc;
+      HIPC(c, hipMemcpyAsync(ck_jobs, jobs.data(), nj * sizeof(Job), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(ck_states, sts.data(), nj * sizeof(State), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemsetAsync(ck_cnt, 0, std::max(1LL, ncnt) * sizeof(int), c->stream));
+      unsigned long long* counter = (unsigned long long*)c->small;
+      int* ferr = (int*)((char*)c->small + 192);
+      HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+      HIPC(c, hipMemsetAsync(ferr, 0, sizeof(int), c->stream));
+      int khz = 0;
+      if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+      const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;
+      HIPC(c, launch_ckpt_forward(ew, nblocks, c->stream, ck_jobs, nj, nclaims, counter, ck_rowck, ck_colck, ck_cnt,
+                                  ck_rings, ring, rshift, ferr, limit));
+      for (int it = 0; it < maxit; it++) {
+        HIPC(c, launch_ckpt_block(ew, c->stream, ck_jobs, nj, ck_states, ck_rowck, ck_colck, ck_mv, job_words,
+                                  ck_slots, R + 256, rshift));
+        HIPC(c, launch_ckpt_trace(c->stream, ck_jobs, nj, ck_states, ck_mv, job_words, runs, rshift, ew,
+                                  rec ? pbuf : nullptr));
+      }
+      int herr = 0;
+      HIPC(c, hipMemcpyAsync(&herr, ferr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipMemcpyAsync(sts.data(), ck_states, nj * sizeof(State), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      if (herr & 8) return fail(c, ODBA_E_HIP, "checkpoint forward pass: edge hand-off timed out");
+      for (int q = 0; q < nj; q++) {
+        if (sts[q].err & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+        if (sts[q].err & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
+        if (!sts[q].done) return fail(c, ODBA_E_TRACEBACK, "checkpointed traceback of sequence %d did not finish", idx[q]);
+      }
+      return ODBA_OK;
+    }
   }
 
   int run_stripe_batch(const std::vector<int>& idx, int b0) {
@@ -941,10 +1065,17 @@ template <typename T> struct DbaState {
     const char* e = std::getenv("ODBA_DBA_GRAPH");
     return !(e && !std::strcmp(e, "0"));
   }
+  // Capturing and instantiating a round costs about as much as a few
+  // eager rounds' launch overhead, so replays start after a few rounds.
+  static int graph_after() {
+    const char* e = std::getenv("ODBA_DBA_GRAPH_AFTER");
+    return (e && *e) ? std::max(1, std::atoi(e)) : 4;
+  }
 
   // One DBA round. With host_out, the new centroid is also copied there.
   int round(double* delta, T* host_out = nullptr) {
-    if (g_ok && !rec && graphs_enabled()) return graph_round(delta, host_out);
+    if (g_ok && !rec && graphs_enabled() && g_eager >= graph_after()) return graph_round(delta, host_out);
+    if (!g_ok) g_eager = 0;
     g_ok = false;
     HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPC(c, hipMemsetAsync(dbits, 0, sizeof(unsigned long long), c->stream));
@@ -976,8 +1107,13 @@ template <typename T> struct DbaState {
         const int flip = (oe && lc < ls) ? 1 : 0;
         const int ly = flip ? lc : ls, lx = flip ? ls : lc;
         const bool st = use_stripe(b1);
-        const long long need = st ? ((long long)((lx + ODBA_W - 1) / ODBA_W) * (ly + 256) * (long long)sizeof(T) + swath_words(ly, os, oe) * 4) / 4
-                                  : (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        long long need = st ? ((long long)((lx + ODBA_W - 1) / ODBA_W) * (ly + 256) * (long long)sizeof(T) + swath_words(ly, os, oe) * 4) / 4
+                            : (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        if (st && use_ckpt()) {  // checkpoint rows and columns plus one block's words (4-byte units)
+          const long long R = 1LL << std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 14)));
+          const long long E = (long long)(env_int("ODBA_CKPT_EW", 16) == 2 ? 2 : 16) * ODBA_W;
+          need = ((ly + R - 1) / R) * (long long)lx + ((lx + E - 1) / E) * ((long long)ly + 256) + (E / ODBA_W) * (R + 63) * 64 * 2;
+        }
         if (b1 > b0 && mvw + sbytes + need > budget_words) break;
         if (st) {
           stripe_idx.push_back(b1);
@@ -1019,7 +1155,7 @@ template <typename T> struct DbaState {
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
-      rc = run_stripe_batch(stripe_idx, b0);
+      rc = use_ckpt() ? run_ckpt_batch(stripe_idx, b0) : run_stripe_batch(stripe_idx, b0);
       if (rc) return rc;
       HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, b1 - b0, lc, acc, counts, first ? 1 : 0));
       // the host vectors must outlive the async copies
@@ -1045,6 +1181,7 @@ template <typename T> struct DbaState {
       HIPC(c, hipMemcpyAsync(host_out, cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
       HIPC(c, hipStreamSynchronize(c->stream));
     }
+    g_eager = g_ok ? g_eager + 1 : 0;
     return ODBA_OK;
   }
 
@@ -1056,17 +1193,27 @@ template <typename T> struct DbaState {
   int graph_round(double* delta, T* host_out) {
     const int p = (cen == cbuf[0]) ? 0 : 1;
     const int nj = (int)g_pj.size();
+    const size_t need = 64 + (size_t)std::max(1, lc) * sizeof(T);
+    if (c->pinned_bytes < need) {
+      if (c->pinned) hipHostFree(c->pinned);
+      c->pinned = nullptr;
+      c->pinned_bytes = 0;
+      const size_t sz = std::max<size_t>(need, (size_t)1 << 20);
+      HIPC(c, hipHostMalloc(&c->pinned, sz));
+      c->pinned_bytes = sz;
+    }
+    h_err = (int*)c->pinned;
+    h_bits = (unsigned long long*)((char*)c->pinned + 8);
+    h_cen = (T*)((char*)c->pinned + 64);
     std::vector<void*> sig = {c->esc, c->small, (void*)mv, (void*)mtop, (void*)runs, (void*)acc, (void*)counts,
                               (void*)abr, (void*)tbj, (void*)err, (void*)dbits, (void*)cbuf[0], (void*)cbuf[1],
-                              (void*)seqdata, (void*)offs};
+                              (void*)seqdata, (void*)offs, c->pinned};
     if (gexec[p] && gsig[p] != sig) {
       hipGraphExecDestroy(gexec[p]);
       gexec[p] = nullptr;
     }
     if (!gexec[p]) {
-      if (!h_err) HIPC(c, hipHostMalloc((void**)&h_err, sizeof(int)));
-      if (!h_bits) HIPC(c, hipHostMalloc((void**)&h_bits, sizeof(unsigned long long)));
-      if (!h_cen) HIPC(c, hipHostMalloc((void**)&h_cen, std::max(1, lc) * sizeof(T)));
+
       if (!gjobs[p]) HIPC(c, hipMalloc(&gjobs[p], nj * sizeof(PJ<T>)));
       std::vector<PJ<T>> pj = g_pj;
       for (int q = 0; q < nj; q++) {

@@ -18,8 +18,6 @@
 #include <float.h>
 #include <limits.h>
 #include <stdint.h>
-#include <stdlib.h>
-#include <string.h>
 
 #include "odba_dtw.hpp"
 
@@ -261,206 +259,125 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
 }
 
 // ---------------------------------------------------------------------------
-// Row-segment traceback: the same walk, results and recorded cells as
-// k_traceback, one path ROW per iteration instead of one cell. In a row the
-// path enters at column c_in, takes RIGHT steps to the first cell (going
-// left) whose code is UP or DIAG, and leaves the row there. The wave finds
-// that cell with one search: every lane tests its C columns of the row
-// (2-bit codes of one word) for a non-RIGHT code at or left of c_in, a
-// ballot picks the rightmost lane with one. The row's words come from a
-// de-skewed 64-row tile of the swath in LDS (tile[row][lane], loaded with
-// coalesced step rows; stride 65 on the write side, so conflict-free), so a
-// row costs one LDS read and a dozen scalar and vector instructions. The
-// open-end top row searches its own words the same way; runs of positional
-// cells (abrogated swaths, row 0) are emitted in bulk. Every run of cells
-// goes through run_cells, which applies exactly k_traceback's per-cell
-// bookkeeping (position change -> flush, counted / uncounted runs), with
-// the middle of long runs written lane-parallel.
+// Checkpointed global paths: the walk inside the block k_ckpt_block has just
+// recomputed (rows bR .. bi, columns mE .. j0; bi and j0 are the state's
+// cell at launch), cell by cell as k_traceback walks a full matrix, until
+// the path leaves the block downwards or to the left (the state then holds
+// the first cell outside it) or reaches the corner. Global mode: every cell
+// is counted, row 0 is positional (RIGHT, then NIL at the corner).
 // ---------------------------------------------------------------------------
-struct RowWalk {
-  int4* R;
-  unsigned long long* P;
-  bool flip;
-  int lane;
-  int curc;
-  RunRec rr;
+struct CkJobD {  // layout of CkJob in the swath kernels
+  const float* y;
+  const float* x;
+  int ly, lx;
+  long long rowck_off, colck_off, cnt_off, claim0, runs_off, path_off;
+};
+struct CkStateD {  // layout of CkState
+  int i, j, curc, lo1, hi1, lo2, hi2, gap;
   long long np;
-  int e;
-
-  // Cells (i, chi), (i, chi-1), ..., (i, clo) in traceback order, all with
-  // step code `code`, counted or not.
-  __device__ void run_cells(int i, int chi, int clo, int code, bool counted, bool rec) {
-    const int n = chi - clo + 1;
-    if (rec && P)
-      for (int b = 0; b < n; b += 64)
-        if (b + lane < n) P[1 + np + b + lane] = pack_step(i, chi - b - lane, code);
-    np += n;
-    if (!flip) {  // position = column
-      if (chi != curc) {
-        if (lane == 0) R[curc] = rr.out();
-        curc = chi;
-        rr.reset();
-      }
-      if (!rr.add(counted, i)) e |= 4;
-      if (clo < chi) {
-        if (lane == 0) R[chi] = rr.out();
-        const int4 one = counted ? make_int4(i, i, 1, 0) : make_int4(1, 0, 1, 0);
-        for (int c = clo + 1 + lane; c <= chi - 1; c += 64) R[c] = one;
-        curc = clo;
-        rr.reset();
-        rr.add(counted, i);
-      }
-    } else {  // position = row
-      if (i != curc) {
-        if (lane == 0) R[curc] = rr.out();
-        curc = i;
-        rr.reset();
-      }
-      if (!counted) {
-        if (rr.hi1 >= rr.lo1) rr.gap = 1;
-      } else if (!rr.gap) {
-        if (rr.hi1 < rr.lo1) rr.hi1 = chi;
-        rr.lo1 = clo;
-      } else if (rr.hi2 < rr.lo2) {
-        rr.hi2 = chi;
-        rr.lo2 = clo;
-      } else if (rr.lo2 == chi + 1) {
-        rr.lo2 = clo;
-      } else {
-        e |= 4;
-      }
-    }
-  }
+  int done, err, bi, pad;
 };
 
-// Rightmost column <= cmax of the lane words `w` (lane l: columns off + l*C
-// .. off + l*C + C-1, 2 bits each) whose code is not RIGHT (1). Returns the
-// column, or -1, and the 2-bit code in *code2.
-__device__ __forceinline__ int find_non_right(uint32_t w, int off, int C, int cmax, int lane, uint32_t* code2) {
-  const int qmax = min(C - 1, cmax - (off + lane * C));
-  uint32_t cand = 0;
-  if (qmax >= 0) {
-    const uint32_t x = w ^ 0x55555555u;
-    const uint32_t nz = (x | (x >> 1)) & 0x55555555u;
-    const uint32_t m = qmax >= 15 ? 0xFFFFFFFFu : ((1u << (2 * qmax + 2)) - 1u);
-    cand = nz & m;
-  }
-  const unsigned long long bal = __ballot(cand != 0u);
-  if (!bal) return -1;
-  const int L = 63 - __clzll((long long)bal);
-  const uint32_t cw = (uint32_t)__builtin_amdgcn_readlane((int)cand, L);
-  const uint32_t ww = (uint32_t)__builtin_amdgcn_readlane((int)w, L);
-  const int qb = (31 - __clz((int)cw)) >> 1;
-  *code2 = (ww >> (2 * qb)) & 3u;
-  return off + L * C + qb;
-}
-
-constexpr int kTbWaves = 2;  // waves per workgroup (16 KB of LDS tile each)
-
 template <bool REC>
-__global__ void __launch_bounds__(64 * kTbWaves) k_traceback_rows(const TbJob* __restrict__ jobs, int njobs,
-                                                                  const uint32_t* __restrict__ mv,
-                                                                  const uint32_t* __restrict__ mtop,
-                                                                  const int* __restrict__ abr, int4* __restrict__ runs,
-                                                                  int lc, int os, int oe, int* __restrict__ err,
-                                                                  unsigned long long* __restrict__ pbuf) {
-  __shared__ uint32_t s_tile[kTbWaves][64 * 64];
+__global__ void __launch_bounds__(256) k_ckpt_trace(const CkJobD* __restrict__ jobs, int njobs,
+                                                    CkStateD* __restrict__ st, const uint32_t* __restrict__ mv,
+                                                    long long job_words, int4* __restrict__ runs, int rshift, int ew,
+                                                    unsigned long long* __restrict__ pbuf) {
   const int lane = threadIdx.x & 63;
-  const int wid = (int)(threadIdx.x >> 6);
-  const int jid = wave_uniform((int)(blockIdx.x * kTbWaves + wid));
-  if (jid >= njobs) return;
-  uint32_t* T = s_tile[wid];
-  const TbJob J = jobs[jid];
-  const Geo g = make_geo(J.ly, J.lx, os, oe, abr[jid]);
-  const uint32_t* M = mv + J.mv_off;
-  const uint32_t* TP = mtop + J.top_off;
-  RowWalk W;
-  W.R = J.rp ? J.rp : runs + J.runs_off;
-  W.P = (REC && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
-  W.flip = J.flip != 0;
-  W.lane = lane;
-  W.np = 0;
-  W.e = 0;
-  W.rr.reset();
-  const int top = (g.oe && g.ly >= 2) ? g.ly - 1 : INT32_MAX;  // open-end top row (its own words)
-  const int abcol = g.ab >= 0 ? g.ab * ODBA_W : INT32_MAX;
-  const int nsteps = (int)(g.S64 >> 6);
-  int i = wave_uniform(J.ly - 1), j = wave_uniform(J.lx - 1);
-  W.curc = W.flip ? i : j;
-  int tk = -1, tib = -1;  // tile: swath tk, rows tib-63 .. tib
+  const int t = wave_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (t >= njobs) return;
+  const CkJobD J = jobs[t];
+  CkStateD S = st[t];
+  if (wave_uniform(S.done | S.err)) return;
+  int i = wave_uniform(S.i), j = wave_uniform(S.j);
+  const int bR = (i >> rshift) << rshift;
+  const int E = ew * ODBA_W;
+  const int mE = (j / E) * E;
+  const int Rb = i - bR + 1;
+  const long long S64 = (long long)(Rb + 63) * 64;
+  const int K = (J.lx + ODBA_W - 1) / ODBA_W;
+  const int wl = J.lx - (K - 1) * ODBA_W;
+  const int lastC = wl <= 64 ? 1 : wl <= 128 ? 2 : wl <= 256 ? 4 : wl <= 512 ? 8 : wl <= 768 ? 12 : 16;
+  const uint32_t* M = mv + (long long)t * job_words;
+  int4* R = runs + J.runs_off;
+  unsigned long long* P = (REC && J.path_off >= 0) ? pbuf + J.path_off : nullptr;
+  RunRec rr;
+  rr.lo1 = S.lo1; rr.hi1 = S.hi1; rr.lo2 = S.lo2; rr.hi2 = S.hi2; rr.gap = S.gap;
+  int curc = wave_uniform(S.curc);
+  long long np = S.np;
+  int e = 0, done = 0;
+  ColCursor cc;
+  cc.set(j, K, lastC);
+  const int k0 = mE / ODBA_W;  // the block's first swath
+  WordTile tile;
+  tile.init();
   for (;;) {
-    i = wave_uniform(i);
-    j = wave_uniform(j);
-    if (i == 0 && j == 0) {  // the corner (dba.hpp:412-415)
-      const int code = g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
-      W.run_cells(0, 0, 0, code, code == ODBA_NIL, REC);
+    if (i == 0) {  // row 0 to the corner: RIGHT steps, then NIL, all counted
+      for (; j >= 0; j--) {
+        if (REC) {
+          if (P && lane == 0) P[1 + np] = pack_step(0, j, j == 0 ? ODBA_NIL : ODBA_RIGHT);
+          np++;
+        }
+        if (j != curc) {
+          if (lane == 0) R[curc] = rr.out();
+          curc = j;
+          rr.reset();
+        }
+        if (!rr.add(true, 0)) e |= 4;
+      }
+      done = 1;
       break;
     }
-    if (j >= abcol) {  // top row of abrogated swaths: OPEN_RIGHT, not counted
-      if (i != g.ly - 1) { W.e |= 1; break; }
-      W.run_cells(i, j, abcol, ODBA_OPEN_RIGHT, false, REC);
-      j = abcol - 1;
-      continue;
+    if (i < bR || j < mE) break;  // left the block: the next launch recomputes the next one
+    i = wave_uniform(i); j = wave_uniform(j); curc = wave_uniform(curc);
+    cc.k = wave_uniform(cc.k); cc.l = wave_uniform(cc.l); cc.q = wave_uniform(cc.q); cc.C = wave_uniform(cc.C);
+    tile.i0 = wave_uniform(tile.i0); tile.l0 = wave_uniform(tile.l0); tile.k0 = wave_uniform(tile.k0);
+    rr.lo1 = wave_uniform(rr.lo1); rr.hi1 = wave_uniform(rr.hi1); rr.lo2 = wave_uniform(rr.lo2);
+    rr.hi2 = wave_uniform(rr.hi2); rr.gap = wave_uniform(rr.gap);
+    if (j != curc) {
+      if (lane == 0) R[curc] = rr.out();
+      curc = j;
+      rr.reset();
     }
-    if (i == 0) {  // row 0 to the corner
-      const int code = g.os ? ODBA_OPEN_RIGHT : ODBA_RIGHT;
-      W.run_cells(0, j, 1, code, !g.os, REC);
-      j = 0;
-      continue;
+    const uint32_t c2 = (tile.get(M, S64, 0, cc.k - k0, i - bR, cc.l, lane) >> (2 * cc.q)) & 3u;  // 0 DIAG 1 RIGHT 2 UP
+    if (REC) {
+      if (P && lane == 0) P[1 + np] = pack_step(i, j, c2 == 0 ? ODBA_DIAGONAL : (c2 == 2 ? ODBA_UP : ODBA_RIGHT));
+      np++;
     }
-    // a row with stored words (main region or the open-end top row)
-    const bool is_top = (i == top);
-    int cmax = j, cout = -1;
-    uint32_t code2 = 0;
-    for (;;) {
-      const int k = cmax >> 10;
-      const int off = k * ODBA_W;
-      const int C = (k == g.K - 1) ? g.lastC : 16;
-      uint32_t w;
-      if (is_top) {
-        w = TP[(long long)k * 64 + lane];
-      } else {
-        if (!(k == tk && i <= tib && i > tib - 64)) {  // (re)load the tile of rows i-63 .. i
-          tk = k;
-          tib = i;
-          const long long base = (long long)k * g.S64;
-          const int s_lo = i - 63 - g.r0;
-#pragma unroll 8
-          for (int u = 0; u < 127; u++) {
-            const int s = s_lo + u;
-            if (s >= 0 && s < nsteps) {
-              const int t = i - (s - lane + g.r0);
-              const uint32_t x = M[base + (long long)s * 64 + lane];
-              if (t >= 0 && t < 64) T[t * 64 + lane] = x;
-            }
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-        w = T[(tib - i) * 64 + lane];
-      }
-      cout = wave_uniform(find_non_right(w, off, C, cmax, lane, &code2));
-      code2 = (uint32_t)wave_uniform((int)code2);
-      if (cout >= 0) break;
-      // the rest of this swath's part of the row is RIGHT (OPEN_RIGHT on the top row)
-      W.run_cells(i, cmax, off, is_top ? ODBA_OPEN_RIGHT : ODBA_RIGHT, !is_top, REC);
-      cmax = off - 1;
-      if (cmax < 0) break;
+    if (!rr.add(true, i)) e |= 4;
+    const bool up = c2 != 1u, lf = c2 != 2u;
+    if (lf) {
+      if (lane == 0) R[curc] = rr.out();
+      rr.reset();
+      curc = j - 1;
     }
-    if (cout < 0) { W.e |= 1; break; }
-    if (cmax > cout) W.run_cells(i, cmax, cout + 1, is_top ? ODBA_OPEN_RIGHT : ODBA_RIGHT, !is_top, REC);
-    const int code = code2 == 0 ? ODBA_DIAGONAL : ODBA_UP;
-    W.run_cells(i, cout, cout, code, true, REC);
-    i--;
-    j = code == ODBA_DIAGONAL ? cout - 1 : cout;
-    if (i < 0 || j < 0) { W.e |= 1; break; }
+    i -= up ? 1 : 0;
+    if (lf) { j--; cc.left(); }
+    if (i < 0 || j < 0) { e |= 1; break; }
   }
   if (lane == 0) {
-    W.R[W.curc] = W.rr.out();
-    if (REC && W.P) W.P[0] = (unsigned long long)W.np;
-    if (W.e) atomicOr(err, W.e);
+    if (done) R[curc] = rr.out();
+    S.i = i; S.j = j; S.curc = curc;
+    S.lo1 = rr.lo1; S.hi1 = rr.hi1; S.lo2 = rr.lo2; S.hi2 = rr.hi2; S.gap = rr.gap;
+    S.np = np;
+    S.done = done;
+    S.err = e;
+    st[t] = S;
+    if (REC && P && done) P[0] = (unsigned long long)np;
   }
+}
+
+hipError_t launch_ckpt_trace(hipStream_t st, const void* jobs, int njobs, void* states, const uint32_t* mv,
+                             long long job_words, int4* runs, int rshift, int ew, unsigned long long* pbuf) {
+  if (njobs <= 0) return hipSuccess;
+  const dim3 g((njobs + 3) / 4), b(256);
+  if (pbuf)
+    hipLaunchKernelGGL(k_ckpt_trace<true>, g, b, 0, st, (const CkJobD*)jobs, njobs, (CkStateD*)states, mv, job_words,
+                       runs, rshift, ew, pbuf);
+  else
+    hipLaunchKernelGGL(k_ckpt_trace<false>, g, b, 0, st, (const CkJobD*)jobs, njobs, (CkStateD*)states, mv, job_words,
+                       runs, rshift, ew, pbuf);
+  return hipGetLastError();
 }
 
 // Stripe-mode traceback for stripe k = K-1-m of every job (updateCentroid
@@ -561,6 +478,22 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
   }
 }
 
+// a += p[hi], p[hi-1], ..., p[lo], in that order. The values are loaded
+// eight at a time before they are added, so a long run costs one memory
+// latency per eight values instead of one per value.
+template <typename T>
+__device__ __forceinline__ T add_run_desc(T a, const T* __restrict__ p, int hi, int lo) {
+  for (int k = hi; k >= lo; k -= 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (k - u >= lo) ? p[k - u] : T(0);
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k - u >= lo) a = (T)(a + v[u]);
+  }
+  return a;
+}
+
 template <typename T>
 __global__ void k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
                              const int4* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
@@ -590,12 +523,12 @@ __global__ void k_accumulate(const T* __restrict__ data, const long long* __rest
     for (int u = 0; u < B; u++) {
       if (r[u].y >= r[u].x) {
         a = (T)(a + f1[u]);
-        for (int k = r[u].y - 1; k >= r[u].x; k--) a = (T)(a + p[u][k]);
+        if (r[u].y > r[u].x) a = add_run_desc<T>(a, p[u], r[u].y - 1, r[u].x);
         n += (uint32_t)(r[u].y - r[u].x + 1);
       }
       if (r[u].w >= r[u].z) {
         a = (T)(a + f2[u]);
-        for (int k = r[u].w - 1; k >= r[u].z; k--) a = (T)(a + p[u][k]);
+        if (r[u].w > r[u].z) a = add_run_desc<T>(a, p[u], r[u].w - 1, r[u].z);
         n += (uint32_t)(r[u].w - r[u].z + 1);
       }
     }
@@ -672,11 +605,11 @@ __global__ void k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, lon
     const int4 r = S.runs[(long long)s * S.lc + c];
     const T* p = data + S.moffs[s];
     if (r.y >= r.x) {
-      for (int k = r.y; k >= r.x; k--) a = (T)(a + p[k]);
+      a = add_run_desc<T>(a, p, r.y, r.x);
       n += (uint32_t)(r.y - r.x + 1);
     }
     if (r.w >= r.z) {
-      for (int k = r.w; k >= r.z; k--) a = (T)(a + p[k]);
+      a = add_run_desc<T>(a, p, r.w, r.z);
       n += (uint32_t)(r.w - r.z + 1);
     }
   }
@@ -811,13 +744,6 @@ __global__ void k_decode_moves(const uint32_t* __restrict__ mv, const uint32_t* 
 hipError_t launch_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv, const uint32_t* mtop,
                             const int* abr, int4* runs, int lc, int os, int oe, int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  const char* env = getenv("ODBA_TRACEBACK");
-  if (!(env && !strcmp(env, "steps"))) {  // the row-segment walk (ODBA_TRACEBACK=steps: one cell per step)
-    const dim3 g((njobs + kTbWaves - 1) / kTbWaves), b(64 * kTbWaves);
-    if (pbuf) hipLaunchKernelGGL(k_traceback_rows<true>, g, b, 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err, pbuf);
-    else hipLaunchKernelGGL(k_traceback_rows<false>, g, b, 0, st, jobs, njobs, mv, mtop, abr, runs, lc, os, oe, err, pbuf);
-    return hipGetLastError();
-  }
   if (pbuf)
     hipLaunchKernelGGL(k_traceback<true>, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, mv, mtop, abr, runs, lc,
                        os, oe, err, pbuf);

@@ -94,6 +94,21 @@ hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                               long long stride);
 
+// Checkpointed global float paths (see k_ckpt_forward in the swath kernels):
+// jobs are CkJob records (ckpt_job_size() bytes), states CkState records
+// (ckpt_state_size() bytes); ew = strips of 1024 columns per column block
+// (2 or 16), rows per band = 2^rshift.
+hipError_t launch_ckpt_forward(int ew, int nblocks, hipStream_t st, const void* jobs, int njobs, long long nclaims,
+                               unsigned long long* counter, float* rowck, float* colck, int* cnt, float* rings,
+                               int ring, int rshift, int* err, long long tick_limit);
+hipError_t launch_ckpt_block(int ew, hipStream_t st, const void* jobs, int njobs, void* states, const float* rowck,
+                             const float* colck, uint32_t* mv, long long job_words, float* slots,
+                             long long slot_stride, int rshift);
+hipError_t launch_ckpt_trace(hipStream_t st, const void* jobs, int njobs, void* states, const uint32_t* mv,
+                             long long job_words, int4* runs, int rshift, int ew, unsigned long long* pbuf);
+size_t ckpt_job_size();
+size_t ckpt_state_size();
+
 hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
                          const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,
                          uint32_t* mtop, StripeState* ss);

@@ -234,6 +234,15 @@ template <typename T> struct SwathIO {
   int ref_left;    // 1: `off` is a reference swath start (a multiple of 1024)
   int imask = -1;  // row index mask of ein: -1 (whole column) or ring rows - 1
   int omask = -1;  // row index mask of eout
+  // Checkpointed global paths: `top` (when set) is the cost row above the
+  // first row, indexed by column (top[off-1] included when off > 0); with
+  // the CKPT template flag, the rows whose global index (row + ck_rbase) is
+  // one below a multiple of 2^ck_rshift, and below ck_rlimit, are stored to
+  // rowck[(global row >> ck_rshift) * ck_stride + column].
+  const T* top = nullptr;
+  T* rowck = nullptr;
+  long long ck_stride = 0;
+  int ck_rbase = 0, ck_rshift = 0, ck_rlimit = 0;
 };
 
 // Pipeline hooks of run_swath. A wave that sweeps a whole pair needs none.
@@ -256,7 +265,7 @@ template <typename T> struct SwathRes {
 
 // Runs one swath of C columns per lane. Rows r0..r1 go through the skewed
 // main loop; with open_end the top row goes through the prefix-min scan.
-template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH, class Pipe = NoPipe>
+template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH, class Pipe = NoPipe, bool CKPT = false>
 __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int lane, const Pipe& pipe = Pipe()) {
   const T MAXV = Ar<T>::maxv();
   const T RINF = Ar<T>::rowinf();
@@ -285,6 +294,21 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
   } else {
     diag_in = OS ? T(0) : RINF;
   }
+  if (a.top) {  // a block below a stored cost row (global mode); columns past lx-1 are padding
+#pragma unroll
+    for (int q = 0; q < C; q++) D[q] = (col0 + q < a.lx) ? a.top[col0 + q] : MAXV;
+    diag_in = (col0 > 0 && col0 - 1 < a.lx) ? a.top[col0 - 1] : MAXV;
+  }
+  const int ck_mask = (1 << a.ck_rshift) - 1;
+  auto ckpt = [&](int row, bool on) {  // store a checkpoint row of this lane's columns
+    const int grow = row + a.ck_rbase;
+    if (on && ((grow + 1) & ck_mask) == 0 && grow < a.ck_rlimit) {
+      T* dst = a.rowck + (long long)(grow >> a.ck_rshift) * a.ck_stride + col0;
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        if (col0 + q < a.lx) dst[q] = D[q];  // the row holds columns 0 .. lx-1
+    }
+  };
 
   SwathRes<T> res;
   T runmin = MAXV;
@@ -364,6 +388,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           l = v;
         }
         if (PATH) a.mv[(size_t)s * 64 + lane] = word;
+        if (CKPT) ckpt(r0 + s - lane, true);
         W->o[u - lane + 64] = D[C - 1];
       }
     } else {
@@ -401,6 +426,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           l = v;
         }
         if (PATH && active) a.mv[(size_t)s * 64 + lane] = word;
+        if (CKPT) ckpt(row, active);
         W->o[u - lane + 64] = D[C - 1];
       }
     }
@@ -1111,6 +1137,188 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Checkpointed paths (global mode, float): the step codes of a long pair
+// without its whole step matrix. In global mode the path does not depend on
+// how the matrix is cut (every cell's code follows from exact cost values),
+// so the stripe mode's result is the full matrix's, and any exact recompute
+// gives it.
+//
+//   * Forward (k_ckpt_forward): the whole matrix once, as bands of 16
+//     strips of 1024 columns (one block of E columns per claim, claims in
+//     job order), keeping two kinds of checkpoint: the cost row at the
+//     bottom of every band of R rows (rowck[b][j] = D(bR+R-1, j)) and the
+//     cost column at the right edge of every column block
+//     (colck[m][i] = D(i, mE+E-1), which is also the hand-off to the next
+//     block, published in a per-block progress word).
+//   * Then, block by block along the path (k_ckpt_block, one workgroup per
+//     job): the block that holds the path's current cell is recomputed from
+//     its top checkpoint row and left checkpoint column, rows up to the
+//     current cell's, columns up to its column, storing step codes; the
+//     traceback (k_ckpt_trace) walks it until the path leaves the block
+//     (down or left), carrying the walk state to the next launch.
+// ---------------------------------------------------------------------------
+struct CkJob {
+  const float* y;  // rows
+  const float* x;  // columns
+  int ly, lx;
+  long long rowck_off;  // elements: (nb - 1) rows of lx
+  long long colck_off;  // elements: (nm - 1) columns of ly + 256
+  long long cnt_off;    // ints: nm - 1 progress words
+  long long claim0;     // the job's first claim (column block 0)
+  long long runs_off;   // its run records (one per column of X)
+  long long path_off;   // its recorded cells, or -1
+};
+
+template <int EW>
+__global__ void __launch_bounds__(1024) k_ckpt_forward(const CkJob* __restrict__ jobs, int njobs, long long nclaims,
+                                                       unsigned long long* __restrict__ counter,
+                                                       float* __restrict__ rowck, float* __restrict__ colck,
+                                                       int* __restrict__ cnt, float* __restrict__ rings, int ring,
+                                                       int rshift, int* __restrict__ err, long long limit) {
+  __shared__ float s_y[kMaxStripWaves][128];
+  __shared__ float s_e[kMaxStripWaves][64];
+  __shared__ float s_o[kMaxStripWaves][128];
+  __shared__ int s_cnt[kMaxStripWaves];
+  __shared__ int s_used[kMaxStripWaves];
+  __shared__ unsigned long long s_q;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  float* rbase = rings + (long long)blockIdx.x * EW * ring;
+  const WaveWin<float> win{s_y[wid], s_e[wid], s_o[wid]};
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
+    if (threadIdx.x < kMaxStripWaves) {
+      s_cnt[threadIdx.x] = 0;
+      s_used[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const long long q = (long long)uniform_u64(s_q);
+    if (q >= nclaims) break;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].claim0 <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const int t = uniform_i32(lo);
+    const CkJob J = jobs[t];
+    const int m = uniform_i32((int)(q - J.claim0));
+    const int S = (J.lx + ODBA_W - 1) / ODBA_W;
+    const int s0 = m * EW, slast = min(s0 + EW, S) - 1;
+    const int s = s0 + wid;
+    if (wid >= EW || s > slast) continue;
+    const int off = s * ODBA_W;
+    const bool has_out = s < S - 1;
+    const bool in_global = (wid == 0) && m > 0;
+    const bool out_global = (s == slast) && has_out;
+    const long long cstride = (long long)J.ly + 256;
+    BandPipe pp;
+    pp.in_global = in_global;
+    pp.in_cnt = in_global ? &cnt[J.cnt_off + m - 1] : (wid > 0 ? &s_cnt[wid - 1] : nullptr);
+    pp.in_used = (wid > 0) ? &s_used[wid - 1] : nullptr;
+    pp.out_cnt = has_out ? (out_global ? &cnt[J.cnt_off + m] : &s_cnt[wid]) : nullptr;
+    pp.out_global = out_global;
+    pp.out_used = (has_out && !out_global) ? &s_used[wid] : nullptr;
+    pp.ring = ring;
+    pp.main_end = J.ly;
+    pp.lane = lane;
+    pp.err = err;
+    pp.limit = limit;
+    SwathIO<float> io;
+    io.y = J.y; io.ly = J.ly; io.x = J.x; io.lx = J.lx; io.off = off;
+    io.ein = s == 0 ? nullptr : (in_global ? colck + J.colck_off + (long long)(m - 1) * cstride : rbase + (long long)(wid - 1) * ring);
+    io.imask = in_global ? -1 : ring - 1;
+    io.eout = has_out ? (out_global ? colck + J.colck_off + (long long)m * cstride : rbase + (long long)wid * ring) : nullptr;
+    io.omask = out_global ? -1 : ring - 1;
+    io.mv = nullptr; io.mtop = nullptr;
+    io.ref_left = 1;
+    io.rowck = rowck + J.rowck_off;
+    io.ck_stride = J.lx;
+    io.ck_rbase = 0;
+    io.ck_rshift = rshift;
+    const int nb = (J.ly + (1 << rshift) - 1) >> rshift;
+    io.ck_rlimit = (nb - 1) << rshift;
+    const int wcols = min(ODBA_W, J.lx - off);
+    const int C = (s == S - 1) ? last_swath_c(wcols) : 16;
+    switch (C) {
+      case 1: run_swath<float, 1, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+      case 2: run_swath<float, 2, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+      case 4: run_swath<float, 4, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+      case 8: run_swath<float, 8, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+      case 12: run_swath<float, 12, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+      default: run_swath<float, 16, false, false, true, false, BandPipe, true>(io, &win, lane, pp); break;
+    }
+  }
+}
+
+// Walk state of one checkpointed path between launches.
+struct CkState {
+  int i, j;          // current cell
+  int curc;          // current centroid position
+  int lo1, hi1, lo2, hi2, gap;
+  long long np;      // recorded cells so far
+  int done, err;
+  int bi;            // the block's top row when it was recomputed (its rows bR .. bi)
+  int pad;
+};
+
+// Recomputes, for every job not done, the block holding its current cell:
+// rows bR .. i, columns mE .. j, step words [strip][step][lane] (strip =
+// 1024 columns, steps over rows bR .. i) at mv + t * job_words. Strip w
+// hands its right edge to strip w+1 through slot w (LDS progress words);
+// strip 0 reads the left checkpoint column, which is complete.
+template <int EW>
+__global__ void __launch_bounds__(1024) k_ckpt_block(const CkJob* __restrict__ jobs, CkState* __restrict__ st,
+                                                     const float* __restrict__ rowck, const float* __restrict__ colck,
+                                                     uint32_t* __restrict__ mv, long long job_words,
+                                                     float* __restrict__ slots, long long slot_stride, int rshift) {
+  __shared__ float s_y[kMaxStripWaves][128];
+  __shared__ float s_e[kMaxStripWaves][64];
+  __shared__ float s_o[kMaxStripWaves][128];
+  __shared__ int s_owner[kMaxStripWaves + 1];
+  __shared__ int s_cnt[kMaxStripWaves + 1];
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int t = blockIdx.x;
+  const CkJob J = jobs[t];
+  const int done = uniform_i32(st[t].done | st[t].err);
+  const int i = uniform_i32(st[t].i), j = uniform_i32(st[t].j);
+  if (threadIdx.x <= kMaxStripWaves) {
+    s_owner[threadIdx.x] = threadIdx.x < kMaxStripWaves ? (int)threadIdx.x : -1;
+    s_cnt[threadIdx.x] = threadIdx.x < kMaxStripWaves ? 0 : INT_MAX;  // the last one: always ready
+  }
+  __syncthreads();
+  if (done || i < 0 || j < 0) return;
+  const int R = 1 << rshift;
+  const int b = i >> rshift;
+  const int m = j / (EW * ODBA_W);
+  const int bR = b * R;
+  const int Rb = i - bR + 1;
+  const int s = m * EW + wid;  // global strip (swath) index
+  const int off = s * ODBA_W;
+  if (wid >= EW || off > j) return;
+  const int S = (J.lx + ODBA_W - 1) / ODBA_W;
+  const bool has_out = off + ODBA_W <= j;  // strip w+1 of this block is needed
+  const int in_slot = wid > 0 ? wid - 1 : kMaxStripWaves;
+  StripPipe pp{&s_owner[in_slot], &s_cnt[in_slot], wid > 0 ? wid - 1 : -1, has_out ? &s_cnt[wid] : nullptr, lane};
+  const WaveWin<float> win{s_y[wid], s_e[wid], s_o[wid]};
+  SwathIO<float> io;
+  io.y = J.y + bR; io.ly = Rb; io.x = J.x; io.lx = J.lx; io.off = off;
+  const long long cstride = (long long)J.ly + 256;
+  if (wid == 0) io.ein = m > 0 ? colck + J.colck_off + (long long)(m - 1) * cstride + bR : nullptr;
+  else io.ein = slots + ((long long)t * EW + wid - 1) * slot_stride;
+  io.eout = has_out ? slots + ((long long)t * EW + wid) * slot_stride : nullptr;
+  io.mv = mv + (long long)t * job_words + (long long)wid * ((long long)(Rb + 63) * 64);
+  io.mtop = nullptr;
+  io.ref_left = 1;
+  io.top = b > 0 ? rowck + J.rowck_off + (long long)(b - 1) * J.lx : nullptr;
+  const int wcols = min(ODBA_W, J.lx - off);
+  const int C = (s == S - 1) ? last_swath_c(wcols) : 16;
+  (void)swath_dispatch_pipe<float, false, false, true, StripPipe>(C, io, &win, lane, pp);
+}
+
+// ---------------------------------------------------------------------------
 // Stripe mode (dba.hpp:523-548, 646-763). Forward: every swath's right edge
 // column goes to its own slot (slots + k*H), no step codes. Backward: the
 // host steps m = 0, 1, ...; job j recomputes its stripe k = K_j-1-m with the
@@ -1324,6 +1532,37 @@ hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, i
 }
 
 size_t band_pair_size() { return sizeof(BandPair); }
+
+hipError_t launch_ckpt_forward(int ew, int nblocks, hipStream_t st, const void* jobs, int njobs, long long nclaims,
+                               unsigned long long* counter, float* rowck, float* colck, int* cnt, float* rings,
+                               int ring, int rshift, int* err, long long tick_limit) {
+  if (ring < 128 || (ring & (ring - 1))) return hipErrorInvalidValue;
+  const CkJob* J = (const CkJob*)jobs;
+  const dim3 g(nblocks), b(64 * ew);
+  switch (ew) {
+    case 2: hipLaunchKernelGGL(k_ckpt_forward<2>, g, b, 0, st, J, njobs, nclaims, counter, rowck, colck, cnt, rings, ring, rshift, err, tick_limit); break;
+    case 16: hipLaunchKernelGGL(k_ckpt_forward<16>, g, b, 0, st, J, njobs, nclaims, counter, rowck, colck, cnt, rings, ring, rshift, err, tick_limit); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ckpt_block(int ew, hipStream_t st, const void* jobs, int njobs, void* states, const float* rowck,
+                             const float* colck, uint32_t* mv, long long job_words, float* slots,
+                             long long slot_stride, int rshift) {
+  const CkJob* J = (const CkJob*)jobs;
+  CkState* S = (CkState*)states;
+  const dim3 g(njobs), b(64 * ew);
+  switch (ew) {
+    case 2: hipLaunchKernelGGL(k_ckpt_block<2>, g, b, 0, st, J, S, rowck, colck, mv, job_words, slots, slot_stride, rshift); break;
+    case 16: hipLaunchKernelGGL(k_ckpt_block<16>, g, b, 0, st, J, S, rowck, colck, mv, job_words, slots, slot_stride, rshift); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+size_t ckpt_job_size() { return sizeof(CkJob); }
+size_t ckpt_state_size() { return sizeof(CkState); }
 
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,

@@ -12,6 +12,7 @@ import oracle_lib as O
 from opendba_amd import native as N
 
 pytestmark = pytest.mark.gpu
+os.environ["ODBA_DBA_GRAPH_AFTER"] = "1"  # replay from the second round (the default waits longer)
 
 
 @pytest.fixture(scope="module")
