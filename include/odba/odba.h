@@ -70,6 +70,14 @@ int odba_dtw_all_pairs_resident(odba_ctx* ctx, odba_seqset* set, int use_open_st
                                 void* dist_out, int* flags_out, double* kernel_ms);
 
 /*
+ * DBA rounds of this context are replayed as hipGraphs after a few eager
+ * rounds (default on; no counterpart in the reference). Measured 1-3%
+ * faster alone on a device, but slower when several contexts keep the same
+ * device busy (the serving loop turns it off for its worker contexts).
+ */
+int odba_ctx_set_graphs(odba_ctx* ctx, int on);
+
+/*
  * All-vs-all autotuner (no counterpart in the reference). Times every
  * candidate schedule on the resident set in this mode: one wave per pair in
  * both packing orders, and, when the set is eligible (float, or integers

@@ -38,6 +38,7 @@ struct odba_ctx {
   // can wait for the whole device, which would stall other contexts' work.
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  bool graphs = true;  // replay DBA rounds as hipGraphs (off for contexts that share a busy device)
 };
 
 struct odba_seqset {
@@ -444,6 +445,12 @@ int run_all_pairs(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, i
 }  // namespace
 
 extern "C" {
+
+int odba_ctx_set_graphs(odba_ctx* c, int on) {
+  if (!c) return ODBA_E_INVALID;
+  c->graphs = on != 0;
+  return ODBA_OK;
+}
 
 int odba_ctx_set_mem_cap(odba_ctx* c, uint64_t bytes) {
   if (!c || bytes == 0) return ODBA_E_INVALID;
@@ -1061,9 +1068,10 @@ template <typename T> struct DbaState {
     return ODBA_OK;
   }
 
-  static bool graphs_enabled() {
+  bool graphs_enabled() const {
     const char* e = std::getenv("ODBA_DBA_GRAPH");
-    return !(e && !std::strcmp(e, "0"));
+    if (e && !std::strcmp(e, "1")) return true;
+    return c->graphs && !(e && !std::strcmp(e, "0"));
   }
   // Capturing and instantiating a round costs about as much as a few
   // eager rounds' launch overhead, so replays start after a few rounds.

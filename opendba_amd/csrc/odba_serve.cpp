@@ -175,7 +175,10 @@ int odba_server_create(int device, uint64_t hbm_budget, int slots, odba_server**
   odba_device_info(s->ctxs[0], &cus, &clk, &mem, name, sizeof(name));
   s->cus = cus > 0 ? cus : 256;
   s->budget = hbm_budget ? std::min<uint64_t>(hbm_budget, mem) : mem / 10 * 8;
-  for (odba_ctx* c : s->ctxs) odba_ctx_set_mem_cap(c, s->budget / (uint64_t)slots);
+  for (odba_ctx* c : s->ctxs) {
+    odba_ctx_set_mem_cap(c, s->budget / (uint64_t)slots);
+    odba_ctx_set_graphs(c, 0);  // graph replays serialise the slots' streams (measured)
+  }
   for (int k = 0; k < slots; k++) s->workers.emplace_back([s, k] { s->worker(k); });
   *out = s;
   return ODBA_OK;

@@ -26,6 +26,7 @@ EXPORTS = [
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
     "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
+    "odba_ctx_set_graphs",
 ]
 SCHED_WAVE, SCHED_BANDS = 0, 1
 
@@ -81,6 +82,7 @@ def load():
     L.odba_dba_record_paths.argtypes = [vp, i32]
     L.odba_dba_get_path.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
     L.odba_ctx_set_mem_cap.argtypes = [vp, u64]
+    L.odba_ctx_set_graphs.argtypes = [vp, i32]
     L.odba_prefix_chop.argtypes = [vp, i32, vp, u64, vp, vp, i32, i32, vp, vp]
     L.odba_tune_all_pairs.argtypes = [vp, vp, i32, i32, i32, C.POINTER(Schedule), C.POINTER(C.c_double),
                                       C.POINTER(i32)]
