@@ -379,7 +379,7 @@ def main():
     ap.add_argument("--clusters", type=int, default=250, help="clusters: K (clusters of 2+ members)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
     ap.add_argument("--scale", type=float, default=0.5, help="serve: series-count and length scale")
-    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent job slots")
+    ap.add_argument("--slots", type=int, default=16, help="serve: concurrent job slots")
     ap.add_argument("--max-rounds", type=int, default=250, help="serve: DBA round limit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the schedule autotuner")
@@ -391,6 +391,12 @@ def main():
               f"pair cost across GPUs by design, but this build measures one GPU)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve":
+        # One hardware queue per slot stream (HIP reads this when it starts;
+        # with fewer queues than streams, slots share queues and their kernels
+        # run one after another).
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < args.slots:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(args.slots)
     from opendba_amd import native as N
     from opendba_amd import workloads as WL
 

@@ -141,6 +141,91 @@ struct ColCursor {
   }
 };
 
+// Even bits of w packed down (bit 2q -> bit q): the low or high bits of 16
+// two-bit step codes as a 16-bit mask.
+__device__ __forceinline__ uint32_t even_bits(uint32_t x) {
+  x &= 0x55555555u;
+  x = (x | (x >> 1)) & 0x33333333u;
+  x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+  x = (x | (x >> 4)) & 0x00FF00FFu;
+  x = (x | (x >> 8)) & 0x0000FFFFu;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Move masks of a 64-row tile for the run-skipping walk: lane t holds row
+// i0 - t; bit x of rm / um is set when the cell in tile column x (absolute
+// column cb + x, word-lanes l-3..l of one swath, 4*C columns) steps RIGHT /
+// UP; a cell with neither bit steps DIAG. Columns below xmin (word-lanes of
+// the previous swath) and lanes past tmax (rows below the main region) are
+// outside the tile.
+struct MoveTile {
+  uint64_t rm, um;          // per lane
+  int i0, cb, xmin, tmax;   // wave-uniform
+  __device__ void init() { rm = um = 0; i0 = -1; cb = 0; xmin = 0; tmax = -1; }
+  __device__ bool covers(int i, int j) const {
+    const int t = i0 - i, x = j - cb;
+    return t >= 0 && t <= tmax && x >= xmin && x < 64;
+  }
+  // Place cell (i, j) at lane 0 in the tile's rightmost word-lane.
+  __device__ void fill(const Geo& g, const uint32_t* __restrict__ M, int i, int j, int rmin, int lane) {
+    const int k = j >> 10, tt = j & (ODBA_W - 1);
+    const int C = (k == g.K - 1) ? g.lastC : 16;
+    const int l = tt / C, q = tt - l * C;
+    const int row = i - lane;
+    const uint32_t* base = M + (long long)k * g.S64;
+    uint32_t w[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const int lw = l - 3 + d;
+      w[d] = (row >= g.r0 && lw >= 0) ? base[(long long)(row - g.r0 + lw) * 64 + lw] : 0u;
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    const uint32_t cm = (1u << C) - 1u;
+    uint64_t r = 0, u = 0;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const uint32_t lo = even_bits(w[d]) & cm, hi = even_bits(w[d] >> 1) & cm;
+      r |= (uint64_t)lo << (d * C);        // codes 1 and 3 step RIGHT
+      u |= (uint64_t)(hi & ~lo) << (d * C);  // code 2 steps UP
+    }
+    rm = r; um = u;
+    i0 = i; cb = j - 3 * C - q;
+    xmin = l >= 3 ? 0 : (3 - l) * C;
+    tmax = min(63, i - rmin);
+  }
+};
+
+// First column at or left of jj whose open-end top-row code is DIAG or UP
+// (not OPEN_RIGHT), or -1: lane l of swath k holds the codes of columns
+// k*W + l*C .. + C-1, so one ballot per swath finds the word and a bit scan
+// the column.
+__device__ int top_run_end(const Geo& g, const uint32_t* __restrict__ TP, int jj, int lane) {
+  int k = jj >> 10;
+  const int tt = jj & (ODBA_W - 1);
+  int C = (k == g.K - 1) ? g.lastC : 16;
+  int l = tt / C, q = tt - l * C;
+  for (; k >= 0; k--) {
+    const uint32_t w = TP[(long long)k * 64 + lane];
+    uint32_t nr = ~even_bits(w) & ((1u << C) - 1u);  // codes 0 (DIAG) and 2 (UP)
+    if (lane > l) nr = 0u;
+    else if (lane == l) nr &= (2u << q) - 1u;
+    const uint64_t b = (uint64_t)__ballot(nr != 0u);
+    if (b) {
+      const int lh = 63 - __builtin_clzll(b);
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)nr, lh);
+      return k * ODBA_W + lh * C + (31 - __builtin_clz(m));
+    }
+    C = 16; l = 63; q = 15;
+  }
+  return -1;
+}
+
 // One wavefront per aligned sequence (all lanes walk the same path; lane 0
 // stores). runs[seq][c] = counted runs of sequence indices for centroid
 // position c. With a path buffer and J.path_off >= 0 every visited cell is
@@ -174,6 +259,8 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
   rr.reset();
   WordTile tile;
   tile.init();
+  MoveTile mt;
+  mt.init();
   int topk = -1;
   uint32_t topw = 0;
   long long np = 0;
@@ -189,6 +276,61 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
         curc = c;
         rr.reset();
       }
+    }
+    // Without a recorded path, the main region is walked a run at a time: a
+    // run is a maximal stretch of one move (RIGHT along the row, UP down the
+    // column, DIAG down the diagonal) inside a MoveTile, found with one mask
+    // scan or ballot. A run either stays in the current centroid position
+    // (UP on columns, RIGHT on rows), extending its counted block, or leaves
+    // it after its first cell, and then each further cell is a position of
+    // its own with a one-cell block; those records are stored one per lane.
+    // The records are the ones the cell-by-cell walk below writes.
+    if (!REC && rr.gap == 0) {
+      while (i >= rmin && i < top && j < abcol) {
+        i = wave_uniform(i); j = wave_uniform(j); curc = wave_uniform(curc);
+        rr.lo1 = wave_uniform(rr.lo1); rr.hi1 = wave_uniform(rr.hi1);
+        mt.i0 = wave_uniform(mt.i0); mt.cb = wave_uniform(mt.cb);
+        mt.xmin = wave_uniform(mt.xmin); mt.tmax = wave_uniform(mt.tmax);
+        if (!mt.covers(i, j)) mt.fill(g, M, i, j, rmin, lane);
+        const int t = wave_uniform(mt.i0 - i), x = wave_uniform(j - mt.cb);
+        const uint64_t rw = readlane64(mt.rm, t), uw = readlane64(mt.um, t);
+        int L, di, dj;
+        if ((rw >> x) & 1u) {  // RIGHT: scan the row's mask down to the first other move
+          const uint64_t keep = (x >= 63 ? ~0ull : ((2ull << x) - 1ull)) & (~0ull << mt.xmin);
+          const uint64_t stop = ~rw & keep;
+          const int xs = stop ? 63 - __builtin_clzll(stop) : mt.xmin - 1;
+          L = x - xs; di = 0; dj = 1;
+        } else if ((uw >> x) & 1u) {  // UP: first lane below whose cell in column x is not UP
+          const bool st = lane > mt.tmax || !((mt.um >> x) & 1u);
+          const uint64_t s = (uint64_t)__ballot(st) & (~0ull << t);
+          L = (s ? __builtin_ctzll(s) : 64) - t; di = 1; dj = 0;
+        } else {  // DIAG: lane t + s holds the run's cell s at column x - s
+          const int pos = x + t - lane;
+          const bool dg = lane >= t && lane <= mt.tmax && pos >= mt.xmin && pos <= 63 &&
+                          !(((mt.rm | mt.um) >> (pos & 63)) & 1u);
+          const uint64_t s = (uint64_t)__ballot(!dg) & (~0ull << t);
+          L = (s ? __builtin_ctzll(s) : 64) - t; di = 1; dj = 1;
+        }
+        L = wave_uniform(L);
+        const int si = flip ? j : i;  // sequence index of the run's first cell
+        if (flip ? di == 0 : dj == 0) {  // the whole run is in the current position
+          if (rr.hi1 < rr.lo1) rr.hi1 = si;
+          rr.lo1 = si - L + 1;
+        } else {
+          if (rr.hi1 < rr.lo1) rr.hi1 = si;
+          rr.lo1 = si;
+          const int ds = flip ? dj : di;  // sequence-index step along the run
+          if (lane < L)
+            R[curc - lane] = lane == 0 ? rr.out() : make_int4(si - lane * ds, si - lane * ds, 1, 0);
+          rr.reset();
+          curc -= L;
+        }
+        i -= L * di;
+        j -= L * dj;
+        if (j < 0) { e |= 1; break; }
+      }
+      if (e & 1) break;
+      cc.set(j, g.K, g.lastC);
     }
     while (i >= rmin && i < top && j < abcol) {
       // The walk state is the same in every lane; saying so keeps it in
@@ -215,6 +357,60 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
       if (i < 0 || j < 0) { e |= 1; break; }
     }
     if (e & 1) break;
+    // Without a recorded path, stretches of positional moves are skipped in
+    // one go, with the records the cell-by-cell steps below would write:
+    // the open-end top row's OPEN_RIGHT cells up to its first DIAG or UP,
+    // and row 0 from column j to the corner.
+    if (!REC && rr.gap == 0 && i == top && top != INT32_MAX) {
+      const int js = top_run_end(g, TP, min(j, abcol - 1), lane);  // the run is columns j .. js + 1
+      const int L = j - js;
+      if (L > 0 && (flip || j != curc || rr.hi1 < rr.lo1)) {
+        const int c = flip ? i : j;
+        if (c != curc) {
+          if (lane == 0) R[curc] = rr.out();
+          curc = c;
+          rr.reset();
+        }
+        if (flip && rr.hi1 >= rr.lo1) rr.gap = 1;
+        if (!flip) {  // every column of the run is left holding only its uncounted cell
+          for (int b = 0; b < L - 1; b += 64)
+            if (b + lane < L - 1) R[j - b - lane] = make_int4(1, 0, 1, 0);
+          curc = js + 1;
+        }  // on the rows the uncounted cells leave the (empty) block of the top row as it is
+        j = js;
+        if (j < 0) { e |= 1; break; }
+        cc.set(j, g.K, g.lastC);
+      }
+    } else if (!REC && rr.gap == 0 && i == 0 && j > 0 && j < abcol) {
+      const bool cnt = !g.os;  // RIGHT is counted, OPEN_RIGHT is not (dba.hpp:395-404)
+      const int c = flip ? 0 : j;
+      if (c != curc) {
+        if (lane == 0) R[curc] = rr.out();
+        curc = c;
+        rr.reset();
+      }
+      if (flip) {  // columns j .. 1 all feed row 0
+        if (cnt) {
+          if (rr.hi1 < rr.lo1) rr.hi1 = j;
+          rr.lo1 = 1;
+        } else if (rr.hi1 >= rr.lo1) {
+          rr.gap = 1;
+        }
+      } else {  // column j closes with cell (0, j); columns j-1 .. 1 hold one cell each
+        if (cnt) {
+          if (rr.hi1 < rr.lo1) rr.hi1 = 0;
+          rr.lo1 = 0;
+        }
+        const int4 one = cnt ? make_int4(0, 0, 1, 0) : make_int4(1, 0, 1, 0);
+        const int4 first = rr.out();
+        for (int b = 0; b < j; b += 64)
+          if (b + lane < j) R[j - b - lane] = (b + lane == 0) ? first : one;
+        curc = 0;
+        rr.reset();
+      }
+      j = 0;
+      cc.set(0, g.K, g.lastC);
+    }
     uint32_t code;
     if (i == 0 && j == 0) {
       code = g.os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
