@@ -690,47 +690,206 @@ __device__ __forceinline__ T add_run_desc(T a, const T* __restrict__ p, int hi, 
   return a;
 }
 
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Value of v in lane t (wave-uniform t), for 4- and 8-byte types.
+template <typename T> __device__ __forceinline__ T bcast(T v, int t) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), t));
+  } else {
+    return __builtin_bit_cast(T, readlane64(__builtin_bit_cast(uint64_t, v), t));
+  }
+}
+
+// Sum of one centroid position's runs over sequences 0..nseq-1 in order,
+// each run from hi down to lo (runs[s * stride] is sequence s's record).
+// Eight sequences at a time: while their first eight run values are in
+// flight the records and offsets of the next eight are loaded, so a batch
+// costs about one memory latency; only runs longer than eight values and
+// second blocks load more. The additions stay in the reference's order.
 template <typename T>
-__global__ void k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
-                             const int4* __restrict__ runs, int nseq, int lc, T* __restrict__ acc,
-                             uint32_t* __restrict__ counts, int first) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= lc) return;
-  T a = first ? T(0) : acc[c];
-  uint32_t n = first ? 0u : counts[c];
-  // Eight sequences at a time: their run records and the first value of
-  // each run are loaded together; the additions stay in sequence order.
-  constexpr int B = 8;
-  for (int s0 = 0; s0 < nseq; s0 += B) {
+__device__ __forceinline__ void accumulate_runs(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride,
+                                                int nseq, const T* __restrict__ data,
+                                                const long long* __restrict__ offs) {
+  constexpr int B = 8, V = 8;
+  // Every lane of the wave takes part in every batch (the long-run loads
+  // need all 64), up to the wave's largest sequence count.
+  int nmax = nseq;
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+  int4 rn[B];
+  long long on[B];
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    rn[u] = (u < nseq) ? runs[(long long)u * stride] : make_int4(1, 0, 1, 0);
+    on[u] = (u < nseq) ? offs[u] : 0;
+  }
+  for (int s0 = 0; s0 < nmax; s0 += B) {
     int4 r[B];
     const T* p[B];
-    T f1[B], f2[B];
 #pragma unroll
     for (int u = 0; u < B; u++) {
-      r[u] = (s0 + u < nseq) ? runs[(long long)(s0 + u) * lc + c] : make_int4(1, 0, 1, 0);
-      p[u] = data + ((s0 + u < nseq) ? offs[s0 + u] : 0);
+      r[u] = rn[u];
+      p[u] = data + on[u];
     }
+    T v[B][V];
+#pragma unroll
+    for (int u = 0; u < B; u++)
+#pragma unroll
+      for (int k = 0; k < V; k++) v[u][k] = (r[u].y - k >= r[u].x) ? p[u][r[u].y - k] : T(0);
 #pragma unroll
     for (int u = 0; u < B; u++) {
-      f1[u] = (r[u].y >= r[u].x) ? p[u][r[u].y] : T(0);
-      f2[u] = (r[u].w >= r[u].z) ? p[u][r[u].w] : T(0);
+      const int s = s0 + B + u;
+      rn[u] = (s < nseq) ? runs[(long long)s * stride] : make_int4(1, 0, 1, 0);
+      on[u] = (s < nseq) ? offs[s] : 0;
     }
 #pragma unroll
     for (int u = 0; u < B; u++) {
       if (r[u].y >= r[u].x) {
-        a = (T)(a + f1[u]);
-        if (r[u].y > r[u].x) a = add_run_desc<T>(a, p[u], r[u].y - 1, r[u].x);
-        n += (uint32_t)(r[u].y - r[u].x + 1);
+#pragma unroll
+        for (int k = 0; k < V; k++)
+          if (r[u].y - k >= r[u].x) a = (T)(a + v[u][k]);
+        if (r[u].y - V >= r[u].x) a = add_run_desc<T>(a, p[u], r[u].y - V, r[u].x);
       }
+      if (r[u].y >= r[u].x) n += (uint32_t)(r[u].y - r[u].x + 1);
       if (r[u].w >= r[u].z) {
-        a = (T)(a + f2[u]);
-        if (r[u].w > r[u].z) a = add_run_desc<T>(a, p[u], r[u].w - 1, r[u].z);
+        a = add_run_desc<T>(a, p[u], r[u].w, r[u].z);
         n += (uint32_t)(r[u].w - r[u].z + 1);
       }
     }
   }
-  acc[c] = a;
-  counts[c] = n;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+constexpr int kStageB = 8;    // sequences per batch of the staged accumulation
+constexpr int kStageW = 256;  // values per staged window (4 per lane)
+
+// Sum of the runs of a wave's 64 consecutive centroid positions, one lane
+// per position, over sequences 0..nall-1 of one cluster (runs[s * stride]
+// is this lane's record of sequence s; lanes with nseq == 0 have none).
+// Along a monotone path the positions' runs tile one contiguous range of
+// each sequence, so the wave loads that range with all 64 lanes into LDS,
+// 256 values per window, and each lane then adds its own run from LDS, hi
+// down to lo, in sequence order: the reference's order of additions. The
+// first window of eight sequences is loaded at once (one memory latency
+// per batch); longer ranges load further windows as they are reached.
+// buf holds (kStageB + 1) * kStageW values; every lane of the block's one
+// wave must call this.
+template <typename T>
+__device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride, int nseq,
+                                  int nall, const T* __restrict__ data, const long long* __restrict__ offs,
+                                  T* __restrict__ buf) {
+  constexpr int B = kStageB, SW = kStageW;
+  const int lane = lane_id();
+  int4 rn[B];
+  long long on[B];
+#pragma unroll
+  for (int u = 0; u < B; u++) {
+    rn[u] = (u < nseq) ? runs[(long long)u * stride] : make_int4(1, 0, 1, 0);
+    on[u] = (u < nall) ? offs[u] : 0;
+  }
+  for (int s0 = 0; s0 < nall; s0 += B) {
+    int4 r[B];
+    long long o[B];
+    int lo[B], hi[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      r[u] = rn[u];
+      o[u] = on[u];
+      const bool ne = r[u].y >= r[u].x;
+      lo[u] = wave_min_i(ne ? r[u].x : INT32_MAX);
+      hi[u] = wave_max_i(ne ? r[u].y : INT32_MIN);
+    }
+    T w[B][4];
+#pragma unroll
+    for (int u = 0; u < B; u++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int idx = hi[u] - (lane * 4 + e);
+        w[u][e] = (hi[u] >= lo[u] && idx >= lo[u]) ? data[o[u] + idx] : T(0);
+      }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int sn = s0 + B + u;
+      rn[u] = (sn < nseq) ? runs[(long long)sn * stride] : make_int4(1, 0, 1, 0);
+      on[u] = (sn < nall) ? offs[sn] : 0;
+    }
+    __syncthreads();  // the previous batch's reads of buf are done
+#pragma unroll
+    for (int u = 0; u < B; u++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) buf[u * SW + lane * 4 + e] = w[u][e];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if (hi[u] >= lo[u]) {  // wave-uniform
+        int top = hi[u];
+        const T* wb = buf + u * SW;  // wb[t] = value top - t
+        for (;;) {
+          const int wlo = max(lo[u], top - SW + 1);
+          if (r[u].y >= r[u].x) {
+            // Eight LDS reads in flight before their eight (dependent) adds:
+            // a run can be tens of thousands of values (a path along row 0),
+            // and its adds are the kernel's longest chain.
+            const int k1 = min(r[u].y, top), k0 = max(r[u].x, wlo);
+            const T* q = wb + (top - k1);
+            int m = k1 - k0 + 1;
+            for (; m >= 8; m -= 8, q += 8) {
+              T v[8];
+#pragma unroll
+              for (int e = 0; e < 8; e++) v[e] = q[e];
+#pragma unroll
+              for (int e = 0; e < 8; e++) a = (T)(a + v[e]);
+            }
+            for (; m > 0; m--, q++) a = (T)(a + *q);
+          }
+          top = wlo - 1;
+          if (top < lo[u]) break;
+          T x[4];
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const int idx = top - (lane * 4 + e);
+            x[e] = idx >= lo[u] ? data[o[u] + idx] : T(0);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < 4; e++) buf[B * SW + lane * 4 + e] = x[e];
+          __syncthreads();
+          wb = buf + B * SW;
+        }
+      }
+      if (r[u].y >= r[u].x) n += (uint32_t)(r[u].y - r[u].x + 1);
+      if (r[u].w >= r[u].z) {  // second block: one cell of a height-cut stripe row
+        a = add_run_desc<T>(a, data + o[u], r[u].w, r[u].z);
+        n += (uint32_t)(r[u].w - r[u].z + 1);
+      }
+    }
+  }
+}
+
+// One 64-lane block per 64 centroid positions (no early exit: lanes past lc
+// take part with no records).
+template <typename T>
+__global__ void __launch_bounds__(64) k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
+                                                   const int4* __restrict__ runs, int nseq, int lc,
+                                                   T* __restrict__ acc, uint32_t* __restrict__ counts, int first) {
+  __shared__ T buf[(kStageB + 1) * kStageW];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = c < lc;
+  T a = (first || !on) ? T(0) : acc[c];
+  uint32_t n = (first || !on) ? 0u : counts[c];
+  accumulate_staged<T>(a, n, runs + (on ? c : 0), lc, on ? nseq : 0, nseq, data, offs, buf);
+  if (on) {
+    acc[c] = a;
+    counts[c] = n;
+  }
 }
 
 // mean = sum / count with the host's T / unsigned arithmetic (dba.hpp:820-822)
@@ -787,30 +946,28 @@ __device__ __forceinline__ int slot_of(const MultiSlot* __restrict__ sl, int ns,
   return lo;
 }
 
+// One 64-lane block per 64 global positions: a block inside one cluster
+// takes the staged path, one that straddles clusters adds lane by lane.
 template <typename T>
-__global__ void k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total, T* __restrict__ acc,
-                                   uint32_t* __restrict__ counts) {
+__global__ void __launch_bounds__(64) k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total,
+                                                         T* __restrict__ acc, uint32_t* __restrict__ counts) {
+  __shared__ T buf[(kStageB + 1) * kStageW];
+  // No early exit: lanes past total take part with no sequences.
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const MultiSlot S = sl[slot_of<T>(sl, ns, g)];
-  const int c = (int)(g - S.pos0);
-  const T* data = (const T*)S.data;
+  const bool on = g < total;
+  const int si = slot_of<T>(sl, ns, on ? g : total - 1);
+  const MultiSlot S = sl[si];
+  const int c = on ? (int)(g - S.pos0) : 0;
   T a = T(0);
   uint32_t n = 0;
-  for (int s = 0; s < S.nmem; s++) {
-    const int4 r = S.runs[(long long)s * S.lc + c];
-    const T* p = data + S.moffs[s];
-    if (r.y >= r.x) {
-      a = add_run_desc<T>(a, p, r.y, r.x);
-      n += (uint32_t)(r.y - r.x + 1);
-    }
-    if (r.w >= r.z) {
-      a = add_run_desc<T>(a, p, r.w, r.z);
-      n += (uint32_t)(r.w - r.z + 1);
-    }
+  if (wave_min_i(si) == wave_max_i(si))
+    accumulate_staged<T>(a, n, S.runs + c, S.lc, on ? S.nmem : 0, S.nmem, (const T*)S.data, S.moffs, buf);
+  else
+    accumulate_runs<T>(a, n, S.runs + c, S.lc, on ? S.nmem : 0, (const T*)S.data, S.moffs);
+  if (on) {
+    acc[g] = a;
+    counts[g] = n;
   }
-  acc[g] = a;
-  counts[g] = n;
 }
 
 template <typename T>
@@ -961,7 +1118,7 @@ hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs,
 template <typename T>
 static hipError_t acc_t(hipStream_t st, const void* data, const long long* offs, const int4* runs, int nseq, int lc,
                         void* acc, uint32_t* counts, int first) {
-  hipLaunchKernelGGL(k_accumulate<T>, dim3((lc + 255) / 256), dim3(256), 0, st, (const T*)data, offs, runs, nseq, lc,
+  hipLaunchKernelGGL(k_accumulate<T>, dim3((lc + 63) / 64), dim3(64), 0, st, (const T*)data, offs, runs, nseq, lc,
                      (T*)acc, counts, first);
   return hipGetLastError();
 }
@@ -990,7 +1147,8 @@ template <typename T>
 static hipError_t multi_t(hipStream_t st, const MultiSlot* sl, int ns, long long total, void* acc, uint32_t* counts,
                           unsigned long long* delta_bits, int* err) {
   const unsigned nb = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(k_accumulate_multi<T>, dim3(nb), dim3(256), 0, st, sl, ns, total, (T*)acc, counts);
+  hipLaunchKernelGGL(k_accumulate_multi<T>, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, st, sl, ns, total,
+                     (T*)acc, counts);
   hipLaunchKernelGGL(k_mean_delta_multi<T>, dim3(nb), dim3(256), 0, st, (const T*)acc, counts, sl, ns, total,
                      delta_bits, err);
   return hipGetLastError();
