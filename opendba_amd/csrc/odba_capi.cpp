@@ -624,19 +624,25 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
   HIPC(c, launch_fill_u32(c->stream, counter, 0u, 1));  // a kernel: this may run inside a graph capture
+  // Few jobs: a workgroup per job, one wave per swath, or (pairs of at most
+  // four swaths) one wave per quarter swath.
   const char* env = std::getenv("ODBA_PATH_SCHEDULE");
-  bool strips = njobs < 4 * c->cus && maxlx >= 2 * ODBA_W && minly >= 64;
+  const int kmax = (maxlx + ODBA_W - 1) / ODBA_W;
+  bool strips = njobs < 4 * c->cus && maxlx > 256 && minly >= 64;
+  bool quarters = kmax <= 4;
   if (env && !std::strcmp(env, "wave")) strips = false;
-  if (env && !std::strcmp(env, "strip")) strips = maxlx > 0 && minly >= 64;
+  if (env && !std::strcmp(env, "strip")) strips = maxlx > 0 && minly >= 64, quarters = false;
+  if (env && !std::strcmp(env, "quarter")) strips = maxlx > 0 && minly >= 64, quarters = true;
   strips = strips && dtype == ODBA_F32;
   hipError_t e;
   if (strips) {
-    const int waves = std::min(16, (maxlx + ODBA_W - 1) / ODBA_W);
+    const int waves = std::min(16, quarters ? 4 * kmax : kmax);
     const int per_cu = std::max(1, 16 / waves);
     const int nblocks = std::max(1, std::min(njobs, per_cu * c->cus));
     int rc = ensure_scratch(c, (size_t)nblocks * 2 * waves * stride * es);
     if (rc) return rc;
-    e = launch_path_strips(waves, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr, c->esc, stride);
+    e = launch_path_strips(waves, os, oe, nblocks, c->stream, jobs_dev, njobs, counter, mv, mtop, abr, c->esc, stride,
+                           quarters ? 1 : 0);
   } else {
     const int nblocks = grid_blocks(c, njobs);
     int rc = ensure_scratch(c, (size_t)nblocks * 4 * 2 * stride * es);

@@ -92,7 +92,7 @@ size_t band_pair_size();
 // Float path jobs with a workgroup of `waves` waves per job (strips = swaths).
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
-                              long long stride);
+                              long long stride, int quarters);
 
 // Checkpointed global float paths (see k_ckpt_forward in the swath kernels):
 // jobs are CkJob records (ckpt_job_size() bytes), states CkState records

@@ -234,6 +234,7 @@ template <typename T> struct SwathIO {
   int ref_left;    // 1: `off` is a reference swath start (a multiple of 1024)
   int imask = -1;  // row index mask of ein: -1 (whole column) or ring rows - 1
   int omask = -1;  // row index mask of eout
+  int lbase = 0;   // quarter strips: word-lane of this wave's lane 0 in the swath's word layout
   // Checkpointed global paths: `top` (when set) is the cost row above the
   // first row, indexed by column (top[off-1] included when off > 0); with
   // the CKPT template flag, the rows whose global index (row + ck_rbase) is
@@ -265,7 +266,19 @@ template <typename T> struct SwathRes {
 
 // Runs one swath of C columns per lane. Rows r0..r1 go through the skewed
 // main loop; with open_end the top row goes through the prefix-min scan.
-template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH, class Pipe = NoPipe, bool CKPT = false>
+// Lane i receives v of lane i - K within its row of 16 lanes (DPP row_shr).
+template <int K> __device__ __forceinline__ uint32_t row_shr_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + K, 0xF, 0xF, false);
+}
+
+// QW (quarter strips): the wave covers a quarter of a swath, C = C_swath / 4
+// columns per lane, so four consecutive lanes share one step word of the
+// swath's layout (word-lane a.lbase + lane / 4). Lane 4m+k computes row r
+// k steps after lane 4m, so the fourth lane of each group ORs the three
+// earlier fragments of row r (kept for 1..3 steps, fetched with DPP) into
+// its own and stores the whole word where the one-wave swath would.
+template <typename T, int C, bool OS, bool OE, bool MINFIRST, bool PATH, class Pipe = NoPipe, bool CKPT = false,
+          bool QW = false>
 __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int lane, const Pipe& pipe = Pipe()) {
   const T MAXV = Ar<T>::maxv();
   const T RINF = Ar<T>::rowinf();
@@ -312,6 +325,20 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
 
   SwathRes<T> res;
   T runmin = MAXV;
+  uint32_t h1 = 0, h2 = 0, h3 = 0;  // QW: this lane's fragments 1, 2, 3 steps back
+  auto put_word = [&](int s, uint32_t word, bool act) {
+    if constexpr (QW) {
+      const uint32_t f1 = row_shr_u32<1>(h1), f2 = row_shr_u32<2>(h2), f3 = row_shr_u32<3>(h3);
+      const uint32_t full = f3 | (f2 << (2 * C)) | (f1 << (4 * C)) | (word << (6 * C));
+      h3 = h2;
+      h2 = h1;
+      h1 = word;
+      const int l = a.lbase + (lane >> 2);
+      if ((lane & 3) == 3 && act) a.mv[(size_t)(s - lane + l) * 64 + l] = full;
+    } else {
+      if (act) a.mv[(size_t)s * 64 + lane] = word;
+    }
+  };
 
   // prefetch block 0 inputs
   auto load_y = [&](int b) -> T { return a.y[r0 + 64 * b + lane]; };
@@ -387,7 +414,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           D[q] = v;
           l = v;
         }
-        if (PATH) a.mv[(size_t)s * 64 + lane] = word;
+        if (PATH) put_word(s, word, true);
         if (CKPT) ckpt(r0 + s - lane, true);
         W->o[u - lane + 64] = D[C - 1];
       }
@@ -425,7 +452,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           D[q] = active ? v : up;
           l = v;
         }
-        if (PATH && active) a.mv[(size_t)s * 64 + lane] = word;
+        if (PATH) put_word(s, word, active);
         if (CKPT) ckpt(row, active);
         W->o[u - lane + 64] = D[C - 1];
       }
@@ -501,7 +528,15 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
       dprev_old = D[q];
       D[q] = v;
     }
-    if (PATH) a.mtop[lane] = word;
+    if (PATH) {
+      if constexpr (QW) {
+        const uint32_t full = row_shr_u32<3>(word) | (row_shr_u32<2>(word) << (2 * C)) |
+                              (row_shr_u32<1>(word) << (4 * C)) | (word << (6 * C));
+        if ((lane & 3) == 3) a.mtop[a.lbase + (lane >> 2)] = full;
+      } else {
+        a.mtop[lane] = word;
+      }
+    }
     if (edge_out) {
       T top = __shfl(D[C - 1], 63, 64);
       pipe.reserve(ly);
@@ -578,6 +613,19 @@ __device__ SwathRes<T> swath_dispatch_pipe(int C, const SwathIO<T>& io, const Wa
     case 8: return run_swath<T, 8, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
     case 12: return run_swath<T, 12, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
     default: return run_swath<T, 16, OS, OE, true, PATH, Pipe>(io, W, lane, pp);
+  }
+}
+
+// Quarter strips of a path swath: C = 1, 2, 3, 4 columns per lane (a
+// quarter of a swath of 4, 8, 12, 16).
+template <typename T, bool OS, bool OE, class Pipe>
+__device__ SwathRes<T> swath_dispatch_quarter(int C, const SwathIO<T>& io, const WaveWin<T>* W, int lane,
+                                              const Pipe& pp) {
+  switch (C) {
+    case 1: return run_swath<T, 1, OS, OE, true, true, Pipe, false, true>(io, W, lane, pp);
+    case 2: return run_swath<T, 2, OS, OE, true, true, Pipe, false, true>(io, W, lane, pp);
+    case 3: return run_swath<T, 3, OS, OE, true, true, Pipe, false, true>(io, W, lane, pp);
+    default: return run_swath<T, 4, OS, OE, true, true, Pipe, false, true>(io, W, lane, pp);
   }
 }
 
@@ -1061,7 +1109,11 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
 // the stop are written but never read: the traceback treats those cells
 // positionally).
 // ---------------------------------------------------------------------------
-template <typename T, bool OS, bool OE>
+// With QW every swath of 4+ columns per lane is split into four quarter
+// strips (C/4 columns per lane, one wave each), so a pair of one to four
+// swaths (a long sequence against a short centroid) still spreads over four
+// times as many waves; the step words land in the same layout.
+template <typename T, bool OS, bool OE, bool QW>
 __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restrict__ jobs, int njobs,
                                                       unsigned int* __restrict__ counter, uint32_t* __restrict__ mv,
                                                       uint32_t* __restrict__ mtop, int* __restrict__ abrogated,
@@ -1099,10 +1151,16 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
     const int K = (lx + ODBA_W - 1) / ODBA_W;
     const int nrows_main = ((OE && ly >= 2) ? ly - 2 : ly - 1) - (OS ? 1 : 0) + 1;
     const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
-    for (int s = wid; s < K; s += nw) {
-      const int off = s * ODBA_W;
+    const int Cl = last_swath_c(lx - (K - 1) * ODBA_W);
+    const int qlast = (QW && Cl >= 4) ? 4 : 1;  // strips of the last swath
+    const int NS = QW ? 4 * (K - 1) + qlast : K;
+    for (int s = wid; s < NS; s += nw) {
+      const int k = QW ? (s >> 2) : s, q = QW ? (s & 3) : 0;
+      const bool split = QW && (k < K - 1 || qlast == 4);
+      const int C = (k < K - 1) ? (split ? 4 : 16) : (split ? Cl / 4 : Cl);
+      const int off = k * ODBA_W + q * 64 * C;
       const int so = s % nslot, si = (s + nslot - 1) % nslot;
-      const bool has_out = (s < K - 1);
+      const bool has_out = (s < NS - 1);
       if (has_out) {
         for (;;) {
           const int f = uniform_i32(__hip_atomic_load(&s_free[so], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
@@ -1120,16 +1178,20 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
       io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = off;
       io.ein = s > 0 ? ebase + (long long)si * stride : nullptr;
       io.eout = has_out ? ebase + (long long)so * stride : nullptr;
-      io.mv = mv + jb.mv_off + (size_t)s * swath_words;
-      io.mtop = mtop + jb.top_off + (size_t)s * 64;
-      io.ref_left = 1;
-      const int C = has_out ? 16 : last_swath_c(lx - off);
-      SwathRes<T> res = swath_dispatch_pipe<T, OS, OE, true, StripPipe>(C, io, &win, lane, pp);
+      io.mv = mv + jb.mv_off + (size_t)k * swath_words;
+      io.mtop = mtop + jb.top_off + (size_t)k * 64;
+      io.ref_left = (q == 0);
+      io.lbase = q * 16;
+      SwathRes<T> res = split ? swath_dispatch_quarter<T, OS, OE, StripPipe>(C, io, &win, lane, pp)
+                              : swath_dispatch_pipe<T, OS, OE, true, StripPipe>(C, io, &win, lane, pp);
       if (s > 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         if (lane == 0) __hip_atomic_store(&s_free[si], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      if (OE && !OS && has_out && off + ODBA_W > ly && res.top == res.colmin && lane == 0) atomicMin(&s_abr, s + 1);
+      // the open-end stop is decided on a swath's right edge column (dtw.hpp:74-129)
+      const bool swath_end = !split || q == 3;
+      if (OE && !OS && has_out && swath_end && k * ODBA_W + ODBA_W > ly && res.top == res.colmin && lane == 0)
+        atomicMin(&s_abr, k + 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) abrogated[jix] = (s_abr != INT_MAX) ? s_abr : -1;
@@ -1566,13 +1628,17 @@ size_t ckpt_state_size() { return sizeof(CkState); }
 
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
-                              long long stride) {
+                              long long stride, int quarters) {
   if (waves < 1 || waves > kMaxStripWaves) return hipErrorInvalidValue;
   dim3 g(nblocks), b(64 * waves);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
   float* E = (float*)esc;
-#define ODBA_PSTRIPS(A, B) hipLaunchKernelGGL((k_path_strips<float, A, B>), g, b, 0, st, J, njobs, counter, mv, mtop, abr, E, stride)
-  if (os && oe) ODBA_PSTRIPS(true, true); else if (os) ODBA_PSTRIPS(true, false); else if (oe) ODBA_PSTRIPS(false, true); else ODBA_PSTRIPS(false, false);
+#define ODBA_PSTRIPS(A, B, Q) hipLaunchKernelGGL((k_path_strips<float, A, B, Q>), g, b, 0, st, J, njobs, counter, mv, mtop, abr, E, stride)
+  if (quarters) {
+    if (os && oe) ODBA_PSTRIPS(true, true, true); else if (os) ODBA_PSTRIPS(true, false, true); else if (oe) ODBA_PSTRIPS(false, true, true); else ODBA_PSTRIPS(false, false, true);
+  } else {
+    if (os && oe) ODBA_PSTRIPS(true, true, false); else if (os) ODBA_PSTRIPS(true, false, false); else if (oe) ODBA_PSTRIPS(false, true, false); else ODBA_PSTRIPS(false, false, false);
+  }
 #undef ODBA_PSTRIPS
   return hipGetLastError();
 }

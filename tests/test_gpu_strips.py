@@ -174,14 +174,18 @@ def path_schedule(name):
 
 
 @pytest.mark.parametrize("os_,oe", MODES)
-@pytest.mark.parametrize("ly,lx", [(300, 2100), (2500, 4100), (1030, 1025), (5000, 3300)])
-def test_path_strips_step_matrix(ctx, os_, oe, ly, lx):
-    """Path jobs swept by a workgroup of waves (one 1024-column swath each)
-    give the oracle's step matrix, open-end stop included."""
+@pytest.mark.parametrize("ly,lx", [(300, 2100), (2500, 4100), (1030, 1025), (5000, 3300), (90, 200), (1500, 400),
+                                   (300, 700), (2000, 1000)])
+@pytest.mark.parametrize("sched", ["strip", "quarter"])
+def test_path_strips_step_matrix(ctx, os_, oe, ly, lx, sched):
+    """Path jobs swept by a workgroup of waves (one 1024-column swath each,
+    or one quarter swath each, whose four waves share the step words) give
+    the oracle's step matrix, open-end stop included. The widths cover last
+    swaths of 1 (unsplit), 4, 8, 12 and 16 columns per lane."""
     rng = np.random.default_rng(ly + lx + 2 * os_ + oe)
     y, x = O.znorm(_walks(rng, [ly, lx]))[0]
-    print(f"path strips os={os_} oe={oe} {ly}x{lx}", flush=True)
-    with path_schedule("strip"):
+    print(f"path {sched}s os={os_} oe={oe} {ly}x{lx}", flush=True)
+    with path_schedule(sched):
         d, M = ctx.dtw_path(y, x, os_, oe)
     ed, EM, _, _ = O.dtw_pair(y, x, os_=os_, oe=oe, moves=True)
     assert np.array_equal(M, EM)
@@ -189,12 +193,13 @@ def test_path_strips_step_matrix(ctx, os_, oe, ly, lx):
 
 
 @pytest.mark.parametrize("os_,oe", MODES)
-def test_path_strips_dba_round(ctx, os_, oe):
+@pytest.mark.parametrize("sched", ["strip", "quarter"])
+def test_path_strips_dba_round(ctx, os_, oe, sched):
     rng = np.random.default_rng(70 + 2 * os_ + oe)
     seqs = O.znorm(_walks(rng, [2100, 3600, 1500, 4200, 2900]))[0]
     cen = seqs[1].copy()
-    print(f"dba round path strips os={os_} oe={oe}", flush=True)
-    with path_schedule("strip"):
+    print(f"dba round path {sched}s os={os_} oe={oe}", flush=True)
+    with path_schedule(sched):
         got, counts, delta = ctx.dba_round(cen, seqs, os_, oe)
     rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_=os_, oe=oe)
     assert rc == 0
