@@ -619,23 +619,51 @@ void plan_job(int ly, int lx, int os, int oe, long long& mv_words, long long& to
 // waves per job sweeping its swaths as a pipeline. ODBA_PATH_SCHEDULE =
 // wave|strip overrides the choice.
 int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
-                  uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0) {
+                  uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0, int* err = nullptr) {
   const size_t es = dtype_size(dtype);
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
   HIPC(c, launch_fill_u32(c->stream, counter, 0u, 1));  // a kernel: this may run inside a graph capture
-  // Few jobs: a workgroup per job, one wave per swath, or (pairs of at most
-  // four swaths) one wave per quarter swath.
+  // Few jobs (float): a workgroup per job, one wave per swath or (pairs of
+  // at most four swaths) one wave per quarter swath.
+  // ODBA_PATH_SCHEDULE = wave|strip|quarter.
+  // The band schedule (launch_path_bands: a job's swaths on several
+  // workgroups with global edge slots) is switched off: its first open-end
+  // run on the device made an illegal memory access whose cause has not
+  // been found, so nothing selects it and asking for it is an error.
   const char* env = std::getenv("ODBA_PATH_SCHEDULE");
   const int kmax = (maxlx + ODBA_W - 1) / ODBA_W;
   bool strips = njobs < 4 * c->cus && maxlx > 256 && minly >= 64;
   bool quarters = kmax <= 4;
+  const bool bands = false;
+  if (env && !std::strcmp(env, "band"))
+    return fail(c, ODBA_E_INVALID, "the path band schedule is switched off (an open-end run faulted; cause not found)");
   if (env && !std::strcmp(env, "wave")) strips = false;
   if (env && !std::strcmp(env, "strip")) strips = maxlx > 0 && minly >= 64, quarters = false;
   if (env && !std::strcmp(env, "quarter")) strips = maxlx > 0 && minly >= 64, quarters = true;
   strips = strips && dtype == ODBA_F32;
+  (void)err;
   hipError_t e;
-  if (strips) {
+  if (bands) {
+    const long long max_claims = (long long)njobs * std::max(1, kmax);
+    const int nblocks = (int)std::max(1LL, std::min(max_claims, 4LL * c->cus));
+    const int R = 2 * nblocks + 2;
+    const int ring = std::max(128, env_int("ODBA_BAND_RING", 2048));
+    const long long gstride = maxly + 256;
+    const size_t ring_bytes = (size_t)nblocks * 4 * ring * es, slot_bytes = (size_t)R * gstride * es;
+    const size_t sync_bytes = path_band_sync_bytes(R, max_claims, njobs);
+    int rc = ensure_scratch(c, ring_bytes + slot_bytes + sync_bytes + 256);
+    if (rc) return rc;
+    char* base = (char*)c->esc;
+    void* sync = (void*)(((uintptr_t)(base + ring_bytes + slot_bytes) + 255) & ~(uintptr_t)255);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+    const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;
+    unsigned long long* counter64 = (unsigned long long*)((char*)c->small + 64);
+    HIPC(c, launch_fill_u32(c->stream, (unsigned int*)counter64, 0u, 2));
+    e = launch_path_bands(os, oe, nblocks, c->stream, jobs_dev, njobs, max_claims, counter64, mv, mtop, abr, base, ring,
+                          base + ring_bytes, gstride, sync, R, err, limit);
+  } else if (strips) {
     const int waves = std::min(16, quarters ? 4 * kmax : kmax);
     const int per_cu = std::max(1, 16 / waves);
     const int nblocks = std::max(1, std::min(njobs, per_cu * c->cus));
@@ -673,22 +701,24 @@ int dtw_path_t(odba_ctx* c, const T* y, int ly, const T* x, int lx, int os, int 
   HIPC(c, hipMalloc(&jd, sizeof(job)));
   HIPC(c, hipMalloc((void**)&mv, std::max(1LL, mvw) * 4));
   HIPC(c, hipMalloc((void**)&mtop, std::max(1LL, tw) * 4));
-  HIPC(c, hipMalloc((void**)&abr, sizeof(int)));
+  HIPC(c, hipMalloc((void**)&abr, 2 * sizeof(int)));  // [abrogated swath, band hand-off error]
+  HIPC(c, hipMemsetAsync(abr + 1, 0, sizeof(int), c->stream));
   HIPC(c, hipMemcpyAsync(jd, &job, sizeof(job), hipMemcpyHostToDevice, c->stream));
   std::vector<const T*> sp{y, x};
   std::vector<uint64_t> sl{(uint64_t)ly, (uint64_t)lx};
   bool minfirst = int_cannot_wrap<T>(sp.data(), sl.data(), 2);
   int rc = run_path_jobs(c, std::is_same<T, float>::value ? ODBA_F32 : std::is_same<T, double>::value ? ODBA_F64
                               : std::is_same<T, int32_t>::value ? ODBA_I32 : std::is_same<T, uint32_t>::value ? ODBA_U32 : ODBA_U64,
-                         minfirst, os, oe, jd, 1, ly, mv, mtop, abr, lx, ly);
+                         minfirst, os, oe, jd, 1, ly, mv, mtop, abr, lx, ly, abr + 1);
   if (rc) return rc;
-  int ab = -1;
-  HIPC(c, hipMemcpyAsync(&ab, abr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int ab[2] = {-1, 0};
+  HIPC(c, hipMemcpyAsync(ab, abr, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  if (ab[1] & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
   if (moves_out) {
     uint8_t* md = nullptr;
     HIPC(c, hipMalloc((void**)&md, (size_t)ly * lx));
-    HIPC(c, launch_decode_moves(c->stream, mv, mtop, ab, ly, lx, os, oe, md));
+    HIPC(c, launch_decode_moves(c->stream, mv, mtop, ab[0], ly, lx, os, oe, md));
     HIPC(c, hipMemcpyAsync(moves_out, md, (size_t)ly * lx, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     hipFree(md);
@@ -1165,7 +1195,7 @@ template <typename T> struct DbaState {
         if (rc) return rc;
         HIPC(c, hipMemcpyAsync(jobs, pj.data(), nfull * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
         HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
-        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly);
+        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly, err);
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
@@ -1183,6 +1213,7 @@ template <typename T> struct DbaState {
     HIPC(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipMemcpyAsync(&bits, dbits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
+    if (herr & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
     if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
     if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
     if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
@@ -1241,7 +1272,7 @@ template <typename T> struct DbaState {
       hipError_t e = launch_fill_u32(c->stream, err, 0u, 1);
       if (e == hipSuccess) e = launch_fill_u32(c->stream, dbits, 0u, 2);
       if (e != hipSuccess) rc = fail(c, ODBA_E_HIP, "graph capture: %s", hipGetErrorString(e));
-      if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly);
+      if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly, err);
       if (!rc) {
         e = launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, lc, os, oe, err, nullptr);
         if (e == hipSuccess) e = launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs, runs, n, lc, acc, counts, 1);
@@ -1265,6 +1296,7 @@ template <typename T> struct DbaState {
     HIPC(c, hipGraphLaunch(gexec[p], c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     const int herr = *h_err;
+    if (herr & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
     if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
     if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
     if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
@@ -1648,7 +1680,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
       HIPC(c, hipMemcpyAsync(dslots, slots.data(), slots.size() * sizeof(MultiSlot), hipMemcpyHostToDevice, c->stream));
       HIPC(c, hipMemsetAsync(dbits, 0, nc * sizeof(unsigned long long), c->stream));
       HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
-      rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly);
+      rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly, derr);
       if (rc) return rc;
       HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, 0, os, oe, derr, nullptr));
       HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, dslots, (int)slots.size(), pos, acc, counts, dbits, derr));
@@ -1660,7 +1692,8 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
         HIPC(c, hipMemcpyAsync(nb[k].data(), newc + cen_off[k], lcs[k] * sizeof(T), hipMemcpyDeviceToHost, c->stream));
       }
       HIPC(c, hipStreamSynchronize(c->stream));
-      if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+      if (herr & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
+    if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
       if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
       if (herr & 2) return fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
       for (size_t q = a0; q < a1; q++) {  // dba.hpp:1084-1110, per cluster
@@ -1788,13 +1821,14 @@ int prefix_chop_t(odba_ctx* c, const T* prefix_in, int P, T* const* seqs, const 
     HIPC(c, hipMemcpyAsync(jobs, pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(tbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
-    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P);
+    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P, derr);
     if (rc) return rc;
     HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, P, 0, 1, derr, nullptr));
     int herr = 0;
     HIPC(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipMemcpyAsync(&hr[(size_t)b0 * P], runs, (size_t)nj * P * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
+    if (herr & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
     if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
     if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one prefix row");
     b0 = b1;

@@ -90,6 +90,13 @@ size_t band_sync_bytes(int R, long long nclaims, int ntab);
 size_t band_pair_size();
 
 // Float path jobs with a workgroup of `waves` waves per job (strips = swaths).
+// Path jobs as bands of quarter-swath strips over the whole device (float);
+// njobs <= 1024, max_claims >= total swaths; timeouts set err bit 8.
+size_t path_band_sync_bytes(int R, long long max_claims, int njobs);
+hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                             long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
+                             int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
+                             int* err, long long tick_limit);
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                               long long stride, int quarters);

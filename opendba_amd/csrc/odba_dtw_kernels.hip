@@ -1199,6 +1199,148 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Path jobs as bands (float): one band per 1024-column swath of a job, swept
+// by a workgroup of four waves, one per quarter swath (or wave 0 alone for
+// a last swath of 1 or 2 columns per lane), so the swaths of one job run on
+// as many CUs as it has swaths. Claims go in job order from the last job to
+// the first (callers list jobs by ascending length: the longest start first)
+// and swath order within a job; inside a band the edge column passes
+// through LDS-counted rings, between bands through whole-column global slots
+// from a free pool, exactly as in k_all_pairs_bands, so every wait targets a
+// lower claim held by a running workgroup and times out into err bit 8. The
+// step words land in the one-wave layout; a swath's last quarter decides the
+// open-end stop (dtw.hpp:74-129) into abr[job] (INT_MAX: none).
+// band0[p] = first claim of the p-th job in claim order; band0[njobs] = all.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
+                                                     long long* __restrict__ band0) {
+  __shared__ long long s[1024];
+  const int t = threadIdx.x;
+  const long long v = (t < njobs) ? (long long)((jobs[njobs - 1 - t].lx + ODBA_W - 1) / ODBA_W) : 0;
+  s[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const long long a = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  if (t < njobs) band0[t] = s[t] - v;
+  if (t == njobs - 1) band0[njobs] = s[t];
+}
+
+__global__ void k_abr_index(int* __restrict__ abr, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && abr[i] == INT_MAX) abr[i] = -1;
+}
+
+template <bool OS, bool OE>
+__global__ void __launch_bounds__(256) k_path_bands(const PathJob<float>* __restrict__ jobs, int njobs,
+                                                    const long long* __restrict__ band0,
+                                                    unsigned long long* __restrict__ counter,
+                                                    uint32_t* __restrict__ mv, uint32_t* __restrict__ mtop,
+                                                    float* __restrict__ rings, int ring, float* __restrict__ gslots,
+                                                    long long gstride, int* __restrict__ g_cnt,
+                                                    unsigned long long* __restrict__ g_free, int R,
+                                                    int* __restrict__ slot_of, int* __restrict__ abr,
+                                                    int* __restrict__ err, long long limit) {
+  using T = float;
+  constexpr int NW = 4;
+  __shared__ T s_y[NW][128];
+  __shared__ T s_e[NW][64];
+  __shared__ T s_o[NW][128];
+  __shared__ int s_cnt[NW];   // rows published into ring w
+  __shared__ int s_used[NW];  // rows of ring w read by wave w+1
+  __shared__ unsigned long long s_q;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  T* rbase = rings + (long long)blockIdx.x * NW * ring;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  const long long nclaims = band0[njobs];
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous claim
+    if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
+    if (threadIdx.x < NW) {
+      s_cnt[threadIdx.x] = 0;
+      s_used[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const long long q = (long long)uniform_u64(s_q);
+    if (q >= nclaims) break;
+    int lo = 0, hi = njobs - 1;  // claim-order position: last with band0 <= q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (band0[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const int pos = uniform_i32(lo);
+    const int j = njobs - 1 - pos;
+    const PathJob<T> jb = jobs[j];
+    const int ly = jb.ly, lx = jb.lx;
+    const int K = (lx + ODBA_W - 1) / ODBA_W;
+    const int k = uniform_i32((int)(q - band0[pos]));  // this band's swath
+    const int Cl = last_swath_c(lx - (K - 1) * ODBA_W);
+    const bool split = (k < K - 1) || Cl >= 4;
+    const int C = (k < K - 1) ? 4 : (split ? Cl / 4 : Cl);
+    const int nstr = split ? 4 : 1;
+    if (wid >= nstr) continue;  // a last swath of 1 or 2 columns per lane: wave 0 alone
+    const int off = k * ODBA_W + wid * 64 * C;
+    const bool has_out = !(k == K - 1 && wid == nstr - 1);
+    const bool in_global = (wid == 0) && k > 0;
+    const bool out_global = (wid == nstr - 1) && has_out;
+    int gout = 0, gin = 0;
+    if (out_global) {  // take a slot, reset its progress word, then announce it
+      gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
+      if (lane == 0) {
+        __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slot_of[q], gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (in_global && !band_failed(err)) {  // the left swath's slot
+      const long long t0 = wall_clock64();
+      for (;;) {
+        gin = uniform_i32(__hip_atomic_load(&slot_of[q - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (gin >= 0) break;
+        if (band_timed_out(t0, limit, err, lane)) { gin = 0; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    BandPipe pp;
+    pp.in_global = in_global;
+    pp.in_cnt = in_global ? &g_cnt[gin] : (wid > 0 ? &s_cnt[wid - 1] : nullptr);
+    pp.in_used = (wid > 0) ? &s_used[wid - 1] : nullptr;
+    pp.out_cnt = has_out ? (out_global ? &g_cnt[gout] : &s_cnt[wid]) : nullptr;
+    pp.out_global = out_global;
+    pp.out_used = (has_out && !out_global) ? &s_used[wid] : nullptr;
+    pp.ring = ring;
+    pp.main_end = (OE && ly >= 2) ? ly - 1 : ly;
+    pp.lane = lane;
+    pp.err = err;
+    pp.limit = limit;
+    const int nrows_main = ((OE && ly >= 2) ? ly - 2 : ly - 1) - (OS ? 1 : 0) + 1;
+    const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
+    SwathIO<T> io;
+    io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = off;
+    io.ein = (k == 0 && wid == 0) ? nullptr : (in_global ? gslots + (long long)gin * gstride : rbase + (long long)(wid - 1) * ring);
+    io.imask = in_global ? -1 : ring - 1;
+    io.eout = has_out ? (out_global ? gslots + (long long)gout * gstride : rbase + (long long)wid * ring) : nullptr;
+    io.omask = out_global ? -1 : ring - 1;
+    io.mv = mv + jb.mv_off + (size_t)k * swath_words;
+    io.mtop = mtop + jb.top_off + (size_t)k * 64;
+    io.ref_left = (wid == 0);
+    io.lbase = wid * 16;
+    SwathRes<T> res = split ? swath_dispatch_quarter<T, OS, OE, BandPipe>(C, io, &win, lane, pp)
+                            : swath_dispatch_pipe<T, OS, OE, true, BandPipe>(C, io, &win, lane, pp);
+    if (in_global && lane == 0) {  // every read of the slot has returned: back to the pool
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      atomicOr(&g_free[gin >> 6], 1ull << (gin & 63));
+    }
+    if (OE && !OS && out_global && (k + 1) * ODBA_W > ly && res.top == res.colmin && lane == 0)
+      atomicMin(&abr[j], k + 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Checkpointed paths (global mode, float): the step codes of a long pair
 // without its whole step matrix. In global mode the path does not depend on
 // how the matrix is cut (every cell's code follows from exact cost values),
@@ -1594,6 +1736,35 @@ hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, i
 }
 
 size_t band_pair_size() { return sizeof(BandPair); }
+
+size_t path_band_sync_bytes(int R, long long max_claims, int njobs) {
+  return (size_t)((R + 63) / 64) * 8 + (size_t)R * 4 + (size_t)max_claims * 4 + 64 + (size_t)(njobs + 1) * 8;
+}
+
+hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                             long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
+                             int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
+                             int* err, long long tick_limit) {
+  if (njobs < 1 || njobs > 1024 || ring < 128 || (ring & (ring - 1)) || R < 2 * nblocks + 2) return hipErrorInvalidValue;
+  // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1] (8-aligned)
+  unsigned long long* g_free = (unsigned long long*)sync;
+  int* g_cnt = (int*)(g_free + (R + 63) / 64);
+  int* slot_of = g_cnt + R;
+  long long* band0 = (long long*)(((uintptr_t)(slot_of + max_claims) + 63) & ~(uintptr_t)63);
+  const PathJob<float>* J = (const PathJob<float>*)jobs;
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0);
+  const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
+  hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
+                     max_claims, abr, njobs);
+  dim3 g(nblocks), b(256);
+  float* rg = (float*)rings;
+  float* gs = (float*)gslots;
+#define ODBA_PBANDS(A, B) hipLaunchKernelGGL((k_path_bands<A, B>), g, b, 0, st, J, njobs, band0, counter, mv, mtop, rg, ring, gs, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit)
+  if (os && oe) ODBA_PBANDS(true, true); else if (os) ODBA_PBANDS(true, false); else if (oe) ODBA_PBANDS(false, true); else ODBA_PBANDS(false, false);
+#undef ODBA_PBANDS
+  hipLaunchKernelGGL(k_abr_index, dim3((njobs + 255) / 256), dim3(256), 0, st, abr, njobs);
+  return hipGetLastError();
+}
 
 hipError_t launch_ckpt_forward(int ew, int nblocks, hipStream_t st, const void* jobs, int njobs, long long nclaims,
                                unsigned long long* counter, float* rowck, float* colck, int* cnt, float* rings,

@@ -160,6 +160,13 @@ def test_small_workload_both_schedules_match(ctx):
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), sched
 
 
+# The band schedule of path jobs made an illegal memory access in its first
+# open-end run (300 x 2100, global edge slots between workgroups); the cause is
+# not found, the library refuses the schedule, and its cases stay skipped.
+BAND_OFF = pytest.mark.skip(reason="path band schedule switched off: an open-end run faulted, cause not found")
+PATH_SCHEDS = ["strip", "quarter", pytest.param("band", marks=BAND_OFF)]
+
+
 @contextlib.contextmanager
 def path_schedule(name):
     old = os.environ.get("ODBA_PATH_SCHEDULE")
@@ -176,12 +183,14 @@ def path_schedule(name):
 @pytest.mark.parametrize("os_,oe", MODES)
 @pytest.mark.parametrize("ly,lx", [(300, 2100), (2500, 4100), (1030, 1025), (5000, 3300), (90, 200), (1500, 400),
                                    (300, 700), (2000, 1000)])
-@pytest.mark.parametrize("sched", ["strip", "quarter"])
+@pytest.mark.parametrize("sched", PATH_SCHEDS)
 def test_path_strips_step_matrix(ctx, os_, oe, ly, lx, sched):
     """Path jobs swept by a workgroup of waves (one 1024-column swath each,
-    or one quarter swath each, whose four waves share the step words) give
-    the oracle's step matrix, open-end stop included. The widths cover last
-    swaths of 1 (unsplit), 4, 8, 12 and 16 columns per lane."""
+    or one quarter swath each, whose four waves share the step words), or
+    as bands (one swath per workgroup of four quarter-swath waves, swaths
+    handed over between workgroups), give the oracle's step matrix, open-end
+    stop included. The widths cover last swaths of 1 (unsplit), 4, 8, 12 and
+    16 columns per lane."""
     rng = np.random.default_rng(ly + lx + 2 * os_ + oe)
     y, x = O.znorm(_walks(rng, [ly, lx]))[0]
     print(f"path {sched}s os={os_} oe={oe} {ly}x{lx}", flush=True)
@@ -193,7 +202,7 @@ def test_path_strips_step_matrix(ctx, os_, oe, ly, lx, sched):
 
 
 @pytest.mark.parametrize("os_,oe", MODES)
-@pytest.mark.parametrize("sched", ["strip", "quarter"])
+@pytest.mark.parametrize("sched", PATH_SCHEDS)
 def test_path_strips_dba_round(ctx, os_, oe, sched):
     rng = np.random.default_rng(70 + 2 * os_ + oe)
     seqs = O.znorm(_walks(rng, [2100, 3600, 1500, 4200, 2900]))[0]
