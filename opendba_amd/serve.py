@@ -18,6 +18,10 @@ from . import native as N
 
 
 class Server:
+    """Continuous-batching DBA server over odba_server_* (slots 1..16). The
+    slots' streams overlap only with one hardware queue each: the process
+    must start HIP with GPU_MAX_HW_QUEUES >= slots (HIP's default is 4)."""
+
     def __init__(self, device=0, hbm_budget=0, slots=4):
         self.L = N.load()
         self.h = C.c_void_p()
