@@ -1116,6 +1116,9 @@ template <typename T> struct DbaState {
     return (e && *e) ? std::max(1, std::atoi(e)) : 4;
   }
 
+  std::vector<PJ<T>> keep_pj;  // a one-batch round's host job tables, alive until the round's last sync
+  std::vector<TbJob> keep_tb;
+
   // One DBA round. With host_out, the new centroid is also copied there.
   int round(double* delta, T* host_out = nullptr) {
     if (g_ok && !rec && graphs_enabled() && g_eager >= graph_after()) return graph_round(delta, host_out);
@@ -1202,8 +1205,14 @@ template <typename T> struct DbaState {
       rc = use_ckpt() ? run_ckpt_batch(stripe_idx, b0) : run_stripe_batch(stripe_idx, b0);
       if (rc) return rc;
       HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, b1 - b0, lc, acc, counts, first ? 1 : 0));
-      // the host vectors must outlive the async copies
-      HIPC(c, hipStreamSynchronize(c->stream));
+      // The host vectors must outlive the async copies: a round of one batch
+      // keeps them until its final synchronisation, others wait here.
+      if (b0 == 0 && b1 == n && stripe_idx.empty()) {
+        keep_pj.swap(pj);
+        keep_tb.swap(tb);
+      } else {
+        HIPC(c, hipStreamSynchronize(c->stream));
+      }
       first = false;
       b0 = b1;
     }
@@ -1212,7 +1221,10 @@ template <typename T> struct DbaState {
     unsigned long long bits = 0;
     HIPC(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipMemcpyAsync(&bits, dbits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    if (host_out) HIPC(c, hipMemcpyAsync(host_out, newc, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
+    keep_pj.clear();
+    keep_tb.clear();
     if (herr & 8) return fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
     if (herr & 1) return fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
     if (herr & 4) return fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
@@ -1222,10 +1234,6 @@ template <typename T> struct DbaState {
     if (delta) *delta = d;
     std::swap(cen, newc);
     rec_valid = rec;
-    if (host_out) {
-      HIPC(c, hipMemcpyAsync(host_out, cen, lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-      HIPC(c, hipStreamSynchronize(c->stream));
-    }
     g_eager = g_ok ? g_eager + 1 : 0;
     return ODBA_OK;
   }
