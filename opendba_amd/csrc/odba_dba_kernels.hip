@@ -770,6 +770,8 @@ __device__ __forceinline__ int wave_max_i(int v) {
 
 constexpr int kStageB = 8;    // sequences per batch of the staged accumulation
 constexpr int kStageW = 256;  // values per staged window (4 per lane)
+// windows loaded together per batch (4 values per lane each, in registers)
+template <typename T> constexpr int stage_windows() { return sizeof(T) == 4 ? 32 : 16; }
 
 // Sum of the runs of a wave's 64 consecutive centroid positions, one lane
 // per position, over sequences 0..nall-1 of one cluster (runs[s * stride]
@@ -778,15 +780,16 @@ constexpr int kStageW = 256;  // values per staged window (4 per lane)
 // each sequence, so the wave loads that range with all 64 lanes into LDS,
 // 256 values per window, and each lane then adds its own run from LDS, hi
 // down to lo, in sequence order: the reference's order of additions. The
-// first window of eight sequences is loaded at once (one memory latency
-// per batch); longer ranges load further windows as they are reached.
-// buf holds (kStageB + 1) * kStageW values; every lane of the block's one
-// wave must call this.
+// windows of a batch of eight sequences, in sequence order, are loaded
+// together up to stage_windows<T>() of them (one memory latency per
+// batch); a batch whose ranges need more loads the rest one window at a
+// time as they are reached. buf holds (stage_windows<T>() + 1) * kStageW
+// values; every lane of the block's one wave must call this.
 template <typename T>
 __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride, int nseq,
                                   int nall, const T* __restrict__ data, const long long* __restrict__ offs,
                                   T* __restrict__ buf) {
-  constexpr int B = kStageB, SW = kStageW;
+  constexpr int B = kStageB, SW = kStageW, NW = stage_windows<T>();
   const int lane = lane_id();
   int4 rn[B];
   long long on[B];
@@ -798,7 +801,8 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
   for (int s0 = 0; s0 < nall; s0 += B) {
     int4 r[B];
     long long o[B];
-    int lo[B], hi[B];
+    int lo[B], hi[B], wb0[B], nw[B];
+    int tot = 0;
 #pragma unroll
     for (int u = 0; u < B; u++) {
       r[u] = rn[u];
@@ -806,15 +810,30 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
       const bool ne = r[u].y >= r[u].x;
       lo[u] = wave_min_i(ne ? r[u].x : INT32_MAX);
       hi[u] = wave_max_i(ne ? r[u].y : INT32_MIN);
+      nw[u] = hi[u] >= lo[u] ? (hi[u] - lo[u]) / SW + 1 : 0;
+      wb0[u] = tot;
+      tot += nw[u];
     }
-    T w[B][4];
+    // window k of the batch: sequence u with wb0[u] <= k < wb0[u] + nw[u],
+    // values top, top-1, ... of that sequence (selected without indexing)
+    T w[NW][4];
 #pragma unroll
-    for (int u = 0; u < B; u++)
+    for (int k = 0; k < NW; k++) {
+      int ktop = -1, klo = 0;
+      long long koff = 0;
+#pragma unroll
+      for (int u = 0; u < B; u++)
+        if (k >= wb0[u] && k < wb0[u] + nw[u]) {
+          ktop = hi[u] - (k - wb0[u]) * SW;
+          klo = lo[u];
+          koff = o[u];
+        }
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int idx = hi[u] - (lane * 4 + e);
-        w[u][e] = (hi[u] >= lo[u] && idx >= lo[u]) ? data[o[u] + idx] : T(0);
+        const int idx = ktop - (lane * 4 + e);
+        w[k][e] = (k < tot && idx >= klo) ? data[koff + idx] : T(0);
       }
+    }
 #pragma unroll
     for (int u = 0; u < B; u++) {
       const int sn = s0 + B + u;
@@ -823,35 +842,19 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
     }
     __syncthreads();  // the previous batch's reads of buf are done
 #pragma unroll
-    for (int u = 0; u < B; u++)
+    for (int k = 0; k < NW; k++)
 #pragma unroll
-      for (int e = 0; e < 4; e++) buf[u * SW + lane * 4 + e] = w[u][e];
+      for (int e = 0; e < 4; e++) buf[k * SW + lane * 4 + e] = w[k][e];
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < B; u++) {
-      if (hi[u] >= lo[u]) {  // wave-uniform
-        int top = hi[u];
-        const T* wb = buf + u * SW;  // wb[t] = value top - t
-        for (;;) {
-          const int wlo = max(lo[u], top - SW + 1);
-          if (r[u].y >= r[u].x) {
-            // Eight LDS reads in flight before their eight (dependent) adds:
-            // a run can be tens of thousands of values (a path along row 0),
-            // and its adds are the kernel's longest chain.
-            const int k1 = min(r[u].y, top), k0 = max(r[u].x, wlo);
-            const T* q = wb + (top - k1);
-            int m = k1 - k0 + 1;
-            for (; m >= 8; m -= 8, q += 8) {
-              T v[8];
-#pragma unroll
-              for (int e = 0; e < 8; e++) v[e] = q[e];
-#pragma unroll
-              for (int e = 0; e < 8; e++) a = (T)(a + v[e]);
-            }
-            for (; m > 0; m--, q++) a = (T)(a + *q);
-          }
-          top = wlo - 1;
-          if (top < lo[u]) break;
+      int top = hi[u];
+      for (int wi = 0; wi < nw[u]; wi++) {  // wave-uniform
+        const T* wb;                        // wb[t] = value top - t
+        const int slot = wb0[u] + wi;
+        if (slot < NW) {
+          wb = buf + slot * SW;
+        } else {  // past the batch's loaded windows: load this one now
           T x[4];
 #pragma unroll
           for (int e = 0; e < 4; e++) {
@@ -860,10 +863,42 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
           }
           __syncthreads();
 #pragma unroll
-          for (int e = 0; e < 4; e++) buf[B * SW + lane * 4 + e] = x[e];
+          for (int e = 0; e < 4; e++) buf[NW * SW + lane * 4 + e] = x[e];
           __syncthreads();
-          wb = buf + B * SW;
+          wb = buf + NW * SW;
         }
+        const int wlo = max(lo[u], top - SW + 1);
+        if (r[u].y >= r[u].x) {
+          // A run can be tens of thousands of values (a path along row 0 or
+          // column 0), and its dependent adds are the kernel's longest
+          // chain: the LDS reads of the next eight values are issued before
+          // the adds of the current eight, so the chain does not wait on
+          // LDS latency.
+          const int k1 = min(r[u].y, top), k0 = max(r[u].x, wlo);
+          const T* q = wb + (top - k1);
+          int m = k1 - k0 + 1;
+          if (m >= 8) {
+            T v[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = q[e];
+            q += 8;
+            m -= 8;
+#pragma unroll 2
+            for (; m >= 8; m -= 8, q += 8) {
+              T nx[8];
+#pragma unroll
+              for (int e = 0; e < 8; e++) nx[e] = q[e];
+#pragma unroll
+              for (int e = 0; e < 8; e++) a = (T)(a + v[e]);
+#pragma unroll
+              for (int e = 0; e < 8; e++) v[e] = nx[e];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) a = (T)(a + v[e]);
+          }
+          for (; m > 0; m--, q++) a = (T)(a + *q);
+        }
+        top = wlo - 1;
       }
       if (r[u].y >= r[u].x) n += (uint32_t)(r[u].y - r[u].x + 1);
       if (r[u].w >= r[u].z) {  // second block: one cell of a height-cut stripe row
@@ -880,7 +915,7 @@ template <typename T>
 __global__ void __launch_bounds__(64) k_accumulate(const T* __restrict__ data, const long long* __restrict__ offs,
                                                    const int4* __restrict__ runs, int nseq, int lc,
                                                    T* __restrict__ acc, uint32_t* __restrict__ counts, int first) {
-  __shared__ T buf[(kStageB + 1) * kStageW];
+  __shared__ T buf[(stage_windows<T>() + 1) * kStageW];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = c < lc;
   T a = (first || !on) ? T(0) : acc[c];
@@ -951,7 +986,7 @@ __device__ __forceinline__ int slot_of(const MultiSlot* __restrict__ sl, int ns,
 template <typename T>
 __global__ void __launch_bounds__(64) k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total,
                                                          T* __restrict__ acc, uint32_t* __restrict__ counts) {
-  __shared__ T buf[(kStageB + 1) * kStageW];
+  __shared__ T buf[(stage_windows<T>() + 1) * kStageW];
   // No early exit: lanes past total take part with no sequences.
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = g < total;
