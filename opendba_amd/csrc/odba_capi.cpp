@@ -615,11 +615,20 @@ void plan_job(int ly, int lx, int os, int oe, long long& mv_words, long long& to
   top_words += (long long)K * 64;
 }
 
+// Mean swath count (1024-column swaths of X) over a list of path jobs.
+template <typename J> double mean_swaths(const std::vector<J>& jobs) {
+  if (jobs.empty()) return 0.0;
+  double t = 0;
+  for (const J& j : jobs) t += (j.lx + ODBA_W - 1) / ODBA_W;
+  return t / (double)jobs.size();
+}
+
 // Path jobs: one wave per job, or (float, few long jobs) a workgroup of
 // waves per job sweeping its swaths as a pipeline. ODBA_PATH_SCHEDULE =
 // wave|strip overrides the choice.
 int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
-                  uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0, int* err = nullptr) {
+                  uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0, int* err = nullptr,
+                  double mean_swaths = 0.0) {
   const size_t es = dtype_size(dtype);
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
@@ -633,7 +642,11 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   // been found, so nothing selects it and asking for it is an error.
   const char* env = std::getenv("ODBA_PATH_SCHEDULE");
   const int kmax = (maxlx + ODBA_W - 1) / ODBA_W;
-  bool strips = njobs < 4 * c->cus && maxlx > 256 && minly >= 64;
+  // With many jobs a wave per job fills the device, except when the jobs
+  // average eight or more swaths each (a long centroid on the columns):
+  // then the longest runs alone on one SIMD for the whole round, and the
+  // workgroup-per-job schedule (longest first, its waves busy) is faster.
+  bool strips = (njobs < 4 * c->cus || mean_swaths >= 8.0) && maxlx > 256 && minly >= 64;
   bool quarters = kmax <= 4;
   const bool bands = false;
   if (env && !std::strcmp(env, "band"))
@@ -1198,7 +1211,8 @@ template <typename T> struct DbaState {
         if (rc) return rc;
         HIPC(c, hipMemcpyAsync(jobs, pj.data(), nfull * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
         HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
-        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly, err);
+        rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly, err,
+                           mean_swaths(pj));
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
@@ -1280,7 +1294,8 @@ template <typename T> struct DbaState {
       hipError_t e = launch_fill_u32(c->stream, err, 0u, 1);
       if (e == hipSuccess) e = launch_fill_u32(c->stream, dbits, 0u, 2);
       if (e != hipSuccess) rc = fail(c, ODBA_E_HIP, "graph capture: %s", hipGetErrorString(e));
-      if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly, err);
+      if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly, err,
+                                  mean_swaths(g_pj));
       if (!rc) {
         e = launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, lc, os, oe, err, nullptr);
         if (e == hipSuccess) e = launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs, runs, n, lc, acc, counts, 1);
@@ -1688,7 +1703,8 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
       HIPC(c, hipMemcpyAsync(dslots, slots.data(), slots.size() * sizeof(MultiSlot), hipMemcpyHostToDevice, c->stream));
       HIPC(c, hipMemsetAsync(dbits, 0, nc * sizeof(unsigned long long), c->stream));
       HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
-      rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly, derr);
+      rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly, derr,
+                         mean_swaths(pj));
       if (rc) return rc;
       HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, 0, os, oe, derr, nullptr));
       HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, dslots, (int)slots.size(), pos, acc, counts, dbits, derr));
@@ -1829,7 +1845,7 @@ int prefix_chop_t(odba_ctx* c, const T* prefix_in, int P, T* const* seqs, const 
     HIPC(c, hipMemcpyAsync(jobs, pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(tbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
-    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P, derr);
+    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P, derr, mean_swaths(pj));
     if (rc) return rc;
     HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, P, 0, 1, derr, nullptr));
     int herr = 0;
@@ -2021,7 +2037,7 @@ int kseg_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int n, int m
   HIPC(c, hipMemcpyAsync(dl, L.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   const int e = kseg_extent_host(mk, 256, (int)sizeof(T), lds_bytes);
   if (e < 1) return fail(c, ODBA_E_INVALID, "lds_bytes %d leaves no room for the segmentation tables", lds_bytes);
-  const size_t dyn = std::max((size_t)256 * e * 6, (size_t)spb * sizeof(T));
+  const size_t dyn = std::max((size_t)256 * e * 6, (size_t)(spb + 2) * sizeof(T));
   if (dyn > 150 * 1024) return fail(c, ODBA_E_UNSUPPORTED, "k-segmentation needs %zu bytes of LDS", dyn);
   HIPC(c, launch_kseg(dtype_of<T>(), c->stream, d, doffs, dl, n, maxch, dob, dpb, spb, mk, minseg, lds_bytes, kp, dout, dyn));
   std::vector<T> hm((size_t)std::max(1LL, ototal));

@@ -1144,8 +1144,12 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
       s_free[threadIdx.x] = 1;
     }
     __syncthreads();
-    const int jix = uniform_i32(s_job);
-    if (jix >= njobs) break;
+    const int claim = uniform_i32(s_job);
+    if (claim >= njobs) break;
+    // Claims go from the last job to the first: callers list jobs by
+    // ascending sequence length, so the longest start first and the short
+    // ones fill in behind them.
+    const int jix = njobs - 1 - claim;
     const PathJob<T> jb = jobs[jix];
     const int ly = jb.ly, lx = jb.lx;
     const int K = (lx + ODBA_W - 1) / ODBA_W;

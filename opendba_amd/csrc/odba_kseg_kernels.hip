@@ -228,18 +228,36 @@ __global__ void __launch_bounds__(256) k_kseg(const T* __restrict__ data, const 
     const int pos = t * ds + i;
     if (pos < spb && chunk * spb + pos < orig_N) orig[pos] = ser[pos];
   }
+  // Past the chunk's samples the buffer reads as zero, up to spb + 2 values:
+  // an even-length median reads one or two values past its segment's end
+  // (segmentation.hpp:325-330), and a chunk of fewer samples than ds has an
+  // empty segment whose median reads orig[0] and orig[1].
+  for (int pos = N + t; pos < spb + 2; pos += 256) orig[pos] = T(0);
+  __syncthreads();
+  // A median that reads past its segment sees the next segment's values as
+  // loaded, before that segment is sorted (the segments are taken one after
+  // the other): every thread reads its two values past the end now.
+  const bool seg = t > 0 && t <= test_k;
+  int left = 0, right = 0;
+  T past0 = T(0), past1 = T(0);
+  if (seg) {
+    right = bp[t] * ds;
+    if (right > N) right = N;
+    left = bp[t - 1] * ds;
+    past0 = orig[right];
+    past1 = orig[right + 1];
+  }
   __syncthreads();
   if (t > test_k && t <= mk) {
     outp[t - 1] = KL<T>::mx();
-  } else if (t > 0 && t <= test_k) {
-    int right = bp[t] * ds;
-    if (right > N) right = N;
-    const int left = bp[t - 1] * ds;
+  } else if (seg) {
     for (int i = left; i < right - 1; ++i)
       for (int j = i + 1; j < right; ++j)
         if (orig[i] > orig[j]) { T tmp = orig[i]; orig[i] = orig[j]; orig[j] = tmp; }
-    if ((left - right) % 2) outp[t - 1] = orig[(left + right) / 2];
-    else outp[t - 1] = (T)((orig[(left + right) / 2] + orig[(left + right) / 2 + 1]) / T(2));
+    auto at = [&](int i) -> T { return i < right ? orig[i] : (i == right ? past0 : past1); };
+    const int mid = (left + right) / 2;
+    if ((left - right) % 2) outp[t - 1] = at(mid);
+    else outp[t - 1] = (T)((at(mid) + at(mid + 1)) / T(2));
   }
 }
 

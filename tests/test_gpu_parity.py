@@ -257,6 +257,23 @@ def test_kseg_bitexact(ctx, minseg):
         assert np.array_equal(g, e)
 
 
+@pytest.mark.parametrize("minseg", [3, 4, 8])
+def test_kseg_short_last_chunks(ctx, minseg):
+    """Lengths that leave 1 .. ds+1 samples in the last chunk (fewer samples
+    than the downsample factor give an empty segment, whose median reads the
+    zeros past the samples) and even-length segments whose median reads past
+    their end: equal to the oracle on every one of three runs."""
+    ds, spb, mk = O.kseg_params(minseg)
+    rng = np.random.default_rng(90 + minseg)
+    lens = [spb * k + r for k in (1, 3, 5) for r in range(1, ds + 2)]
+    raw = [(90 + 12 * rng.standard_normal(L)).astype(np.int16).astype(np.float32) for L in lens]
+    exp = O.kseg(raw, minseg)
+    for run in range(3):
+        got = ctx.kseg(raw, minseg)
+        for L, g, e in zip(lens, got, exp):
+            assert np.array_equal(g.view(np.uint32), e.view(np.uint32)), (run, L)
+
+
 def test_kseg_prefix_skip_and_lds_budget(ctx):
     raw = WL.short_to_float(WL.squiggles(6, 2000, 5000, seed=77))
     for lds in (49152, 65536):
