@@ -381,6 +381,8 @@ def main():
     ap.add_argument("--scale", type=float, default=0.5, help="serve: series-count and length scale")
     ap.add_argument("--slots", type=int, default=16, help="serve: concurrent job slots")
     ap.add_argument("--max-rounds", type=int, default=250, help="serve: DBA round limit")
+    ap.add_argument("--dba-rounds", type=int, default=250,
+                    help="squiggles: DBA round limit of the perform_dba run after the timed all-vs-all (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the schedule autotuner")
     ap.add_argument("--order", type=int, choices=[0, 1], default=None,
@@ -444,6 +446,18 @@ def main():
             print(f"step {s}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         sset.close()
+        dba = None
+        if args.dba_rounds > 0:  # the workload's second half: DBA to convergence on the same reads
+            t1 = time.perf_counter()
+            r = ctx.perform_dba(WL.short_to_float(raw), 0, 1, norm=1, max_rounds=args.dba_rounds)
+            dwall = time.perf_counter() - t1
+            dba = {"pipeline": "perform_dba open_end: sanitize, stable sort, z-norm, all-vs-all, medoid, DBA rounds "
+                               f"until the stop rule or {args.dba_rounds} rounds, rescale",
+                   "rounds": r["rounds"], "medoid": int(r["medoid"]), "centroid_len": int(len(r["centroid"])),
+                   "wall_s": round(dwall, 3), "jobs_per_s": round(1.0 / dwall, 5),
+                   "ms_per_round_est": round((dwall * 1000.0 - float(np.mean(kms))) / max(r["rounds"], 1), 2),
+                   "ms_per_round_basis": "(wall - one all-vs-all kernel time) / rounds"}
+            print(f"dba: {r['rounds']} rounds in {dwall:.1f} s", file=sys.stderr, flush=True)
     ms_per_step = wall * 1000.0 / args.steps
     value = nominal / (ms_per_step / 1000.0) / 1e9
     kavg = float(np.mean(kms))
@@ -496,6 +510,8 @@ def main():
             "device_clock_mhz": info["clock_khz"] / 1000.0,
         },
     }
+    if dba is not None:
+        line["dba_to_convergence"] = dba
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(seqs)
     print(json.dumps(line))

@@ -649,6 +649,13 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   bool strips = (njobs < 4 * c->cus || mean_swaths >= 8.0) && maxlx > 256 && minly >= 64;
   bool quarters = kmax <= 4;
   const bool bands = false;
+  // Few jobs (at most 1,024) of which the longest has more swaths than a
+  // strip workgroup has waves (16): its swaths would run one after another
+  // on one CU. Bands of whole swaths spread them over several workgroups.
+  bool swath_bands = njobs < 4 * c->cus && njobs <= 1024 && kmax > 16 && minly >= 64;
+  if (env && std::strcmp(env, "swathband")) swath_bands = false;
+  if (env && !std::strcmp(env, "swathband")) swath_bands = njobs <= 1024 && maxlx > 0;
+  swath_bands = swath_bands && dtype == ODBA_F32 && err != nullptr;  // err: the hand-off time-out word
   if (env && !std::strcmp(env, "band"))
     return fail(c, ODBA_E_INVALID, "the path band schedule is switched off (an open-end run faulted; cause not found)");
   if (env && !std::strcmp(env, "wave")) strips = false;
@@ -657,7 +664,28 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   strips = strips && dtype == ODBA_F32;
   (void)err;
   hipError_t e;
-  if (bands) {
+  if (swath_bands) {
+    const int nw = std::max(1, std::min(16, env_int("ODBA_PATH_BAND_WAVES", 4)));
+    const int per_cu = std::max(1, env_int("ODBA_PATH_BANDS_PER_CU", 2));
+    const long long max_claims = (long long)njobs * ((kmax + nw - 1) / nw);
+    const int nblocks = (int)std::max(1LL, std::min(max_claims, (long long)per_cu * c->cus));
+    const int R = 2 * nblocks + 2;
+    const int ring = std::max(128, env_int("ODBA_BAND_RING", 2048));
+    const long long gstride = maxly + 256;
+    const size_t ring_bytes = (size_t)nblocks * nw * ring * es, slot_bytes = (size_t)R * gstride * es;
+    const size_t sync_bytes = path_band_sync_bytes(R, max_claims, njobs);
+    int rc = ensure_scratch(c, ring_bytes + slot_bytes + sync_bytes + 256);
+    if (rc) return rc;
+    char* base = (char*)c->esc;
+    void* sync = (void*)(((uintptr_t)(base + ring_bytes + slot_bytes) + 255) & ~(uintptr_t)255);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+    const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;
+    unsigned long long* counter64 = (unsigned long long*)((char*)c->small + 64);
+    HIPC(c, launch_fill_u32(c->stream, (unsigned int*)counter64, 0u, 2));
+    e = launch_path_swath_bands(nw, os, oe, nblocks, c->stream, jobs_dev, njobs, max_claims, counter64, mv, mtop, abr,
+                                base, ring, base + ring_bytes, gstride, sync, R, err, limit);
+  } else if (bands) {
     const long long max_claims = (long long)njobs * std::max(1, kmax);
     const int nblocks = (int)std::max(1LL, std::min(max_claims, 4LL * c->cus));
     const int R = 2 * nblocks + 2;
@@ -716,6 +744,11 @@ int dtw_path_t(odba_ctx* c, const T* y, int ly, const T* x, int lx, int os, int 
   HIPC(c, hipMalloc((void**)&mtop, std::max(1LL, tw) * 4));
   HIPC(c, hipMalloc((void**)&abr, 2 * sizeof(int)));  // [abrogated swath, band hand-off error]
   HIPC(c, hipMemsetAsync(abr + 1, 0, sizeof(int), c->stream));
+  // Step words of swaths past an open-end stop may stay unwritten; they are
+  // filled with code 3 (decoded as RIGHT) so that a read of one would show
+  // in the returned step matrix instead of passing on zeroed memory.
+  HIPC(c, launch_fill_u32(c->stream, mv, 0xFFFFFFFFu, std::max(1LL, mvw)));
+  HIPC(c, launch_fill_u32(c->stream, mtop, 0xFFFFFFFFu, std::max(1LL, tw)));
   HIPC(c, hipMemcpyAsync(jd, &job, sizeof(job), hipMemcpyHostToDevice, c->stream));
   std::vector<const T*> sp{y, x};
   std::vector<uint64_t> sl{(uint64_t)ly, (uint64_t)lx};

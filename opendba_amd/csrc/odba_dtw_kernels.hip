@@ -256,6 +256,7 @@ struct NoPipe {
   __device__ void publish(int) const {}
   __device__ void consumed(int) const {}  // input rows below the argument have been read
   __device__ void reserve(int) const {}   // output rows below the argument may be written now
+  __device__ bool cancelled() const { return false; }  // the rest of the swath is not needed
 };
 
 template <typename T> struct SwathRes {
@@ -323,7 +324,6 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     }
   };
 
-  SwathRes<T> res;
   T runmin = MAXV;
   uint32_t h1 = 0, h2 = 0, h3 = 0;  // QW: this lane's fragments 1, 2, 3 steps back
   auto put_word = [&](int s, uint32_t word, bool act) {
@@ -359,7 +359,13 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
     e_ready = true;
   }
 
+  SwathRes<T> res;
+  res.colmin = MAXV; res.top = MAXV; res.last = T(0);
   for (int b = 0; b < nblk; b++) {
+    if (pipe.cancelled()) {  // no result of this swath is needed any more
+      __builtin_amdgcn_s_waitcnt(0);
+      return res;
+    }
     // --- stage the window for block b
     if (!e_ready) {
       pipe.wait(rows_of_block(b));
@@ -489,6 +495,10 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
 
   // --- open-end top row: D(t,j) = min(min(U,G), D(t,j-1)) with U,G carrying
   // the local cost, except at the swath-left column (dtw.hpp:175-186).
+  if (pipe.cancelled()) {
+    __builtin_amdgcn_s_waitcnt(0);
+    return res;
+  }
   if (OE && ly >= 2) {
     const int t = ly - 1;
     const T yv = a.y[t];
@@ -766,6 +776,14 @@ struct StripPipe {
   int in_strip;   // the left neighbour's strip index
   int* out_cnt;   // LDS: rows published of this strip's edge (null: none)
   int lane;
+  // Open-end early stop: LDS word holding the lowest swath decided so far
+  // at which the sweep stops (INT_MAX: none yet), and this strip's swath.
+  // A strip at or past a decided stop is cancelled: nothing reads its cells.
+  const int* stop = nullptr;
+  int swath = 0;
+  __device__ bool cancelled() const {
+    return stop && uniform_i32(__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) <= swath;
+  }
   __device__ bool ready(int rows) const {
     const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
@@ -778,6 +796,7 @@ struct StripPipe {
       const int o = uniform_i32(__hip_atomic_load(in_owner, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
       const int c = uniform_i32(__hip_atomic_load(in_cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
       if (o == in_strip && c >= rows) break;
+      if (cancelled()) return;  // the producer (at or past the stop too) may never publish
       __builtin_amdgcn_s_sleep(2);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -895,6 +914,7 @@ struct BandPipe {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0) __hip_atomic_store(in_used, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
+  __device__ bool cancelled() const { return false; }
   __device__ void reserve(int rows) const {
     if (!out_used) return;
     const int need = rows - ring;
@@ -1104,10 +1124,13 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
 // Path jobs with a workgroup of waves per job (few long jobs: the DBA round
 // of a cluster with tens to hundreds of members). The strips are the
 // reference's 1024-column swaths, so the step words land exactly where the
-// one-wave kernel puts them; the edge hand-off and the after-the-fact
-// open-end stop are those of the strip schedules (the words of swaths past
-// the stop are written but never read: the traceback treats those cells
-// positionally).
+// one-wave kernel puts them; the edge hand-off is that of the strip
+// schedules. The open-end stop is decided at each swath's end as the strips
+// finish, and once it is decided the strips at or past it stop sweeping
+// (and later ones never start): a short sequence against a long centroid
+// sweeps a few swaths past its own length, not the whole centroid. Words of
+// swaths at or past the stop may be left unwritten or partly written; the
+// traceback never reads them (it treats those cells positionally).
 // ---------------------------------------------------------------------------
 // With QW every swath of 4+ columns per lane is split into four quarter
 // strips (C/4 columns per lane, one wave each), so a pair of one to four
@@ -1165,19 +1188,32 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
       const int off = k * ODBA_W + q * 64 * C;
       const int so = s % nslot, si = (s + nslot - 1) % nslot;
       const bool has_out = (s < NS - 1);
-      if (has_out) {
+      StripPipe pp{&s_owner[si], &s_cnt[si], s - 1, has_out ? &s_cnt[so] : nullptr, lane};
+      if (OE && !OS) {  // strips at or past a decided open-end stop are cancelled
+        pp.stop = &s_abr;
+        pp.swath = k;
+      }
+      // A cancelled strip claims no slot (a slot's previous reader may still
+      // be running) and sweeps nothing. Strips are cancelled in strip order,
+      // so every strip that waits on a cancelled one is cancelled too and
+      // leaves its waits (run_swath polls the stop word in them).
+      bool skip = pp.cancelled();
+      if (has_out && !skip) {
         for (;;) {
           const int f = uniform_i32(__hip_atomic_load(&s_free[so], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
           if (f) break;
+          if (pp.cancelled()) {
+            skip = true;
+            break;
+          }
           __builtin_amdgcn_s_sleep(2);
         }
-        if (lane == 0) {
+        if (!skip && lane == 0) {
           __hip_atomic_store(&s_free[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           __hip_atomic_store(&s_cnt[so], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           __hip_atomic_store(&s_owner[so], s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
-      StripPipe pp{&s_owner[si], &s_cnt[si], s - 1, has_out ? &s_cnt[so] : nullptr, lane};
       SwathIO<T> io;
       io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = off;
       io.ein = s > 0 ? ebase + (long long)si * stride : nullptr;
@@ -1186,15 +1222,20 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
       io.mtop = mtop + jb.top_off + (size_t)k * 64;
       io.ref_left = (q == 0);
       io.lbase = q * 16;
-      SwathRes<T> res = split ? swath_dispatch_quarter<T, OS, OE, StripPipe>(C, io, &win, lane, pp)
-                              : swath_dispatch_pipe<T, OS, OE, true, StripPipe>(C, io, &win, lane, pp);
+      SwathRes<T> res;
+      if (!skip)
+        res = split ? swath_dispatch_quarter<T, OS, OE, StripPipe>(C, io, &win, lane, pp)
+                    : swath_dispatch_pipe<T, OS, OE, true, StripPipe>(C, io, &win, lane, pp);
       if (s > 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         if (lane == 0) __hip_atomic_store(&s_free[si], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      // the open-end stop is decided on a swath's right edge column (dtw.hpp:74-129)
+      // the open-end stop is decided on a swath's right edge column
+      // (dtw.hpp:74-129); a strip cancelled meanwhile holds no edge, and a
+      // stop already decided at or below its swath is the lower one anyway
       const bool swath_end = !split || q == 3;
-      if (OE && !OS && has_out && swath_end && k * ODBA_W + ODBA_W > ly && res.top == res.colmin && lane == 0)
+      if (OE && !OS && has_out && swath_end && !skip && !pp.cancelled() && k * ODBA_W + ODBA_W > ly &&
+          res.top == res.colmin && lane == 0)
         atomicMin(&s_abr, k + 1);
     }
     __syncthreads();
@@ -1217,10 +1258,11 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
 // band0[p] = first claim of the p-th job in claim order; band0[njobs] = all.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
-                                                     long long* __restrict__ band0) {
+                                                     long long* __restrict__ band0, int per_band) {
   __shared__ long long s[1024];
   const int t = threadIdx.x;
-  const long long v = (t < njobs) ? (long long)((jobs[njobs - 1 - t].lx + ODBA_W - 1) / ODBA_W) : 0;
+  const int K = (t < njobs) ? (jobs[njobs - 1 - t].lx + ODBA_W - 1) / ODBA_W : 0;  // swaths of the job
+  const long long v = (long long)((K + per_band - 1) / per_band);                  // its bands
   s[t] = v;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
@@ -1341,6 +1383,132 @@ __global__ void __launch_bounds__(256) k_path_bands(const PathJob<float>* __rest
     }
     if (OE && !OS && out_global && (k + 1) * ODBA_W > ly && res.top == res.colmin && lane == 0)
       atomicMin(&abr[j], k + 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Path jobs as bands of whole swaths (float): the band hand-off of
+// k_all_pairs_bands with step words. A job's 1024-column swaths are cut
+// into bands of nw consecutive swaths; workgroups of nw waves claim bands
+// from one counter, jobs from the last (longest, as callers list them by
+// ascending length) to the first and bands left to right, and wave w sweeps
+// swath w of its band as one strip of C = 16 columns per lane (the last
+// swath with its own C), so the step words land exactly where the one-wave
+// kernel puts them. Inside a band the edge column goes through LDS-counted
+// rings; between bands through whole-column global slots from a free pool,
+// published every 256 rows. A long pair's swaths thus run on as many
+// workgroups (and CUs) as it has bands instead of on one.
+//
+// No deadlock: every wait targets a lower claim, or a lower strip of the
+// same claim (the ring wait of a producer targets the next strip of its
+// band, whose band's last strip never waits on a ring); claims are taken in
+// order by running workgroups. Every wait still ends after `limit` ticks,
+// setting bit 8 of *err.
+//
+// The open-end stop (dtw.hpp:74-129) is decided at each swath's end into
+// abr[job] (an atomic minimum; INT_MAX until k_abr_index turns it into -1);
+// swaths past it are swept anyway and their words are never read.
+// band0[p] = first claim of the p-th job in claim order; band0[njobs] = all.
+// ---------------------------------------------------------------------------
+template <bool OS, bool OE>
+__global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>* __restrict__ jobs, int njobs,
+                                                           const long long* __restrict__ band0,
+                                                           unsigned long long* __restrict__ counter,
+                                                           uint32_t* __restrict__ mv, uint32_t* __restrict__ mtop,
+                                                           float* __restrict__ rings, int ring,
+                                                           float* __restrict__ gslots, long long gstride,
+                                                           int* __restrict__ g_cnt, unsigned long long* __restrict__ g_free,
+                                                           int R, int* __restrict__ slot_of, int* __restrict__ abr,
+                                                           int* __restrict__ err, long long limit) {
+  using T = float;
+  __shared__ T s_y[kMaxStripWaves][128];
+  __shared__ T s_e[kMaxStripWaves][64];
+  __shared__ T s_o[kMaxStripWaves][128];
+  __shared__ int s_cnt[kMaxStripWaves];   // rows published into ring w
+  __shared__ int s_used[kMaxStripWaves];  // rows of ring w read by wave w+1
+  __shared__ unsigned long long s_q;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int nw = uniform_i32((int)(blockDim.x >> 6));
+  T* rbase = rings + (long long)blockIdx.x * nw * ring;
+  const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
+  const long long nclaims = band0[njobs];
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous claim
+    if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
+    if (threadIdx.x < kMaxStripWaves) {
+      s_cnt[threadIdx.x] = 0;
+      s_used[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const long long q = (long long)uniform_u64(s_q);
+    if (q >= nclaims) break;
+    int lo = 0, hi = njobs - 1;  // claim-order position: last with band0 <= q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (band0[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const int pos = uniform_i32(lo);
+    const int j = njobs - 1 - pos;
+    const PathJob<T> jb = jobs[j];
+    const int ly = jb.ly, lx = jb.lx;
+    const int K = (lx + ODBA_W - 1) / ODBA_W;
+    const int b = uniform_i32((int)(q - band0[pos]));  // band of the job
+    const int s0 = b * nw;
+    const int slast = min(s0 + nw, K) - 1;
+    const int s = s0 + wid;  // this wave's swath
+    if (s > slast) continue;  // no swath for this wave in the job's last band
+    const int C = (s < K - 1) ? 16 : last_swath_c(lx - (K - 1) * ODBA_W);
+    const bool has_out = s < K - 1;
+    const bool in_global = (wid == 0) && s > 0;
+    const bool out_global = (s == slast) && has_out;
+    int gout = 0, gin = 0;
+    if (out_global) {  // take a slot, reset its progress word, then announce it
+      gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
+      if (lane == 0) {
+        __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slot_of[q], gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (in_global && !band_failed(err)) {  // the left band's slot
+      const long long t0 = wall_clock64();
+      for (;;) {
+        gin = uniform_i32(__hip_atomic_load(&slot_of[q - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (gin >= 0) break;
+        if (band_timed_out(t0, limit, err, lane)) { gin = 0; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    BandPipe pp;
+    pp.in_global = in_global;
+    pp.in_cnt = in_global ? &g_cnt[gin] : (s > 0 ? &s_cnt[wid - 1] : nullptr);
+    pp.in_used = (s > 0 && !in_global) ? &s_used[wid - 1] : nullptr;
+    pp.out_cnt = has_out ? (out_global ? &g_cnt[gout] : &s_cnt[wid]) : nullptr;
+    pp.out_global = out_global;
+    pp.out_used = (has_out && !out_global) ? &s_used[wid] : nullptr;
+    pp.ring = ring;
+    pp.main_end = (OE && ly >= 2) ? ly - 1 : ly;
+    pp.lane = lane;
+    pp.err = err;
+    pp.limit = limit;
+    const int nrows_main = ((OE && ly >= 2) ? ly - 2 : ly - 1) - (OS ? 1 : 0) + 1;
+    const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
+    SwathIO<T> io;
+    io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = s * ODBA_W;
+    io.ein = s == 0 ? nullptr : (in_global ? gslots + (long long)gin * gstride : rbase + (long long)(wid - 1) * ring);
+    io.imask = in_global ? -1 : ring - 1;
+    io.eout = has_out ? (out_global ? gslots + (long long)gout * gstride : rbase + (long long)wid * ring) : nullptr;
+    io.omask = out_global ? -1 : ring - 1;
+    io.mv = mv + jb.mv_off + (size_t)s * swath_words;
+    io.mtop = mtop + jb.top_off + (size_t)s * 64;
+    io.ref_left = 1;
+    SwathRes<T> res = swath_dispatch_pipe<T, OS, OE, true, BandPipe>(C, io, &win, lane, pp);
+    if (in_global && lane == 0) {  // every read of the slot has returned: back to the pool
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      atomicOr(&g_free[gin >> 6], 1ull << (gin & 63));
+    }
+    if (OE && !OS && has_out && (s + 1) * ODBA_W > ly && res.top == res.colmin && lane == 0) atomicMin(&abr[j], s + 1);
   }
 }
 
@@ -1756,7 +1924,7 @@ hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const 
   int* slot_of = g_cnt + R;
   long long* band0 = (long long*)(((uintptr_t)(slot_of + max_claims) + 63) & ~(uintptr_t)63);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
-  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0);
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, 1);
   const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
   hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
                      max_claims, abr, njobs);
@@ -1766,6 +1934,33 @@ hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const 
 #define ODBA_PBANDS(A, B) hipLaunchKernelGGL((k_path_bands<A, B>), g, b, 0, st, J, njobs, band0, counter, mv, mtop, rg, ring, gs, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit)
   if (os && oe) ODBA_PBANDS(true, true); else if (os) ODBA_PBANDS(true, false); else if (oe) ODBA_PBANDS(false, true); else ODBA_PBANDS(false, false);
 #undef ODBA_PBANDS
+  hipLaunchKernelGGL(k_abr_index, dim3((njobs + 255) / 256), dim3(256), 0, st, abr, njobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
+                                   long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
+                                   int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
+                                   int* err, long long tick_limit) {
+  if (njobs < 1 || njobs > 1024 || nw < 1 || nw > kMaxStripWaves || ring < 128 || (ring & (ring - 1)) ||
+      R < 2 * nblocks + 2)
+    return hipErrorInvalidValue;
+  // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1] (8-aligned)
+  unsigned long long* g_free = (unsigned long long*)sync;
+  int* g_cnt = (int*)(g_free + (R + 63) / 64);
+  int* slot_of = g_cnt + R;
+  long long* band0 = (long long*)(((uintptr_t)(slot_of + max_claims) + 63) & ~(uintptr_t)63);
+  const PathJob<float>* J = (const PathJob<float>*)jobs;
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, nw);
+  const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
+  hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
+                     max_claims, abr, njobs);
+  dim3 g(nblocks), b(64 * nw);
+  float* rg = (float*)rings;
+  float* gs = (float*)gslots;
+#define ODBA_PSB(A, B) hipLaunchKernelGGL((k_path_swath_bands<A, B>), g, b, 0, st, J, njobs, band0, counter, mv, mtop, rg, ring, gs, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit)
+  if (os && oe) ODBA_PSB(true, true); else if (os) ODBA_PSB(true, false); else if (oe) ODBA_PSB(false, true); else ODBA_PSB(false, false);
+#undef ODBA_PSB
   hipLaunchKernelGGL(k_abr_index, dim3((njobs + 255) / 256), dim3(256), 0, st, abr, njobs);
   return hipGetLastError();
 }

@@ -164,20 +164,15 @@ def test_small_workload_both_schedules_match(ctx):
 # open-end run (300 x 2100, global edge slots between workgroups); the cause is
 # not found, the library refuses the schedule, and its cases stay skipped.
 BAND_OFF = pytest.mark.skip(reason="path band schedule switched off: an open-end run faulted, cause not found")
-PATH_SCHEDS = ["strip", "quarter", pytest.param("band", marks=BAND_OFF)]
+# "swathband": bands of four whole swaths per workgroup; "swathband1": one
+# swath per band, so every swath hands over to the next through a global slot.
+PATH_SCHEDS = ["strip", "quarter", pytest.param("band", marks=BAND_OFF), "swathband", "swathband1"]
 
 
-@contextlib.contextmanager
 def path_schedule(name):
-    old = os.environ.get("ODBA_PATH_SCHEDULE")
-    os.environ["ODBA_PATH_SCHEDULE"] = name
-    try:
-        yield
-    finally:
-        if old is None:
-            del os.environ["ODBA_PATH_SCHEDULE"]
-        else:
-            os.environ["ODBA_PATH_SCHEDULE"] = old
+    if name == "swathband1":
+        return env(ODBA_PATH_SCHEDULE="swathband", ODBA_PATH_BAND_WAVES=1)
+    return env(ODBA_PATH_SCHEDULE=name, ODBA_PATH_BAND_WAVES=None)
 
 
 @pytest.mark.parametrize("os_,oe", MODES)
@@ -233,3 +228,68 @@ def test_band_schedule_long_pairs_default_bands(ctx):
     R = np.zeros((n, n), dtype=np.float32)
     R[iu] = rev
     assert np.array_equal(R[::-1, ::-1].T[iu].view(np.uint32), got.view(np.uint32))
+
+
+# Short sequences against long ones in open end: the reference stops a few
+# swaths past the short sequence's length (dtw.hpp:74-129), and the strip
+# schedules cancel every strip at or past that stop once it is decided. The
+# step matrix returned by dtw_path starts from step words filled with an
+# invalid code, so a read of a word the cancelled strips left unwritten would
+# show as a wrong move.
+# (700, 12600) never stops: every strip is swept, as before.
+STOP_CASES = [(300, 9000), (64, 6000), (500, 12500), (2000, 16400), (1100, 7300), (150, 20000), (700, 12600)]
+
+
+@pytest.mark.parametrize("ly,lx", STOP_CASES)
+@pytest.mark.parametrize("sched", ["strip", "quarter", "swathband", "swathband1"])
+def test_path_strips_open_end_stop_cancels(ctx, ly, lx, sched):
+    rng = np.random.default_rng(ly * 7 + lx)
+    y, x = O.znorm(_walks(rng, [ly, lx]))[0]
+    ed, EM, eab, _ = O.dtw_pair(y, x, os_=0, oe=1, moves=True)
+    K = (lx + 1023) // 1024
+    assert eab == -1 or 1 <= eab < K, (eab, K)
+    print(f"open-end stop {sched}s {ly}x{lx} (stop swath {eab} of {K})", flush=True)
+    with path_schedule(sched):
+        d, M = ctx.dtw_path(y, x, 0, 1)
+    assert np.array_equal(M, EM)
+    assert d == ed
+
+
+@pytest.mark.parametrize("sched", ["strip", "quarter", "swathband", "swathband1"])
+def test_path_strips_dba_round_long_centroid_open_end(ctx, sched):
+    """A DBA round of short sequences against a long centroid (open end, no
+    flip: Y = sequence), plus two sequences longer than the centroid (flipped:
+    Y = centroid), against the oracle, twice in a row on the same context."""
+    rng = np.random.default_rng(91)
+    seqs = O.znorm(_walks(rng, [600, 900, 1500, 2500, 700, 14000, 15500]))[0]
+    cen = O.znorm(_walks(rng, [13000]))[0][0]
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_=0, oe=1)
+    assert rc == 0
+    print(f"dba round long centroid open end, {sched}s", flush=True)
+    for _ in range(2):
+        with path_schedule(sched):
+            got, counts, delta = ctx.dba_round(cen, seqs, 0, 1)
+        assert np.array_equal(counts, ecounts)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+        assert delta == edelta
+
+
+@pytest.mark.parametrize("oe", [0, 1])
+def test_path_swath_bands_chosen_for_a_long_centroid(ctx, oe):
+    """Few jobs whose longest has more than 16 swaths take the swath-band
+    schedule by default: a DBA round of short sequences against an
+    18-swath centroid, plus one longer sequence (flipped in open end),
+    against the oracle over three rounds."""
+    rng = np.random.default_rng(93 + oe)
+    seqs = O.znorm(_walks(rng, [700, 3000, 1800, 19000]))[0]
+    cen = O.znorm(_walks(rng, [17500]))[0][0]
+    print(f"swath bands by default oe={oe}", flush=True)
+    with env(ODBA_PATH_SCHEDULE=None, ODBA_PATH_BAND_WAVES=None):
+        for _ in range(3):
+            got, counts, delta = ctx.dba_round(cen, seqs, 0, oe)
+            rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_=0, oe=oe)
+            assert rc == 0
+            assert np.array_equal(counts, ecounts)
+            assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+            assert delta == edelta
+            cen = exp
