@@ -255,6 +255,10 @@ struct PairSchedule {
   bool strips = false;
   int order = 0;
   int C = 16, waves = 4, nblocks = 1, ring = 2048;
+  // mixed: pairs of more than mixed_cells cells run first as bands of
+  // strips, the rest one wave per pair, largest first
+  bool mixed = false;
+  long long mixed_cells = 0;
 };
 
 int env_int(const char* name, int dflt) {
@@ -321,6 +325,26 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, 
     }
   }
   const char* env = std::getenv("ODBA_PAIR_SCHEDULE");
+  // A few pairs far larger than the rest: one wave per pair would leave the
+  // largest sweeping alone long after the others are done (a 45,000 x 48,000
+  // pair is 47 swaths on one wave). When the largest pair holds more than
+  // twice a resident wave's even share of all cells, the pairs above that
+  // share run first as bands over the whole device and the rest one wave
+  // per pair (largest first).
+  if (!ps.strips && (t == c->tuned.end() || (env && !std::strcmp(env, "mixed"))) && eligible &&
+      !(env && std::strcmp(env, "mixed"))) {
+    long double sum = 0, sq = 0;
+    std::vector<int> L(s->h_lens.begin(), s->h_lens.end());
+    for (int v : L) { sum += v; sq += (long double)v * v; }
+    std::sort(L.begin(), L.end());
+    const long double total = (sum * sum - sq) / 2;
+    const long double share = total / (16.0L * c->cus);  // 4 waves per SIMD resident
+    const long double largest = L.size() >= 2 ? (long double)L[L.size() - 1] * L[L.size() - 2] : 0;
+    if ((largest > 2 * share && largest >= (long double)ODBA_W * 4096) || (env && !std::strcmp(env, "mixed"))) {
+      ps.mixed = true;
+      ps.mixed_cells = (long long)std::max<long double>(share, 1.0L);
+    }
+  }
   if (env && !std::strcmp(env, "wave") && ps.strips) {
     ps = PairSchedule();
     ps.nblocks = grid_blocks(c, npairs);
@@ -361,21 +385,26 @@ int ensure_pair_order(odba_ctx* c, odba_seqset* s, long long npairs) {
 // Bands of strips (k_all_pairs_bands): claim table in the one-wave kernel's
 // pair order, scratch, launch, hand-off check.
 int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, int oe, long long npairs, void* dist,
-                    int* flags) {
+                    int* flags, const std::vector<int2>* pairs = nullptr, bool events = true) {
   const int n = s->n;
   const long long SW = 64LL * ps.C;
   struct Tab { int r, c; long long band0; };
   static_assert(sizeof(Tab) == 16, "claim table entry");
   if (band_pair_size() != sizeof(Tab)) return fail(c, ODBA_E_INVALID, "claim table layout mismatch");
   std::vector<Tab> tab;
-  tab.reserve((size_t)npairs);
+  tab.reserve(pairs ? pairs->size() : (size_t)npairs);
   long long nclaims = 0;
-  for (int cc = n - 1; cc >= 1; cc--)
-    for (int r = cc - 1; r >= 0; r--) {
-      tab.push_back(Tab{r, cc, nclaims});
-      const long long S = ((long long)s->h_lens[cc] + SW - 1) / SW;
-      nclaims += (S + ps.waves - 1) / ps.waves;
-    }
+  auto add = [&](int r, int cc) {
+    tab.push_back(Tab{r, cc, nclaims});
+    const long long S = ((long long)s->h_lens[cc] + SW - 1) / SW;
+    nclaims += (S + ps.waves - 1) / ps.waves;
+  };
+  if (pairs) {
+    for (const int2& pr : *pairs) add(pr.x, pr.y);
+  } else {
+    for (int cc = n - 1; cc >= 1; cc--)
+      for (int r = cc - 1; r >= 0; r--) add(r, cc);
+  }
   const long long per_cu = std::max(1, 16 / ps.waves);
   const int nblocks = (int)std::max<long long>(1, std::min<long long>(nclaims, per_cu * c->cus));
   const int R = 2 * nblocks + 2;
@@ -399,12 +428,12 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
   HIPC(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
   unsigned long long* counter = (unsigned long long*)c->small;
   HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
-  HIPC(c, hipEventRecord(c->ev0, c->stream));
+  if (events) HIPC(c, hipEventRecord(c->ev0, c->stream));
   hipError_t e = launch_all_pairs_bands(s->dtype, ps.C, ps.waves, os, oe, nblocks, c->stream, s->data, s->offs, s->lens,
                                         n, dtab, (int)tab.size(), nclaims, counter, dist, rings, ps.ring, gslots, gstride,
                                         sync, R, err, limit, flags);
   if (e != hipSuccess) return fail(c, ODBA_E_HIP, "band launch: %s", hipGetErrorString(e));
-  HIPC(c, hipEventRecord(c->ev1, c->stream));
+  if (events) HIPC(c, hipEventRecord(c->ev1, c->stream));
   int herr = 0;
   HIPC(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -417,7 +446,53 @@ int all_pairs_bands(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os,
 int run_all_pairs(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, int oe, long long npairs, void* dist,
                   int* flags, double* ms) {
   int rc;
-  if (ps.strips) {
+  if (ps.mixed) {
+    // the pairs above the threshold as bands of strips, then the rest one
+    // wave per pair, largest first; one event pair brackets both launches
+    std::vector<int2> big, rest;
+    rest.reserve((size_t)npairs);
+    int maxx = 0;
+    for (int cc = s->n - 1; cc >= 1; cc--)
+      for (int r = cc - 1; r >= 0; r--) {
+        if ((long long)s->h_lens[r] * s->h_lens[cc] > ps.mixed_cells) {
+          big.push_back(make_int2(r, cc));
+          maxx = std::max(maxx, s->h_lens[cc]);
+        } else {
+          rest.push_back(make_int2(r, cc));
+        }
+      }
+    std::stable_sort(rest.begin(), rest.end(), [&](const int2& a, const int2& b) {
+      return (long long)s->h_lens[a.x] * s->h_lens[a.y] > (long long)s->h_lens[b.x] * s->h_lens[b.y];
+    });
+    PairSchedule bs;
+    bs.strips = true;
+    bs.C = maxx <= 16 * 64 ? 1 : (maxx <= 16 * 256 ? 4 : 16);
+    const long long smax = (maxx + 64LL * bs.C - 1) / (64LL * bs.C);
+    bs.waves = (int)std::min<long long>(16, std::max<long long>(1, smax));
+    bs.ring = std::max(128, env_int("ODBA_BAND_RING", 2048));
+    HIPC(c, hipEventRecord(c->ev0, c->stream));
+    if (!big.empty() && (rc = all_pairs_bands(c, s, bs, os, oe, npairs, dist, flags, &big, false))) return rc;
+    if (!rest.empty()) {
+      DevBufs B;
+      int2* ord = nullptr;
+      if ((rc = B.get(c, ord, rest.size()))) return rc;
+      HIPC(c, hipMemcpyAsync(ord, rest.data(), rest.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+      const long long stride = s->maxlen + 256;
+      const int nb = grid_blocks(c, (long long)rest.size());
+      rc = ensure_scratch(c, (size_t)nb * 4 * 2 * stride * dtype_size(s->dtype));
+      if (rc) return rc;
+      unsigned long long* counter = (unsigned long long*)c->small;
+      HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+      hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nb, c->stream, s->data, s->offs, s->lens, s->n,
+                                      (long long)rest.size(), counter, dist, c->esc, stride, flags, ord);
+      if (e != hipSuccess) return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e));
+      HIPC(c, hipEventRecord(c->ev1, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));  // before B frees the order
+    } else {
+      HIPC(c, hipEventRecord(c->ev1, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+    }
+  } else if (ps.strips) {
     rc = all_pairs_bands(c, s, ps, os, oe, npairs, dist, flags);
     if (rc) return rc;
   } else {

@@ -293,3 +293,41 @@ def test_path_swath_bands_chosen_for_a_long_centroid(ctx, oe):
             assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
             assert delta == edelta
             cen = exp
+
+
+def _mixed_set(rng, dtype=np.float32):
+    # 40 short series and three long ones: the largest pairs hold far more
+    # than a resident wave's even share of the cells
+    lens = list(rng.integers(300, 600, size=40)) + [6100, 8000, 9000]
+    rng.shuffle(lens)
+    if dtype == np.float32:
+        return O.znorm(_walks(rng, lens))[0]
+    lo = -50 if dtype == np.int32 else 0
+    return [rng.integers(lo, 60, size=int(L)).astype(dtype) for L in lens]
+
+
+@pytest.mark.parametrize("sched", ["mixed", "auto"])
+@pytest.mark.parametrize("os_,oe", MODES)
+def test_mixed_schedule_bitexact(ctx, os_, oe, sched):
+    """The largest pairs as bands across the device, the rest one wave per
+    pair: the oracle's distances, and the one-wave schedule's open-end stop
+    flags. "auto" takes the mixed schedule by the default rule."""
+    rng = np.random.default_rng(40 + 2 * os_ + oe)
+    seqs = _mixed_set(rng)
+    print(f"mixed schedule ({sched}) os={os_} oe={oe}", flush=True)
+    got, flags = _run(ctx, seqs, os_, oe, sched)
+    exp = O.all_pairs(seqs, os_, oe)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    _, flags_w = _run(ctx, seqs, os_, oe, "wave")
+    assert np.array_equal(flags, flags_w)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.uint32])
+def test_mixed_schedule_integers(ctx, dtype):
+    rng = np.random.default_rng(44)
+    seqs = _mixed_set(rng, dtype)
+    for os_, oe in MODES:
+        print(f"mixed schedule {np.dtype(dtype).name} os={os_} oe={oe}", flush=True)
+        got, _ = _run(ctx, seqs, os_, oe, "mixed")
+        exp = O.all_pairs(seqs, os_, oe)
+        assert np.array_equal(got, exp), (os_, oe)
