@@ -327,10 +327,14 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, 
   const char* env = std::getenv("ODBA_PAIR_SCHEDULE");
   // A few pairs far larger than the rest: one wave per pair would leave the
   // largest sweeping alone long after the others are done (a 45,000 x 48,000
-  // pair is 47 swaths on one wave). When the largest pair holds more than
-  // twice a resident wave's even share of all cells, the pairs above that
-  // share run first as bands over the whole device and the rest one wave
-  // per pair (largest first).
+  // pair is 47 swaths on one wave). A pair of one share (all cells over the
+  // resident waves) ends about when the whole launch could; when the largest
+  // pair holds more than that, the pairs above half a share run first as
+  // bands over the whole device and the rest one wave per pair (largest
+  // first). With the open-end stop (open end without open start) the bands
+  // sweep every strip while a wave stops at the stop swath, so there the
+  // largest pair must hold twice a share and the bands take the pairs
+  // above one share.
   if (!ps.strips && (t == c->tuned.end() || (env && !std::strcmp(env, "mixed"))) && eligible &&
       !(env && std::strcmp(env, "mixed"))) {
     long double sum = 0, sq = 0;
@@ -340,9 +344,10 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, 
     const long double total = (sum * sum - sq) / 2;
     const long double share = total / (16.0L * c->cus);  // 4 waves per SIMD resident
     const long double largest = L.size() >= 2 ? (long double)L[L.size() - 1] * L[L.size() - 2] : 0;
-    if ((largest > 2 * share && largest >= (long double)ODBA_W * 4096) || (env && !std::strcmp(env, "mixed"))) {
+    const long double f = (oe && !os) ? 2.0L : 1.0L;
+    if ((largest > f * share && largest >= (long double)ODBA_W * 4096) || (env && !std::strcmp(env, "mixed"))) {
       ps.mixed = true;
-      ps.mixed_cells = (long long)std::max<long double>(share, 1.0L);
+      ps.mixed_cells = (long long)std::max<long double>(f * share / 2, 1.0L);
     }
   }
   if (env && !std::strcmp(env, "wave") && ps.strips) {
