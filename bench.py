@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="serve: series-count and length scale (1.0: the serving configuration as named)")
-    ap.add_argument("--slots", type=int, default=16, help="serve: concurrent job slots")
+    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent job slots (8 measured best at full size)")
     ap.add_argument("--max-rounds", type=int, default=250, help="serve: DBA round limit")
     ap.add_argument("--dba-rounds", type=int, default=250,
                     help="squiggles: DBA round limit of the perform_dba run after the timed all-vs-all (0: skip)")

@@ -249,7 +249,7 @@ uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length)
  * of device memory), and run concurrently, each on its own HIP stream.
  * A job's result is the same alone and in a mix. Times are seconds since
  * the server was created: times[0] submit, [1] admission, [2] completion.
- * slots: 1..16. The slots' streams overlap only if the process has a
+ * slots: 1..32. The slots' streams overlap only if the process has a
  * hardware queue per stream: set GPU_MAX_HW_QUEUES >= slots (HIP's default
  * is 4) in the environment before the process first uses HIP.
  */

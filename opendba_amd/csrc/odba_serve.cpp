@@ -155,7 +155,7 @@ struct odba_server {
 extern "C" {
 
 int odba_server_create(int device, uint64_t hbm_budget, int slots, odba_server** out) {
-  if (!out || slots < 1 || slots > 16) return ODBA_E_INVALID;
+  if (!out || slots < 1 || slots > 32) return ODBA_E_INVALID;
   auto* s = new odba_server();
   s->device = device;
   s->slots = slots;

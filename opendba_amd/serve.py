@@ -18,7 +18,7 @@ from . import native as N
 
 
 class Server:
-    """Continuous-batching DBA server over odba_server_* (slots 1..16). The
+    """Continuous-batching DBA server over odba_server_* (slots 1..32). The
     slots' streams overlap only with one hardware queue each: the process
     must start HIP with GPU_MAX_HW_QUEUES >= slots (HIP's default is 4)."""
 
