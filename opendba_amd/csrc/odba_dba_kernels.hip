@@ -783,7 +783,7 @@ template <typename T> constexpr int stage_windows() { return sizeof(T) == 4 ? 32
 // windows of a batch of eight sequences, in sequence order, are loaded
 // together up to stage_windows<T>() of them (one memory latency per
 // batch); a batch whose ranges need more loads the rest one window at a
-// time as they are reached. buf holds (stage_windows<T>() + 1) * kStageW
+// time, each one window ahead of the adds. buf holds (stage_windows<T>() + 1) * kStageW
 // values; every lane of the block's one wave must call this.
 template <typename T>
 __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride, int nseq,
@@ -816,9 +816,7 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
     }
     // window k of the batch: sequence u with wb0[u] <= k < wb0[u] + nw[u],
     // values top, top-1, ... of that sequence (selected without indexing)
-    T w[NW][4];
-#pragma unroll
-    for (int k = 0; k < NW; k++) {
+    auto load_window = [&](int k, T* x) {
       int ktop = -1, klo = 0;
       long long koff = 0;
 #pragma unroll
@@ -831,9 +829,17 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int idx = ktop - (lane * 4 + e);
-        w[k][e] = (k < tot && idx >= klo) ? data[koff + idx] : T(0);
+        x[e] = (k < tot && idx >= klo) ? data[koff + idx] : T(0);
       }
-    }
+    };
+    T w[NW][4];
+#pragma unroll
+    for (int k = 0; k < NW; k++) load_window(k, w[k]);
+    // Windows past the first NW go through one more LDS window, each loaded
+    // into registers one window ahead: the load of window k + 1 is in
+    // flight while the lanes add window k.
+    T xn[4];
+    if (tot > NW) load_window(NW, xn);
 #pragma unroll
     for (int u = 0; u < B; u++) {
       const int sn = s0 + B + u;
@@ -854,18 +860,13 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
         const int slot = wb0[u] + wi;
         if (slot < NW) {
           wb = buf + slot * SW;
-        } else {  // past the batch's loaded windows: load this one now
-          T x[4];
+        } else {  // past the batch's loaded windows: the one loaded ahead
+          __syncthreads();  // every lane is done with the previous extra window
 #pragma unroll
-          for (int e = 0; e < 4; e++) {
-            const int idx = top - (lane * 4 + e);
-            x[e] = idx >= lo[u] ? data[o[u] + idx] : T(0);
-          }
-          __syncthreads();
-#pragma unroll
-          for (int e = 0; e < 4; e++) buf[NW * SW + lane * 4 + e] = x[e];
+          for (int e = 0; e < 4; e++) buf[NW * SW + lane * 4 + e] = xn[e];
           __syncthreads();
           wb = buf + NW * SW;
+          if (slot + 1 < tot) load_window(slot + 1, xn);
         }
         const int wlo = max(lo[u], top - SW + 1);
         if (r[u].y >= r[u].x) {
