@@ -87,6 +87,15 @@ struct RunRec {
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The traceback walk and the ordered accumulation are single dependent
+// chains per wave (one path, one position's additions in order) and use few
+// waves. When other jobs' path kernels share the SIMDs (several DBA jobs
+// served at once), those waves would get a fraction of the issue slots and
+// each chain would stretch accordingly; raising the wave's issue priority
+// inside the SIMD lets the chain issue first. Throughput kernels keep the
+// default priority, and results do not depend on it.
+__device__ __forceinline__ void chain_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their
 // maxima (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]).
 constexpr int kWaitVmcnt0 = 0x0F70;
@@ -238,6 +247,7 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
                                                    const uint32_t* __restrict__ mv, const uint32_t* __restrict__ mtop,
                                                    const int* __restrict__ abr, int4* __restrict__ runs, int lc, int os,
                                                    int oe, int* __restrict__ err, unsigned long long* __restrict__ pbuf) {
+  chain_priority();
   const int lane = threadIdx.x & 63;
   const int jid = wave_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (jid >= njobs) return;
@@ -917,6 +927,7 @@ __global__ void __launch_bounds__(64) k_accumulate(const T* __restrict__ data, c
                                                    const int4* __restrict__ runs, int nseq, int lc,
                                                    T* __restrict__ acc, uint32_t* __restrict__ counts, int first) {
   __shared__ T buf[(stage_windows<T>() + 1) * kStageW];
+  chain_priority();
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = c < lc;
   T a = (first || !on) ? T(0) : acc[c];
@@ -988,6 +999,7 @@ template <typename T>
 __global__ void __launch_bounds__(64) k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total,
                                                          T* __restrict__ acc, uint32_t* __restrict__ counts) {
   __shared__ T buf[(stage_windows<T>() + 1) * kStageW];
+  chain_priority();
   // No early exit: lanes past total take part with no sequences.
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = g < total;
