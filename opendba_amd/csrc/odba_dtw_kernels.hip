@@ -875,6 +875,14 @@ struct BandPipe {
   int* err;
   long long limit;
   mutable int last_pub = 0;
+  // Open-end stop of path jobs (k_path_swath_bands): the job's global stop
+  // word (lowest swath decided so far at which the sweep stops; INT_MAX:
+  // none) and this strip's swath. Null: no cancellation.
+  const int* stop = nullptr;
+  int swath = 0;
+  __device__ int stop_swath() const {
+    return uniform_i32(__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
 
   __device__ bool ready(int rows) const {
     if (in_global) {
@@ -892,6 +900,7 @@ struct BandPipe {
     if (band_failed(err)) return;
     const long long t0 = wall_clock64();
     while (!ready(rows)) {
+      if (cancelled()) return;  // the producer is at or past the stop too
       if (band_timed_out(t0, limit, err, lane)) return;
       if (in_global) __builtin_amdgcn_s_sleep(2);
       else __builtin_amdgcn_s_sleep(1);
@@ -914,7 +923,7 @@ struct BandPipe {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0) __hip_atomic_store(in_used, rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
-  __device__ bool cancelled() const { return false; }
+  __device__ bool cancelled() const { return stop && stop_swath() <= swath; }
   __device__ void reserve(int rows) const {
     if (!out_used) return;
     const int need = rows - ring;
@@ -924,6 +933,7 @@ struct BandPipe {
     for (;;) {
       const int u = uniform_i32(__hip_atomic_load(out_used, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
       if (u >= need) break;
+      if (stop && stop_swath() <= swath + 1) return;  // the consumer is cancelled: nothing reads the ring
       if (band_timed_out(t0, limit, err, lane)) return;
       __builtin_amdgcn_s_sleep(1);
     }
@@ -1463,24 +1473,38 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
     const bool has_out = s < K - 1;
     const bool in_global = (wid == 0) && s > 0;
     const bool out_global = (s == slast) && has_out;
+    BandPipe pp;
+    if (OE && !OS) {  // strips at or past a decided open-end stop are cancelled
+      pp.stop = &abr[j];
+      pp.swath = s;
+    }
+    // A strip already past the stop sweeps nothing; as a producer it takes
+    // no slot and announces none (-2), as a consumer it still returns its
+    // producer's slot once that producer is done with it.
+    bool skip = pp.cancelled();
     int gout = 0, gin = 0;
     if (out_global) {  // take a slot, reset its progress word, then announce it
-      gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
+      if (!skip) gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
       if (lane == 0) {
-        __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&slot_of[q], gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!skip) __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slot_of[q], skip ? -2 : gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (in_global && !band_failed(err)) {  // the left band's slot
+    bool had_slot = false;
+    if (in_global && !band_failed(err)) {  // the left band's slot, or -2: none
       const long long t0 = wall_clock64();
       for (;;) {
         gin = uniform_i32(__hip_atomic_load(&slot_of[q - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-        if (gin >= 0) break;
+        if (gin != -1) break;
         if (band_timed_out(t0, limit, err, lane)) { gin = 0; break; }
         __builtin_amdgcn_s_sleep(2);
       }
+      had_slot = gin >= 0;
+      if (!had_slot) {  // a producer past the stop: so is this strip
+        skip = true;
+        gin = 0;
+      }
     }
-    BandPipe pp;
     pp.in_global = in_global;
     pp.in_cnt = in_global ? &g_cnt[gin] : (s > 0 ? &s_cnt[wid - 1] : nullptr);
     pp.in_used = (s > 0 && !in_global) ? &s_used[wid - 1] : nullptr;
@@ -1503,12 +1527,32 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
     io.mv = mv + jb.mv_off + (size_t)s * swath_words;
     io.mtop = mtop + jb.top_off + (size_t)s * 64;
     io.ref_left = 1;
-    SwathRes<T> res = swath_dispatch_pipe<T, OS, OE, true, BandPipe>(C, io, &win, lane, pp);
-    if (in_global && lane == 0) {  // every read of the slot has returned: back to the pool
+    SwathRes<T> res;
+    if (!skip) res = swath_dispatch_pipe<T, OS, OE, true, BandPipe>(C, io, &win, lane, pp);
+    if (OE && !OS && out_global && !skip && lane == 0) {
+      // done with the slot (every row, or stopped past the stop): a consumer
+      // that stopped early returns the slot only after this mark
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(&g_cnt[gout], INT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (OE && !OS && had_slot && !band_failed(err)) {
+      // the producer may still be writing the slot if this strip stopped
+      // early (or never started): wait for its done mark
+      const long long t0 = wall_clock64();
+      for (;;) {
+        const int c = uniform_i32(__hip_atomic_load(&g_cnt[gin], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (c == INT_MAX) break;
+        if (band_timed_out(t0, limit, err, lane)) break;
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    if (had_slot && lane == 0) {  // every read of the slot has returned: back to the pool
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       atomicOr(&g_free[gin >> 6], 1ull << (gin & 63));
     }
-    if (OE && !OS && has_out && (s + 1) * ODBA_W > ly && res.top == res.colmin && lane == 0) atomicMin(&abr[j], s + 1);
+    if (OE && !OS && has_out && !skip && !pp.cancelled() && (s + 1) * ODBA_W > ly && res.top == res.colmin &&
+        lane == 0)
+      atomicMin(&abr[j], s + 1);
   }
 }
 
