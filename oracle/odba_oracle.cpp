@@ -16,10 +16,27 @@
  * matrix are indexed (i, j): i = row = position in the "first" sequence Y,
  * j = column = position in the "second" sequence X (dtw.hpp:36-38).
  *
- * What pins each function (tests/test_oracle.py):
+ * What pins each function (tests/test_oracle.py, test_oracle_open_modes.py,
+ * test_oracle_kseg_dp.py):
  *   dtw_swath / dtw_pair      textbook brute force (global mode, bit-exact),
  *                             transposition symmetry, the openDBA_test.cu KATs,
- *                             the test1 golden centroids
+ *                             the test1 golden centroids; open end, open start
+ *                             and open against closed forms over textbook and
+ *                             subsequence DTW matrices (distances and step
+ *                             codes), including the swath-left columns at
+ *                             small swath widths; the open-end early stop
+ *                             against the same run with the stop switched off
+ *                             (same distance and traceback counts)
+ *   KsegDP                    brute-force enumeration of every cut into k
+ *                             segments of span < e (optimal cost and the
+ *                             first-strict-minimum backtrack's choice)
+ *   kseg (search, medians)    closed forms only (constant and step signals,
+ *                             medians are input values): the k search and the
+ *                             median read past a segment's end are unpinned
+ *                             (the reference's threads sort neighbouring
+ *                             segments concurrently, so the value it reads
+ *                             past the end is a race there; this restatement
+ *                             reads the values as loaded)
  *   traceback_full            test1 golden centroids (exact in double),
  *                             Sigma count = path length minus open moves
  *   dba_update / perform_dba  openDBA_test.cu KATs (3 decimals) and the
@@ -133,13 +150,16 @@ struct SwathResult {
 template <typename T, bool kMoves, typename MoveFn>
 SwathResult dtw_swath(const T* y, size_t ly, const T* x, size_t lx, size_t W, size_t off,
                       int os, int oe, const T* prev, T* next, MoveFn&& mv, T* dist,
-                      std::vector<T>& colprev, std::vector<T>& colcur, std::vector<T>& row0) {
+                      std::vector<T>& colprev, std::vector<T>& colcur, std::vector<T>& row0,
+                      bool abrogate = true) {
   SwathResult res;
   const T MAXV = Lim<T>::max();
   if (off >= lx) { res.kind = 3; return res; }                         /* dtw.hpp:50-52 */
   const size_t wcols = std::min(W, lx - off);
-  /* Open-end early abrogation, checked only at swath starts (dtw.hpp:74-129). */
-  if (off > ly && oe && !os) {
+  /* Open-end early abrogation, checked only at swath starts (dtw.hpp:74-129).
+     abrogate == false is a test hook (the reference has no such switch): the
+     check is skipped so a test can compare the full recurrence with it. */
+  if (abrogate && off > ly && oe && !os) {
     if (prev[0] == MAXV) {                                               /* dtw.hpp:76-81 */
       if (kMoves) for (size_t t = 0; t < wcols; t++) mv(ly - 1, t, (uint8_t)ODBA_OPEN_RIGHT);
       res.kind = 1;
@@ -232,7 +252,7 @@ SwathResult dtw_swath(const T* y, size_t ly, const T* x, size_t lx, size_t W, si
  */
 template <typename T>
 int dtw_pair(const T* y, size_t ly, const T* x, size_t lx, size_t W, int os, int oe,
-             uint8_t* moves, T* dist, int* abrogated_swath, int* q3_flag) {
+             uint8_t* moves, T* dist, int* abrogated_swath, int* q3_flag, bool abrogate = true) {
   if (ly == 0 || lx == 0 || W == 0) return ODBA_E_INVALID;
   std::vector<T> prev(ly, T(0)), next(ly, T(0)), cp, cc, r0;
   if (abrogated_swath) *abrogated_swath = -1;
@@ -241,10 +261,10 @@ int dtw_pair(const T* y, size_t ly, const T* x, size_t lx, size_t W, int os, int
     SwathResult r;
     if (moves) {
       auto mv = [&](size_t i, size_t t, uint8_t c) { moves[i * lx + off + t] = c; };
-      r = dtw_swath<T, true>(y, ly, x, lx, W, off, os, oe, prev.data(), next.data(), mv, dist, cp, cc, r0);
+      r = dtw_swath<T, true>(y, ly, x, lx, W, off, os, oe, prev.data(), next.data(), mv, dist, cp, cc, r0, abrogate);
     } else {
       auto mv = [](size_t, size_t, uint8_t) {};
-      r = dtw_swath<T, false>(y, ly, x, lx, W, off, os, oe, prev.data(), next.data(), mv, dist, cp, cc, r0);
+      r = dtw_swath<T, false>(y, ly, x, lx, W, off, os, oe, prev.data(), next.data(), mv, dist, cp, cc, r0, abrogate);
     }
     if (r.kind == 2 && abrogated_swath) *abrogated_swath = (int)(off / W);
     q3_last = r.q3_undefined;
@@ -694,6 +714,79 @@ inline unsigned char d2u8(double d) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+/*
+ * The Bellman k-segmentation DP of one chunk for one segment count k
+ * (segmentation.hpp:172-190 tables, 194-262 recurrence and backtrack) on
+ * the chunk's 8-bit levels q[0..N_ds):
+ *   SUMS(l, r) / SQUARES(l, r): u16 / u32 running sums over spans r - l < e;
+ *   DIST(l, r) = SQUARES - SUMS*SUMS / (r - l + 1) (1-based, int arithmetic);
+ *   KD(1, n) = DIST(1, n) when n <= e, else INT_MAX / 2;
+ *   KD(p, n) = min over the last segment's start idx in
+ *              [max(n - e + 1, p), n] of KD(p - 1, idx - 1) + DIST(idx, n),
+ *              the first strict minimum in ascending idx, and
+ *   KP(p, n - 1) = idx - 1; the backtrack starts at bp[k] = N_ds.
+ * run() returns KD(k, N_ds) and writes bp[0..k] (bp[0] = 0).
+ */
+struct KsegDP {
+  int N_ds, e;
+  std::vector<unsigned short> sums;
+  std::vector<uint32_t> squares;
+  std::vector<uint32_t> kdist, options;
+  std::vector<unsigned short> kpath;
+  KsegDP(const unsigned char* q, int n_ds, int e_) : N_ds(n_ds), e(e_) {
+    sums.assign((size_t)std::max(1, N_ds) * std::max(1, e), 0);
+    squares.assign((size_t)std::max(1, N_ds) * std::max(1, e), 0);
+    for (int t = 0; t < N_ds; t++) {
+      S(t, t) = q[t];
+      Q(t, t) = (uint32_t)S(t, t) * (uint32_t)S(t, t);
+    }
+    for (int t = 0; t < N_ds; t++)
+      for (int sp = 1; sp < e; sp++)
+        if (t + sp < N_ds) {
+          S(t, t + sp) = (unsigned short)(S(t, t + sp - 1) + S(t + sp, t + sp));
+          Q(t, t + sp) = Q(t, t + sp - 1) + Q(t + sp, t + sp);
+        }
+    kdist.assign((size_t)2 * (N_ds + 1), 0);
+    options.assign((size_t)N_ds + 1, 0);
+  }
+  unsigned short& S(int l, int r) { return sums[(size_t)(r - l) + (size_t)e * l]; }
+  uint32_t& Q(int l, int r) { return squares[(size_t)(r - l) + (size_t)e * l]; }
+  uint32_t DIST(int l, int r) {  // 1-based (segmentation.hpp:31-35)
+    int a = l - 1, b = r - 1;
+    int s = S(a, b);
+    int ms = s * s / (b - a + 1);
+    return Q(a, b) - (uint32_t)ms;
+  }
+  uint32_t& KD(int p, int n) { return kdist[(size_t)N_ds * (p % 2) + n]; }
+  unsigned short& KP(int p, int n) { return kpath[(size_t)N_ds * (p - 1) + n]; }
+  uint32_t run(int test_k, unsigned short* bp) {
+    if (kpath.size() < (size_t)std::max(1, N_ds) * (size_t)(test_k + 1))
+      kpath.resize((size_t)std::max(1, N_ds) * (size_t)(test_k + 1), 0);
+    for (int t = 0; t < N_ds; t++) {
+      KD(1, t + 1) = (t + 1 <= e) ? DIST(1, t + 1) : (uint32_t)(INT_MAX / 2);
+      if (t < test_k) KP(t + 1, t) = (unsigned short)(test_k - 1);
+    }
+    for (int p = 2; p <= test_k; p++) {
+      for (int n = p; n <= N_ds; n++) {
+        for (int t = 0; t < N_ds; t++)
+          if (t + 1 >= p && t + 1 <= n && n - t - 1 < e) options[t + 1] = KD(p - 1, t) + DIST(t + 1, n);
+        int minval = INT_MAX, minidx = -1;
+        for (int idx = (n - e + 1 > p) ? n - e + 1 : p, start = idx; idx - start < e && idx <= n; idx++)
+          if (options[idx] < (uint32_t)minval) { minval = (int)options[idx]; minidx = idx; }
+        KD(p, n) = (uint32_t)minval;
+        KP(p, n - 1) = (unsigned short)((unsigned short)minidx - 1);
+      }
+    }
+    bp[test_k] = (unsigned short)N_ds;
+    for (int p = test_k - 1; p >= 1; p--) {
+      int n = bp[p + 1] - 1;
+      bp[p] = (n >= 0 && n < N_ds) ? KP(p + 1, n) : 0;
+    }
+    bp[0] = 0;
+    return N_ds >= 1 ? KD(test_k, N_ds) : 0;
+  }
+};
+
 template <typename T>
 void kseg_block(const T* series, int orig_N, int chunk, int spb, int mk, int minseg, int lds_bytes,
                 T* out /* mk slots of this chunk */) {
@@ -741,60 +834,21 @@ void kseg_block(const T* series, int orig_N, int chunk, int spb, int mk, int min
   off = align_up(off, sizeof(T)) + sizeof(T) * (size_t)(N_ds + 2);  // downsample_qtype + max/min
   off = align_up(off, 4);                             // squares start
   const int e = N_ds > 0 ? (int)(((long long)lds_bytes - (long long)off) / ((long long)N_ds * 6)) : 0;
-  // 8-bit rescale, cumulative sums and squares over spans < e
-  std::vector<unsigned short> sums((size_t)std::max(1, N_ds) * std::max(1, e), 0);
-  std::vector<uint32_t> squares((size_t)std::max(1, N_ds) * std::max(1, e), 0);
-  auto SUMS = [&](int l, int r) -> unsigned short& { return sums[(size_t)(r - l) + (size_t)e * l]; };
-  auto SQ = [&](int l, int r) -> uint32_t& { return squares[(size_t)(r - l) + (size_t)e * l]; };
+  // 8-bit rescale (segmentation.hpp:166-170)
+  std::vector<unsigned char> q((size_t)std::max(1, N_ds), 0);
   for (int t = 0; t < N_ds; t++) {
     /* quotient in T: integer division for integer T (segmentation.hpp:168) */
-    const T q = (T)((T)(dq[t] - Tmin) / (T)(Tmax - Tmin + 1));
-    SUMS(t, t) = d2u8(256.0 * (double)q);
-    SQ(t, t) = (uint32_t)SUMS(t, t) * (uint32_t)SUMS(t, t);
+    const T qt = (T)((T)(dq[t] - Tmin) / (T)(Tmax - Tmin + 1));
+    q[t] = d2u8(256.0 * (double)qt);
   }
-  for (int t = 0; t < N_ds; t++)
-    for (int sp = 1; sp < e; sp++)
-      if (t + sp < N_ds) {
-        SUMS(t, t + sp) = (unsigned short)(SUMS(t, t + sp - 1) + SUMS(t + sp, t + sp));
-        SQ(t, t + sp) = SQ(t, t + sp - 1) + SQ(t + sp, t + sp);
-      }
-  auto DIST = [&](int l, int r) -> uint32_t {  // 1-based (segmentation.hpp:31-35)
-    int a = l - 1, b = r - 1;
-    int s = SUMS(a, b);
-    int ms = s * s / (b - a + 1);
-    return SQ(a, b) - (uint32_t)ms;
-  };
-  std::vector<uint32_t> kdist((size_t)2 * (N_ds + 1), 0), options((size_t)N_ds + 1, 0);
-  std::vector<unsigned short> kpath((size_t)std::max(1, N_ds) * (size_t)(mk + 1), 0);
-  auto KD = [&](int p, int n) -> uint32_t& { return kdist[(size_t)N_ds * (p % 2) + n]; };
-  auto KP = [&](int p, int n) -> unsigned short& { return kpath[(size_t)N_ds * (p - 1) + n]; };
+  KsegDP dp(q.data(), N_ds, e);
   std::vector<unsigned short> bp((size_t)mk + 1, 0);
   bool ex = false;
   short smallest_noisy = (short)(expected_k + 1);
   short test_k = (short)((expected_k + 1) / 2);
   short delta = (short)(test_k / 2);
   while (!ex) {
-    for (int t = 0; t < N_ds; t++) {
-      KD(1, t + 1) = (t + 1 <= e) ? DIST(1, t + 1) : (uint32_t)(INT_MAX / 2);
-      if (t < test_k) KP(t + 1, t) = (unsigned short)(test_k - 1);
-    }
-    for (int p = 2; p <= test_k; p++) {
-      for (int n = p; n <= N_ds; n++) {
-        for (int t = 0; t < N_ds; t++)
-          if (t + 1 >= p && t + 1 <= n && n - t - 1 < e) options[t + 1] = KD(p - 1, t) + DIST(t + 1, n);
-        int minval = INT_MAX, minidx = -1;
-        for (int idx = (n - e + 1 > p) ? n - e + 1 : p, start = idx; idx - start < e && idx <= n; idx++)
-          if (options[idx] < (uint32_t)minval) { minval = (int)options[idx]; minidx = idx; }
-        KD(p, n) = (uint32_t)minval;
-        KP(p, n - 1) = (unsigned short)((unsigned short)minidx - 1);
-      }
-    }
-    bp[test_k] = (unsigned short)N_ds;
-    for (int p = test_k - 1; p >= 1; p--) {
-      int n = bp[p + 1] - 1;
-      bp[p] = (n >= 0 && n < N_ds) ? KP(p + 1, n) : 0;
-    }
-    bp[0] = 0;
+    dp.run(test_k, bp.data());
     bool noisy = false;
     for (int i = 1; i <= test_k; i++)
       if ((int)bp[i] - (int)bp[i - 1] < minseg / ds) noisy = true;
@@ -906,6 +960,14 @@ int orc_dtw_pair(int dtype, const void* y, uint64_t ly, const void* x, uint64_t 
   ORC_DISPATCH(dtype, dtw_pair<T>((const T*)y, ly, (const T*)x, lx, w, os, oe, moves, (T*)dist, abrogated_swath, q3_flag))
 }
 
+/* Test hook: dtw_pair with the open-end abrogation check switched off
+ * (abrogate = 0), everything else as orc_dtw_pair. */
+int orc_dtw_pair_abr(int dtype, const void* y, uint64_t ly, const void* x, uint64_t lx, uint64_t w, int os, int oe,
+                     int abrogate, uint8_t* moves, void* dist, int* abrogated_swath) {
+  ORC_DISPATCH(dtype, dtw_pair<T>((const T*)y, ly, (const T*)x, lx, w, os, oe, moves, (T*)dist, abrogated_swath,
+                                  (int*)nullptr, abrogate != 0))
+}
+
 int orc_all_pairs(int dtype, const void* const* seqs, const uint64_t* lens, uint64_t n, int os, int oe, uint64_t w,
                   void* out, int nthreads, const int64_t* subset, uint64_t nsubset) {
   ORC_DISPATCH(dtype, all_pairs<T>((const T* const*)seqs, lens, n, os, oe, w, (T*)out, nthreads, subset, nsubset))
@@ -957,6 +1019,15 @@ int orc_prefix_chop(int dtype, const void* prefix, uint64_t P, const void* seq, 
 }
 
 /* out must hold sum over sequences of mk * ceil(L / (256 * ds)) values. */
+/* Test hook: the k-segmentation DP of one chunk (KsegDP) on given 8-bit
+ * levels q[0..n_ds) with extent e and segment count k; bp needs k + 1 slots. */
+int orc_kseg_dp(const unsigned char* q, int n_ds, int e, int k, uint32_t* cost, unsigned short* bp) {
+  if (n_ds < 1 || e < 1 || k < 1 || k > n_ds) return ODBA_E_INVALID;
+  KsegDP dp(q, n_ds, e);
+  *cost = dp.run(k, bp);
+  return ODBA_OK;
+}
+
 int orc_kseg(int dtype, const void* const* seqs, const uint64_t* lens, uint64_t n, int minseg, int prefix_skip,
              int lds_bytes, void* out, uint64_t* out_lens, uint64_t* out_offs) {
   ORC_DISPATCH(dtype, kseg<T>((const T* const*)seqs, lens, n, minseg, prefix_skip, lds_bytes, (T*)out, out_lens, out_offs))
