@@ -70,6 +70,34 @@ int odba_dtw_all_pairs_resident(odba_ctx* ctx, odba_seqset* set, int use_open_st
                                 void* dist_out, int* flags_out, double* kernel_ms);
 
 /*
+ * Bytes of the fixed per-device array odba_dtw_all_pairs_noalloc works in:
+ * 256 for its pair counter, the rest for two edge columns of
+ * (longest length + 256) values per resident wave, four waves per
+ * workgroup. Every device the library runs on holds it until the process
+ * ends.
+ */
+#define ODBA_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/*
+ * odba_dtw_all_pairs_resident with no device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). dist_dev (n(n-1)/2 values of the set's type, same index as
+ * odba_dtw_all_pairs) and flags_dev (optional, n(n-1)/2 ints, bits as for
+ * odba_dtw_all_pairs_resident) are DEVICE pointers on the context's device;
+ * every element is written. The pairs run one wave each, with as many waves
+ * resident as ODBA_NOALLOC_SCRATCH_BYTES holds; a set whose longest sequence
+ * needs more than that for one workgroup (above about 2M samples of float,
+ * 1M of double) is refused with ODBA_E_INVALID before anything runs.
+ * Results equal odba_dtw_all_pairs_resident's. The array is shared by every
+ * call on one device and keeps the last call's contents: calls that overlap
+ * on one device (two contexts or threads at once) race and must not be
+ * made. Returns when the distances are written.
+ */
+int odba_dtw_all_pairs_noalloc(odba_ctx* ctx, odba_seqset* set, int use_open_start, int use_open_end, void* dist_dev,
+                               int* flags_dev);
+
+/*
  * DBA rounds of this context are replayed as hipGraphs after a few eager
  * rounds (default on; no counterpart in the reference). Measured 1-3%
  * faster alone on a device, but slower when several contexts keep the same

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -55,7 +56,18 @@ struct odba_seqset {
   void* order = nullptr;       // device pair order (int2 per pair), largest cell count first
 };
 
+// Fixed device scratch of odba_dtw_all_pairs_noalloc: the pair counter in
+// the first 256 bytes, then the edge columns of the resident waves. Each
+// device holds its own copy from the module's load until the process ends.
+// In host code `scratch` names a host-side stand-in, not a device address:
+// only the per-device address looked up by odba_ctx_create is passed on.
+static __device__ __align__(256) unsigned char scratch[ODBA_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
 
 size_t dtype_size(int dt) {
   switch (dt) {
@@ -158,6 +170,17 @@ int odba_ctx_create(int device, odba_ctx** out) {
   c->name = p.gcnArchName;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ODBA_E_HIP; }
   if (hipMalloc(&c->small, 256) != hipSuccess) { hipStreamDestroy(c->stream); delete c; return ODBA_E_NOMEM; }
+  if (device < kMaxDevices) {
+    // once per device, with that device current: the fixed scratch's device
+    // address (a failure leaves it unset; odba_dtw_all_pairs_noalloc then
+    // reports it and every other entry point works as before)
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_dev[device] = p;
+      else (void)hipGetLastError();
+    }
+  }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
   *out = c;
@@ -565,6 +588,35 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   hipFree(dist);
   if (flags) hipFree(flags);
   return rc;
+}
+
+int odba_dtw_all_pairs_noalloc(odba_ctx* c, odba_seqset* s, int os, int oe, void* dist_dev, int* flags_dev) {
+  if (!c || !s || s->ctx != c) return fail(c, ODBA_E_INVALID, "invalid sequence set");
+  if (!dist_dev) return fail(c, ODBA_E_INVALID, "no distance buffer");
+  HIPC(c, hipSetDevice(c->device));
+  const int n = s->n;
+  if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "need at least two sequences");
+  void* base = c->device < kMaxDevices ? g_scratch_dev[c->device] : nullptr;
+  if (!base) return fail(c, ODBA_E_HIP, "the fixed scratch of device %d is unavailable", c->device);
+  const long long npairs = (long long)n * (n - 1) / 2;
+  // one wave per pair, as many resident waves as the fixed scratch holds
+  // (two edge columns of stride values per wave, four waves per workgroup)
+  const long long stride = s->maxlen + 256;
+  const size_t per_block = (size_t)4 * 2 * stride * dtype_size(s->dtype);
+  const size_t avail = (size_t)ODBA_NOALLOC_SCRATCH_BYTES - 256;
+  if (per_block > avail)
+    return fail(c, ODBA_E_INVALID,
+                "longest sequence (%d samples) needs %zu bytes of edge columns per workgroup; the fixed scratch holds %zu",
+                s->maxlen, per_block, avail);
+  const int nb = (int)std::min<long long>(grid_blocks(c, npairs), (long long)(avail / per_block));
+  unsigned long long* counter = (unsigned long long*)base;
+  // cleared by a kernel, not a memset node, so the call can sit in a graph capture
+  HIPC(c, launch_fill_u32(c->stream, counter, 0u, 2));
+  hipError_t e = launch_all_pairs(s->dtype, s->int_no_wrap, os, oe, nb, c->stream, s->data, s->offs, s->lens, n, npairs,
+                                  counter, dist_dev, (char*)base + 256, stride, flags_dev, nullptr);
+  if (e != hipSuccess) return fail(c, ODBA_E_HIP, "all-pairs launch: %s", hipGetErrorString(e));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ODBA_OK;
 }
 
 int odba_tune_all_pairs(odba_ctx* c, odba_seqset* s, int os, int oe, int reps, odba_schedule* best, double* best_ms,

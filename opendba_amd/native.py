@@ -26,7 +26,7 @@ EXPORTS = [
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
     "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
-    "odba_ctx_set_graphs",
+    "odba_ctx_set_graphs", "odba_dtw_all_pairs_noalloc",
 ]
 SCHED_WAVE, SCHED_BANDS = 0, 1
 
@@ -67,6 +67,7 @@ def load():
     L.odba_seqset_create.argtypes = [vp, i32, vp, vp, i32, C.POINTER(vp)]
     L.odba_seqset_destroy.argtypes = [vp]
     L.odba_dtw_all_pairs_resident.argtypes = [vp, vp, i32, i32, vp, vp, C.POINTER(C.c_double)]
+    L.odba_dtw_all_pairs_noalloc.argtypes = [vp, vp, i32, i32, vp, vp]
     L.odba_dtw_path.argtypes = [vp, i32, vp, u64, vp, u64, i32, i32, vp, vp]
     L.odba_dba_round.argtypes = [vp, i32, vp, u64, vp, vp, i32, i32, i32, i32, u64, vp, vp, C.POINTER(C.c_double)]
     L.odba_znorm.argtypes = [vp, i32, vp, vp, i32, vp, vp]
