@@ -1483,11 +1483,17 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
     // producer's slot once that producer is done with it.
     bool skip = pp.cancelled();
     int gout = 0, gin = 0;
+    // took: this strip announced a slot, so its consumer waits for the
+    // slot's done mark; it must write that mark even if it turns to skip
+    // below (one wave per band is both the consumer of the left band's slot
+    // and the producer of its own, and may learn only after announcing that
+    // its left neighbour was past the stop)
+    const bool took = out_global && !skip;
     if (out_global) {  // take a slot, reset its progress word, then announce it
-      if (!skip) gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
+      if (took) gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
       if (lane == 0) {
-        if (!skip) __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&slot_of[q], skip ? -2 : gout, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (took) __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slot_of[q], took ? gout : -2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     bool had_slot = false;
@@ -1529,9 +1535,10 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
     io.ref_left = 1;
     SwathRes<T> res;
     if (!skip) res = swath_dispatch_pipe<T, OS, OE, true, BandPipe>(C, io, &win, lane, pp);
-    if (OE && !OS && out_global && !skip && lane == 0) {
-      // done with the slot (every row, or stopped past the stop): a consumer
-      // that stopped early returns the slot only after this mark
+    if (OE && !OS && took && lane == 0) {
+      // done with the slot (every row, stopped past the stop, or never
+      // started): a consumer that stopped early returns the slot only after
+      // this mark
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __hip_atomic_store(&g_cnt[gout], INT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
