@@ -143,7 +143,9 @@ int odba_dtw_path(odba_ctx* ctx, int dtype, const void* y, uint64_t ly, const vo
  * max_delta = max |old - new| (dba.hpp:830-836). counts_out is optional.
  * path_policy: ODBA_PATH_FULL, ODBA_PATH_STRIPE or ODBA_PATH_AUTO with the
  * free-memory figure ref_free_mem the reference's rule compares against
- * (dba.hpp:523-548).
+ * (dba.hpp:523-548); with ODBA_PATH_AUTO and ref_free_mem == 0 the figure is
+ * the device's free memory once the sequences are resident (capped at the
+ * context's memory cap), as the reference reads it with cudaMemGetInfo.
  */
 int odba_dba_round(odba_ctx* ctx, int dtype, const void* centroid, uint64_t lc, const void* const* seqs,
                    const uint64_t* lens, int n, int use_open_start, int use_open_end, int path_policy,

@@ -49,6 +49,14 @@
 #define CANNOT_READ_SEQUENCE_PREFIX_FILE 24
 #define CANNOT_COPY_PREFIX_CHOPPED_SEQ 25
 #define CANNOT_WRITE_DTW_PATH_MATRIX 26
+#define CANNOT_READ_MEMBERSHIP 43
+#define MEMBERSHIP_FILE_FORMAT_VIOLATION 44
+
+/* HIP's stream handle (hip_runtime_api.h declares the same typedef). The
+ * reference's entry points take a CUDA stream as their last argument; here
+ * that slot takes a hipStream_t or 0. The library orders its work on its own
+ * context's stream, so the value is accepted and not used. */
+typedef struct ihipStream_t* hipStream_t;
 
 namespace odba_host {
 
@@ -77,6 +85,34 @@ inline void check(int rc, const char* what) {
     std::cerr << "OpenDBA error in " << what << ": " << odba_last_error(ctx()) << " (code " << rc << ")" << std::endl;
     std::exit(rc);
   }
+}
+
+/* The free-memory figure of the stripe-mode rule (dba.hpp:523-548): 0 lets
+ * the library read the device's free memory; ODBA_REF_FREE_MEM (bytes)
+ * states one instead (tests force stripe mode with it). */
+inline uint64_t ref_free_mem() {
+  const char* v = std::getenv("ODBA_REF_FREE_MEM");
+  return (v && *v) ? (uint64_t)std::strtoull(v, nullptr, 10) : 0;
+}
+
+/* Shared-memory budget the k-segmentation tables are sized with
+ * (segmentation.hpp:458-462: the CUDA device's opt-in maximum per block):
+ * ODBA_KSEG_LDS_BYTES, else 64 KiB. */
+inline int kseg_lds_bytes() {
+  const char* v = std::getenv("ODBA_KSEG_LDS_BYTES");
+  return (v && *v) ? std::atoi(v) : 65536;
+}
+
+inline bool file_exists(const std::string& name) {  // io_utils.hpp:34-37
+  std::ifstream f(name.c_str());
+  return f.good();
+}
+
+inline std::vector<std::string> split(const std::string& line, char delim) {  // io_utils.hpp:40-47
+  std::vector<std::string> out;
+  std::stringstream ss(line);
+  for (std::string tok; std::getline(ss, tok, delim);) out.push_back(tok);
+  return out;
 }
 
 inline char* dup_name(const char* s) {
@@ -223,14 +259,25 @@ int readSequenceTSVFiles(char** filenames, int num_files, T*** sequences, char**
 // ---------------------------------------------------------------------------
 // Hot-path entry points
 // ---------------------------------------------------------------------------
+/* normalizeSequences<T> (gpu_utils.hpp:131-175): z-normalises in place;
+ * sequence_means / sequence_sigmas (optional) receive mu and sigma. */
 template <typename T>
 void normalizeSequences(T** sequences, size_t num_sequences, size_t* sequence_lengths, int refSequenceIndex,
-                        double* sequence_means, double* sequence_sigmas) {
+                        double* sequence_means, double* sequence_sigmas, hipStream_t stream = 0) {
   (void)refSequenceIndex;  // the reference's pipeline always z-normalises (-1, gpu_utils.hpp:154-157)
+  (void)stream;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + num_sequences);
   odba_host::check(odba_znorm(odba_host::ctx(), odba_host::DType<T>::v, (void* const*)sequences, l.data(),
                               (int)num_sequences, sequence_means, sequence_sigmas),
                    "normalizeSequences");
+}
+
+/* The overload without mu / sigma (gpu_utils.hpp:173-175). */
+template <typename T>
+void normalizeSequences(T** sequences, size_t num_sequences, size_t* sequence_lengths, int refSequenceIndex,
+                        hipStream_t stream = 0) {
+  normalizeSequences<T>(sequences, num_sequences, sequence_lengths, refSequenceIndex, (double*)nullptr,
+                        (double*)nullptr, stream);
 }
 
 /* chopPrefixFromSequences<T> (dba.hpp:1192-1395): locates the prefix in
@@ -245,7 +292,7 @@ void normalizeSequences(T** sequences, size_t num_sequences, size_t* sequence_le
 template <typename T>
 void chopPrefixFromSequences(T* sequence_prefix, size_t sequence_prefix_length, T** sequences, int* num_sequences,
                              size_t* sequence_lengths, char** sequence_names, char* output_prefix,
-                             int norm_sequences, const void* stream = nullptr) {
+                             int norm_sequences, hipStream_t stream = 0) {
   (void)stream;
   const int n = *num_sequences;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + n);
@@ -336,22 +383,20 @@ void write_dba_paths(odba_dba* dba, const std::string& prefix, T* const* seqs, c
   }
 }
 
-}  // namespace odba_host
-
-/* Medoid search on sequences already sorted by length (dba.hpp:36-365):
- * all-vs-all distances, <prefix>.pair_dists.txt, sum of squares in T, one
- * cluster when *cdist == 1. Returns new int[num_clusters]. */
+/* The medoid search of approximateMedoidIndices (dba.hpp:36-365) on
+ * sequences already sorted by length: all-vs-all distances,
+ * <prefix>.pair_dists.txt, sum of squares in T, clusters by cdist (> 1: K,
+ * == 1: one, [0, 1): height cut) and their medoids by the reference's rules.
+ * Returns new int[num_clusters]. */
 template <typename T>
-int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* sequence_lengths, char** sequence_names,
-                              int use_open_start, int use_open_end, char* output_prefix, double* cdist,
-                              int* memberships, const void* stream = nullptr) {
-  (void)stream;
+int* medoids_of(T* const* sequences, size_t num_sequences, const size_t* sequence_lengths, char* const* sequence_names,
+                int use_open_start, int use_open_end, const char* output_prefix, double* cdist, int* memberships) {
   const size_t n = num_sequences;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + n);
   std::vector<T> d(n * (n - 1) / 2);
-  odba_host::check(odba_dtw_all_pairs(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)sequences, l.data(),
-                                      (int)n, use_open_start, use_open_end, ODBA_REF_W, d.data()),
-                   "approximateMedoidIndices");
+  check(odba_dtw_all_pairs(ctx(), DType<T>::v, (const void* const*)sequences, l.data(), (int)n, use_open_start,
+                           use_open_end, ODBA_REF_W, d.data()),
+        "approximateMedoidIndices");
   std::vector<T> sos(n, T(0));
   std::ofstream mats((std::string(output_prefix) + ".pair_dists.txt").c_str());
   size_t off = 0;
@@ -372,12 +417,10 @@ int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* seque
   mats << sequence_names[n - 1];
   for (size_t pad = 0; pad < n; ++pad) mats << "\t";
   mats << "0" << std::endl;
-  // clusters (cdist > 1: K, == 1: one, [0,1): height cut) and their medoids
-  // by the reference's rules (dba.hpp:219-351)
   std::vector<int> med(n > 0 ? n : 1);
   int k = 0;
-  odba_host::check(odba_cluster(odba_host::DType<T>::v, d.data(), (int)n, l.data(), *cdist, memberships, med.data(), &k),
-                   "approximateMedoidIndices");
+  check(odba_cluster(DType<T>::v, d.data(), (int)n, l.data(), *cdist, memberships, med.data(), &k),
+        "approximateMedoidIndices");
   std::cerr << "There are " << k << " clusters" << std::endl;
   int* medoids = new int[k];
   for (int c = 0; c < k; c++) {
@@ -387,19 +430,39 @@ int* approximateMedoidIndices(T** sequences, size_t num_sequences, size_t* seque
   return medoids;
 }
 
+}  // namespace odba_host
+
+/* approximateMedoidIndices<T> (dba.hpp:36-365): gpu_sequences holds the
+ * num_sequences sequences (already sorted by length) evenly spaced,
+ * sequence i at gpu_sequences + i * maxSeqLength, as performDBA lays them
+ * out (dba.hpp:919-923). Writes <output_prefix>.pair_dists.txt, fills
+ * memberships[num_sequences] and returns new int[num_clusters] medoids. */
+template <typename T>
+int* approximateMedoidIndices(T* gpu_sequences, size_t maxSeqLength, size_t num_sequences, size_t* sequence_lengths,
+                              char** sequence_names, int use_open_start, int use_open_end, char* output_prefix,
+                              double* cdist, int* memberships, hipStream_t stream = 0) {
+  (void)stream;
+  std::vector<T*> p(num_sequences);
+  for (size_t i = 0; i < num_sequences; i++) p[i] = gpu_sequences + i * maxSeqLength;
+  return odba_host::medoids_of<T>(p.data(), num_sequences, sequence_lengths, sequence_names, use_open_start,
+                                  use_open_end, output_prefix, cdist, memberships);
+}
+
 /* One DBA round (dba.hpp:432-848): the new centroid in updatedMean, the
  * path files <output_prefix>.path<k>.txt; returns the max centroid
- * movement. */
+ * movement. Each sequence takes the full path matrix or stripe mode by the
+ * reference's free-memory rule (dba.hpp:523-548). */
 template <typename T>
 double DBAUpdate(T* C, size_t centerLength, T** sequences, char** sequence_names, size_t num_sequences,
                  size_t* sequence_lengths, int use_open_start, int use_open_end, T* updatedMean,
-                 std::string output_prefix, const void* stream = nullptr) {
+                 std::string output_prefix, hipStream_t stream = 0) {
   (void)stream;
   std::vector<uint64_t> l(sequence_lengths, sequence_lengths + num_sequences);
   double delta = 0;
   odba_dba* dba = nullptr;
   odba_host::check(odba_dba_create(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)sequences, l.data(),
-                                   (int)num_sequences, use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
+                                   (int)num_sequences, use_open_start, use_open_end, ODBA_PATH_AUTO,
+                                   odba_host::ref_free_mem(), &dba),
                    "DBAUpdate");
   odba_host::check(odba_dba_set_centroid(dba, C, centerLength), "DBAUpdate");
   odba_host::check(odba_dba_record_paths(dba, 1), "DBAUpdate");
@@ -421,28 +484,122 @@ void writeCentroidCheckpointToFile(const char* name, const T* c, size_t len) {  
 
 template <typename T>
 int readCentroidCheckpointFromFile(const char* name, T* c, size_t len) {  // io_utils.hpp:80-115
+  if (!odba_host::file_exists(name)) return 0;
   std::ifstream f(name);
-  if (!f.is_open()) return 0;
   std::string line;
   if (!std::getline(f, line)) return 0;
-  std::vector<std::string> vals;
-  std::stringstream ss0(line);
-  for (std::string tok; std::getline(ss0, tok, ' ');) vals.push_back(tok);
-  if (vals.size() != len) return 0;
+  if (odba_host::split(line, ' ').size() != len) return 0;
+  std::cerr << "Resuming convergence from partially converged centroid in file " << name << std::endl;
   std::stringstream ss(line);
   for (size_t i = 0; i < len; i++) ss >> c[i];
   return 1;
 }
 
-/* performDBA<T> (dba.hpp:863-1189): medoids and clusters by cdist, then
- * every cluster's consensus (packed rounds on the device), written to
- * <prefix>.avg.txt with the membership file and per-cluster path files.
- * Reorders sequences / names / lengths in place by stable length sort and
- * z-normalises in place when norm_sequences, as the reference does. */
+/* readMedoidIndices (io_utils.hpp:160-240): memberships by sequence name
+ * from a membership file (a '#' comment line, then name, cluster, medoid
+ * name per line); returns new int[] of each cluster's medoid index (the
+ * sequence whose name is its own medoid's). Exits with the reference's
+ * codes on a missing or malformed file. Sequences the file does not name
+ * keep membership 0; a cluster whose medoid is not among sequence_names
+ * exits with MEDOID_FINDING_ERROR (the reference reads an unset entry). */
+inline int* readMedoidIndices(const char* membership_filename, int num_sequences, char** sequence_names,
+                              int* sequences_membership) {
+  std::ifstream f(membership_filename);
+  if (!f.is_open()) {
+    std::cerr << "Cannot open sequence cluster membership file " << membership_filename << " for reading" << std::endl;
+    std::exit(CANNOT_READ_MEMBERSHIP);
+  }
+  std::string line;
+  std::getline(f, line);
+  if (line.compare(0, 1, "#")) {
+    std::cerr << "The existing sequence cluster membership file " << membership_filename << " has a first line "
+              << "without the expected '#' comment start" << std::endl;
+    std::exit(MEMBERSHIP_FILE_FORMAT_VIOLATION);
+  }
+  auto index_of = [&](const std::string& nm) {
+    for (int i = 0; i < num_sequences; i++)
+      if (nm == sequence_names[i]) return i;
+    return -1;
+  };
+  for (int i = 0; i < num_sequences; i++) sequences_membership[i] = 0;
+  int max_cluster = 0, line_number = 1;
+  std::vector<std::pair<int, std::string>> medoid_rows;
+  while (std::getline(f, line)) {
+    line_number++;
+    const std::vector<std::string> v = odba_host::split(line, '\t');
+    if (v.size() != 3) {
+      std::cerr << "The existing sequence cluster membership file " << membership_filename << " has a line (#"
+                << line_number << ") without the expected three columns (found " << v.size() << ")" << std::endl;
+      std::exit(MEMBERSHIP_FILE_FORMAT_VIOLATION);
+    }
+    int cluster = 0;
+    try {
+      cluster = std::stoi(v[1]);
+    } catch (const std::exception& ex) {
+      std::cerr << "The existing sequence cluster membership file " << membership_filename << " has a line (#"
+                << line_number << ") where the second tab-delimited column value (" << v[1]
+                << ") is not an integer as expected: " << ex.what() << std::endl;
+      std::exit(MEMBERSHIP_FILE_FORMAT_VIOLATION);
+    }
+    const int idx = index_of(v[0]);
+    if (idx == -1) {
+      std::cerr << "The existing sequence cluster membership file " << membership_filename << " has a line (#"
+                << line_number << ") with a sequence name not found in the input (" << v[0]
+                << " not in existing list of " << num_sequences << " names)" << std::endl;
+      std::exit(MEMBERSHIP_FILE_FORMAT_VIOLATION);
+    }
+    max_cluster = std::max(max_cluster, cluster);
+    sequences_membership[idx] = cluster;
+    if (v[0] == v[2]) medoid_rows.push_back(std::make_pair(cluster, v[0]));
+  }
+  int* medoids = new int[max_cluster + 1];
+  for (int c = 0; c <= max_cluster; c++) medoids[c] = -1;
+  for (const auto& mr : medoid_rows) medoids[mr.first] = index_of(mr.second);
+  for (int c = 0; c <= max_cluster; c++)
+    if (medoids[c] < 0) {
+      std::cerr << "The existing sequence cluster membership file " << membership_filename << " names no medoid for "
+                << "cluster " << c << std::endl;
+      std::exit(MEDOID_FINDING_ERROR);
+    }
+  return medoids;
+}
+
+/* writeSequences (io_utils.hpp:242-261): one line per sequence, its name
+ * then its values, tab-separated. */
+template <typename T>
+int writeSequences(T** cpu_sequences, size_t* seq_lengths, char** seq_names, int num_seqs, const char* filename) {
+  std::ofstream out(filename);
+  if (!out.is_open()) {
+    std::cerr << "Cannot write to " << filename << std::endl;
+    return CANNOT_WRITE_DTW_PATH_MATRIX;
+  }
+  for (int i = 0; i < num_seqs; i++) {
+    out << seq_names[i] << "\t" << cpu_sequences[i][0];
+    for (size_t j = 1; j < seq_lengths[i]; j++) out << "\t" << cpu_sequences[i][j];
+    out << std::endl;
+  }
+  return 0;
+}
+
+/* performDBA<T> (dba.hpp:863-1189): stable length sort of sequences /
+ * names / lengths in place, z-norm in place when norm_sequences, then by
+ * algo_mode:
+ *   CLUSTER_AND_CONSENSUS, CLUSTER_ONLY: medoids and clusters by cdist
+ *     (<prefix>.pair_dists.txt; <prefix>.cluster_membership.txt when
+ *     cdist != 1); CLUSTER_ONLY stops there;
+ *   CONSENSUS_ONLY: the memberships and medoids of a previous call, read
+ *     by sequence name from <prefix>.cluster_membership.txt;
+ * then every cluster's consensus (at most 250 rounds, delta == 0 or the
+ * open-mode flip-flop stop), rescaled by its medoid's mu / sigma, appended
+ * to <prefix>.avg.txt, with per-cluster path files and the evolving-centroid
+ * checkpoint <prefix>.<cluster>.evolving_centroid.txt read back at the
+ * cluster's start. Resume rule of the reference's default build
+ * (dba.hpp:981-984, io_utils.hpp:121-123): when <prefix>.avg.txt already
+ * exists, the run starts at cluster index 1. */
 template <typename T>
 void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char** sequence_names, int use_open_start,
                 int use_open_end, char* output_prefix, int norm_sequences, double cdist, char** series_file_names,
-                int num_series, int read_mode, bool is_segmented, int algo_mode, const void* stream = nullptr) {
+                int num_series, int read_mode, bool is_segmented, int algo_mode, hipStream_t stream = 0) {
   (void)series_file_names; (void)num_series; (void)read_mode; (void)is_segmented; (void)stream;
   const int n = num_sequences;
   for (int i = 0; i < n; i++)  // dba.hpp:868-872
@@ -460,17 +617,20 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
   if (norm_sequences) normalizeSequences<T>(sequences, n, sequence_lengths, -1, mu.data(), sg.data());
   std::vector<int> memb(n, 0);
   int* medoids = nullptr;
+  const std::string prefix(output_prefix);
   if (algo_mode == CLUSTER_AND_CONSENSUS || algo_mode == CLUSTER_ONLY) {
-    medoids = approximateMedoidIndices<T>(sequences, n, sequence_lengths, sequence_names, use_open_start, use_open_end,
-                                          output_prefix, &cdist, memb.data());
+    medoids = odba_host::medoids_of<T>(sequences, n, sequence_lengths, sequence_names, use_open_start, use_open_end,
+                                       output_prefix, &cdist, memb.data());
+  } else if (algo_mode == CONSENSUS_ONLY) {
+    std::cerr << "Reading previous clustering data" << std::endl;
+    medoids = readMedoidIndices((prefix + ".cluster_membership.txt").c_str(), n, sequence_names, memb.data());
   } else {
-    std::cerr << "Call to performDBA included an unsupported algorithm mode " << algo_mode << std::endl;
+    std::cerr << "Call to performDBA included an unrecognized algorithm mode " << algo_mode << std::endl;
     std::exit(UNKNOWN_ALGO);
   }
-  const std::string prefix(output_prefix);
   int nclusters = 1;
   for (int i = 0; i < n; i++) nclusters = std::max(nclusters, memb[i] + 1);
-  if (cdist != 1) {  // dba.hpp:951-963
+  if (cdist != 1 && algo_mode != CONSENSUS_ONLY) {  // dba.hpp:951-963
     std::ofstream mf((prefix + ".cluster_membership.txt").c_str());
     if (!mf.is_open()) {
       std::cerr << "Cannot open sequence cluster membership file " << prefix << ".cluster_membership.txt for writing"
@@ -482,13 +642,11 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
     std::cerr << "Found " << nclusters << " clusters using complete linkage and cluster distance cutoff " << cdist << std::endl;
   }
   if (algo_mode == CLUSTER_ONLY) { delete[] medoids; return; }
-  // resume after the clusters whose averages are already in avg.txt (dba.hpp:983-988)
   int first = 0;
-  {
-    std::ifstream prev((prefix + ".avg.txt").c_str());
-    for (std::string line; std::getline(prev, line);)
-      if (!line.empty()) first++;
-    if (first) std::cerr << "Restarting convergence with cluster " << (first + 1) << "/" << nclusters << std::endl;
+  if (odba_host::file_exists(prefix + ".avg.txt")) {  // dba.hpp:981-985
+    first = 1;
+    std::cerr << "Restarting convergence with cluster " << (first + 1) << "/" << nclusters << " based on checkpoint in "
+              << prefix << ".avg.txt" << std::endl;
   }
   std::ofstream avgs((prefix + ".avg.txt").c_str(), std::ios::app);
   if (!avgs.is_open()) {
@@ -511,7 +669,8 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
       std::vector<uint64_t> ul(cl.begin(), cl.end());
       odba_dba* dba = nullptr;
       odba_host::check(odba_dba_create(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)cs.data(), ul.data(),
-                                       (int)cs.size(), use_open_start, use_open_end, ODBA_PATH_FULL, 0, &dba),
+                                       (int)cs.size(), use_open_start, use_open_end, ODBA_PATH_AUTO,
+                                       odba_host::ref_free_mem(), &dba),
                        "DBA session");
       odba_host::check(odba_dba_set_centroid(dba, cen.data(), lc), "DBA centroid");
       odba_host::check(odba_dba_record_paths(dba, 1), "DBA paths");
@@ -556,19 +715,20 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
 }
 
 /* adaptive_segmentation<T> (segmentation.hpp:339-583); allocates the
- * outputs with malloc. lds_bytes defaults to the opt-in shared memory of
- * the GPUs the reference documents (64 KiB). */
+ * outputs with malloc. The shared-memory budget the reference sizes its
+ * tables with comes from ODBA_KSEG_LDS_BYTES (64 KiB by default). */
 template <typename T>
 void adaptive_segmentation(T** sequences, size_t* seq_lengths, int num_seqs, int min_segment_length,
                            T*** segmented_sequences, size_t** segmented_seq_lengths, int prefix_length_to_skip,
-                           int lds_bytes = 65536) {
+                           hipStream_t stream = 0) {
+  (void)stream;
   std::vector<uint64_t> l(seq_lengths, seq_lengths + num_seqs);
   const uint64_t cap = odba_kseg_capacity(l.data(), num_seqs, min_segment_length);
   std::vector<T> out(cap ? cap : 1);
   std::vector<uint64_t> ol(num_seqs), oo(num_seqs);
   odba_host::check(odba_kseg(odba_host::ctx(), odba_host::DType<T>::v, (const void* const*)sequences, l.data(),
-                             num_seqs, min_segment_length, prefix_length_to_skip, lds_bytes, out.data(), cap, ol.data(),
-                             oo.data()),
+                             num_seqs, min_segment_length, prefix_length_to_skip, odba_host::kseg_lds_bytes(),
+                             out.data(), cap, ol.data(), oo.data()),
                    "adaptive_segmentation");
   *segmented_sequences = (T**)std::calloc(num_seqs, sizeof(T*));
   *segmented_seq_lengths = (size_t*)std::calloc(num_seqs, sizeof(size_t));
@@ -579,7 +739,15 @@ void adaptive_segmentation(T** sequences, size_t* seq_lengths, int num_seqs, int
   }
 }
 
-/* setupAndRun<T> (openDBA.cuh:13-200). */
+/* setupAndRun<T> (openDBA.cuh:13-200): read, optional prefix chop, optional
+ * k-segmentation, then performDBA. min_segment_length_string is "m" or
+ * "m,m2": with m2, the segmented sequences are clustered (CLUSTER_ONLY) and
+ * the consensus is built on the unsegmented ones from the membership file
+ * (CONSENSUS_ONLY), as the reference does. The reference keeps one names
+ * array for both sets: the clustering call's length sort reorders it and
+ * the segmentation filter shifts it, while the unsegmented sequences stay
+ * in input order, so the consensus call pairs sequence i with the name at i
+ * after those changes; this layer does the same. */
 template <typename T>
 void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_series, char* output_prefix,
                  int read_mode, int use_open_start, int use_open_end, char* min_segment_length_string,
@@ -588,7 +756,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
   size_t* lens = nullptr;
   T** seqs = nullptr;
   char** names = nullptr;
-  int min2 = -1;
+  int min2 = -1;  // -1: not given (openDBA.cuh:26-33)
   char* pos = std::strchr(min_segment_length_string, ',');
   if (pos) { min2 = std::atoi(pos + 1); *pos = '\0'; }
   const int minseg = std::atoi(min_segment_length_string);
@@ -600,6 +768,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
     std::cerr << "At least two sequences must be provided to calculate an average, but found " << n << ", aborting" << std::endl;
     std::exit(NOT_ENOUGH_SEQUENCES);
   }
+  const int n_read = n;
   for (int i = 0; i < n; i++) { char* z = std::strchr(names[i], '.'); if (z) *z = '\0'; }  // openDBA.cuh:59
   if (seqprefix_file_name != nullptr) {  // openDBA.cuh:62-83
     T** pre = nullptr;
@@ -620,37 +789,61 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
     std::free(pre_name);
     std::free(pre_len);
   }
+  const int n_raw = n;  // unsegmented sequences held (the two-stage flow keeps them)
   if (minseg > 0) {  // openDBA.cuh:85-177
     T** seg = nullptr;
     size_t* seglens = nullptr;
     adaptive_segmentation<T>(seqs, lens, n, minseg, &seg, &seglens, prefix_start);
-    int kept = 0;
+    int removed = 0;
     for (int i = 0; i < n; i++) {
-      std::free(seqs[i]);
-      if (seglens[i] < 2 || (prefix_length > 0 && seglens[i] < (size_t)prefix_length)) { std::free(seg[i]); continue; }
-      seg[kept] = seg[i]; seglens[kept] = seglens[i]; names[kept] = names[i]; kept++;
+      if (min2 == -1) std::free(seqs[i]);
+      const int k = i - removed;
+      if (seglens[k] < 2 || (prefix_length > 0 && seglens[k] < (size_t)prefix_length)) {
+        std::free(seg[k]);
+        for (int j = k + 1; j < n; j++) { seg[j - 1] = seg[j]; seglens[j - 1] = seglens[j]; names[j - 1] = names[j]; }
+        removed++;
+      }
     }
-    if (kept < n) {
-      std::cerr << "Removing " << (n - kept) << " segmented sequences that are too short" << std::endl;
-      n = kept;
-      if (n < 2) { std::cerr << "At least two sequences must survive segmentation filters" << std::endl; std::exit(NOT_ENOUGH_SEQUENCES); }
+    if (removed) {
+      std::cerr << "Removing " << removed << " segmented sequences that are too short, as they may unduly skew the "
+                << "convergence process. To retain more sequences, consider setting a smaller minimum segment size "
+                << "(currently " << minseg << ")" << std::endl;
+      n -= removed;
+      if (n < 2) {
+        std::cerr << "At least two sequences must survive segmentation filters to calculate an average, but found " << n
+                  << ", aborting" << std::endl;
+        std::exit(NOT_ENOUGH_SEQUENCES);
+      }
     }
     if (prefix_length > 0) for (int i = 0; i < n; i++) seglens[i] = (size_t)prefix_length;
-    std::ofstream out((std::string(output_prefix) + ".segmented_seqs.txt").c_str());  // io_utils.hpp:242-261
-    for (int i = 0; i < n; i++) {
-      out << names[i] << "\t" << seg[i][0];
-      for (size_t j = 1; j < seglens[i]; j++) out << "\t" << seg[i][j];
-      out << std::endl;
+    writeSequences<T>(seg, seglens, names, n, (std::string(output_prefix) + ".segmented_seqs.txt").c_str());
+    if (min2 != -1) {
+      std::cerr << "Performing cluster generation with segment size of " << minseg << std::endl;
+      performDBA<T>(seg, n, seglens, names, use_open_start, use_open_end, output_prefix, norm_sequences, cdist,
+                    series_file_names, num_series, read_mode, minseg > 1, CLUSTER_ONLY);
+      for (int i = 0; i < n; i++) std::free(seg[i]);
+      std::free(seg);
+      std::free(seglens);
+    } else {
+      std::free(seqs);
+      std::free(lens);
+      seqs = seg;
+      lens = seglens;
     }
-    std::free(seqs);
-    std::free(lens);
-    seqs = seg;
-    lens = seglens;
-    (void)min2;
   }
-  performDBA<T>(seqs, n, lens, names, use_open_start, use_open_end, output_prefix, norm_sequences, cdist,
-                series_file_names, num_series, read_mode, minseg > 1, CLUSTER_AND_CONSENSUS);
-  for (int i = 0; i < n; i++) { std::free(names[i]); std::free(seqs[i]); }
+  if (min2 != -1) {
+    std::cerr << "Performing consensus generation with segment size of " << min2 << std::endl;
+    performDBA<T>(seqs, n, lens, names, use_open_start, use_open_end, output_prefix, norm_sequences, cdist,
+                  series_file_names, num_series, read_mode, minseg > 1, CONSENSUS_ONLY);
+  } else {
+    std::cerr << "Performing both clustering and consensus generation with segment size of " << minseg << std::endl;
+    performDBA<T>(seqs, n, lens, names, use_open_start, use_open_end, output_prefix, norm_sequences, cdist,
+                  series_file_names, num_series, read_mode, minseg > 1, CLUSTER_AND_CONSENSUS);
+  }
+  const int held = (min2 != -1 || minseg <= 0) ? n_raw : n;
+  for (int i = 0; i < held; i++) std::free(seqs[i]);
+  for (int i = 0; i < n; i++) std::free(names[i]);
+  (void)n_read;
   std::free(names);
   std::free(seqs);
   std::free(lens);

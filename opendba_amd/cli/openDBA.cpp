@@ -12,7 +12,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -52,17 +51,16 @@ int main(int argc, char** argv) {
     use_open_start = 0;
     use_open_end = 1;
     norm_sequences = 1;
-    std::stringstream ss(std::string(argv[3] + 12));
-    std::vector<std::string> fields;
-    std::string tmp;
-    while (std::getline(ss, tmp, '_')) fields.push_back(tmp);
-    if (fields.size() != 2) {
+    // "open_prefix_<skip>_<length>": exactly two numbers after the mode name
+    const std::string spec(std::strlen(argv[3]) > 12 ? argv[3] + 12 : "");
+    const size_t cut = spec.find('_');
+    if (cut == std::string::npos || spec.find('_', cut + 1) != std::string::npos) {
       std::cerr << "Unexpected alignment type specified, expected open_prefix_##_## but did not find a second "
                    "underscore in " << argv[3] << std::endl;
       std::exit(1);
     }
-    prefix_to_skip = std::stoi(fields[0]);
-    prefix_length = std::stoi(fields[1]);
+    prefix_to_skip = std::stoi(spec.substr(0, cut));
+    prefix_length = std::stoi(spec.substr(cut + 1));
   } else if (!std::strcmp(argv[3], "open")) {
     use_open_start = 1;
     use_open_end = 1;

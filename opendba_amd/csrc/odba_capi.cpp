@@ -142,6 +142,20 @@ struct DevBufs {
   }
 };
 
+// The free-memory figure of ODBA_PATH_AUTO: the stated one, or with none
+// (0) the device's free memory when the session's sequences are resident,
+// capped at the context's share; the reference reads cudaMemGetInfo before
+// each sequence's path (dba.hpp:523-524).
+uint64_t auto_free_mem(odba_ctx* c, int policy, uint64_t stated) {
+  if (policy != ODBA_PATH_AUTO || stated) return stated;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return c->mem_cap;
+  }
+  return std::min<uint64_t>((uint64_t)fr, c->mem_cap);
+}
+
 int grid_blocks(odba_ctx* c, long long work_items) {
   long long want = (work_items + 3) / 4;
   long long cap = (long long)c->cus * 8;  // 8 workgroups of 4 waves per CU
@@ -1521,7 +1535,7 @@ int dba_round_t(odba_ctx* c, const T* C, int lc, const T* const* seqs, const uin
   int rc = st.init(c, seqs, lens, n, os, oe);
   if (rc) return rc;
   st.policy = policy;
-  st.free_mem = free_mem;
+  st.free_mem = auto_free_mem(c, policy, free_mem);
   rc = st.set_centroid(C, lc);
   if (rc) return rc;
   rc = st.round(delta);
@@ -1677,7 +1691,7 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   rc = st.init_shared(c, B.d, B.o, ho, hl, no_wrap, os, oe);
   if (rc) return rc;
   st.policy = policy;
-  st.free_mem = free_mem;
+  st.free_mem = auto_free_mem(c, policy, free_mem);
   rc = st.set_centroid_device(B.d + ho[med], lc);
   if (rc) return rc;
   std::vector<T> nb(lc), prev(lc, T(0)), prev2(lc, T(0));
@@ -2268,7 +2282,7 @@ template <typename T> int dba_create_t(odba_ctx* c, const T* const* seqs, const 
   int rc = st->init(c, seqs, lens, n, os, oe);
   if (rc) { delete st; return rc; }
   st->policy = policy;
-  st->free_mem = free_mem;
+  st->free_mem = auto_free_mem(c, policy, free_mem);
   *out = st;
   return ODBA_OK;
 }

@@ -49,3 +49,13 @@ def test_no_device_is_an_error_not_a_fallback(lib):
     lib.odba_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
     rc = lib.odba_ctx_create(0, C.byref(h))
     assert rc == 105  # ODBA_E_NO_DEVICE
+
+
+def test_reference_style_caller_compiles_against_the_header(lib):
+    """tests/cpp/drop_in_calls.cpp calls every template of
+    include/odba/opendba.hpp with the reference's argument lists; it must
+    compile and link unchanged (-Wall, no warnings)."""
+    r = subprocess.run(["make", "-B", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "warning" not in r.stderr.lower(), r.stderr
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "_build", "drop_in_calls"))
