@@ -9,7 +9,16 @@ the same reads go through perform_dba (open end) to convergence, and its
 wall time and round count are reported beside the line's value.
 
 Metric: GCUPS = sum over pairs of |Y|*|X| (the full unbanded matrix, also for
-pairs whose open-end search stops early) / wall time of the step.
+pairs whose open-end search stops early) / wall time of the step;
+evaluated_gcups counts only the cells the kernel evaluates (an open-end pair
+stops at its abrogated swath).
+
+The default line also carries, under a wall-time budget, the other
+configurations the metric is quoted on: the small method (32 random walks,
+global, 496 pairs: kernel time, roofline fraction, pipeline with 3 DBA
+rounds), the full-size serving trace (served vs plain per-job jobs/s, p50,
+p99, identical results) and segment-then-cluster per-cluster DBA (all
+clusters' rounds packed vs one cluster at a time).
 
 Roofline: the swath kernel is bound by VALU issue, not by HBM or MFMA. The
 algorithmic work is 4 VALU lane-ops per evaluated cell (sub, mul, min3, add;
@@ -32,6 +41,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+T_START = time.perf_counter()
 METRIC = "all-vs-all DTW GCUPS + DBA jobs/s with p50/p99 latency vs plain per-job path"
 VALU_OPS_PER_CELL = 4      # sub, mul, min3, add (fp-contract off)
 FP32_VECTOR_TFLOPS = 157.3  # MI355X published FP32 vector peak; 2 flops per FMA lane-op
@@ -66,14 +76,33 @@ def evaluated_cells(lens_sorted, flags):
     return tot
 
 
+def host_cpu():
+    """CPU model, CPUs on the host and CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
+
+
 def cpu_baseline(seqs, target_s=15.0, os_=0, oe=1):
     """Oracle all-vs-all on a seeded uniform sample of pairs, timed on this
-    host's cores."""
+    host's cores: one thread per CPU the process may run on, at most 16 (a
+    one-GPU share of the GPU box's host)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     n = len(seqs)
     npairs = n * (n - 1) // 2
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    model, host_cpus, allowed = host_cpu()
+    threads = max(1, min(allowed, 16))
     rng = np.random.default_rng(123)
     lens = np.array([len(s) for s in seqs], dtype=np.float64)
 
@@ -99,16 +128,17 @@ def cpu_baseline(seqs, target_s=15.0, os_=0, oe=1):
     gc = reps * cells_of(sub) / dt / 1e9
     mode = "open_end" if oe else "global"
     return dict(value=gc, unit="GCUPS", cores=threads, kind="port",
+                cpu_model=model, host_cpus=host_cpus, cpus_allowed=allowed,
+                build="g++ -O3 -march=x86-64-v3 -ffp-contract=off -fwrapv (oracle/Makefile)",
                 sample=f"{size2} seeded uniform pairs of the same workload x {reps} ({reps * cells_of(sub):.3g} "
                        f"full-matrix cells, {mode}, oracle with {threads} threads, {dt:.1f} s)")
 
 
-def run_small32(args, N, WL):
+def small32_leg(args, N, WL, steps=20, warmup=3):
     """The plain small method: 32 float32 random walks (step sd 1), lengths
     uniform in [200, 400], z-normalised, global all-vs-all (496 pairs), then
     medoid and 3 DBA rounds. GCUPS from the median all-pairs kernel time."""
     raw = WL.random_walks(32, 200, 400, seed=0)
-    steps = max(args.steps, 20)
     with N.Context(0) as ctx:
         info = ctx.device_info()
         seqs, _, _ = ctx.znorm(raw)
@@ -119,7 +149,7 @@ def run_small32(args, N, WL):
         t_tune = time.perf_counter()
         sched, tune_ms, tried = sset.tune(0, 0, reps=5)
         t_tune = time.perf_counter() - t_tune
-        for w in range(max(args.warmup, 3)):
+        for w in range(warmup):
             sset.all_pairs(0, 0, copy_out=False)
         kms = []
         t0 = time.perf_counter()
@@ -131,7 +161,7 @@ def run_small32(args, N, WL):
         tp, te = [], []
         for _ in range(5):
             t1 = time.perf_counter()
-            r = ctx.perform_dba(raw, 0, 0, norm=1, max_rounds=3)
+            ctx.perform_dba(raw, 0, 0, norm=1, max_rounds=3)
             tp.append(time.perf_counter() - t1)
         os.environ["ODBA_DBA_GRAPH"] = "0"
         for _ in range(5):
@@ -140,29 +170,49 @@ def run_small32(args, N, WL):
             te.append(time.perf_counter() - t1)
         del os.environ["ODBA_DBA_GRAPH"]
     kmed = float(np.median(kms))
-    value = nominal / (kmed / 1000.0) / 1e9
     achieved = nominal * VALU_OPS_PER_CELL / (kmed / 1000.0) / 1e12
     peak = FP32_VECTOR_TFLOPS / 2.0
+    return dict(seqs=seqs, info=info, nominal=nominal, kmed=kmed, wall=wall, steps=steps, warmup=warmup,
+                sched=sched, tried=tried, t_tune=t_tune, achieved=achieved, peak=peak,
+                gcups=nominal / (kmed / 1000.0) / 1e9,
+                pipeline_ms=float(np.median(tp)) * 1000.0, pipeline_ms_eager=float(np.median(te)) * 1000.0)
+
+
+SMALL32_WORKLOAD = ("plain small: 32 float32 random walks (step sd 1), lengths U[200,400], z-norm, global "
+                    "all-vs-all (496 pairs), then medoid + 3 DBA rounds")
+
+
+def small32_summary(m):
+    """The small method's numbers as one object of the default line."""
+    return {"workload": SMALL32_WORKLOAD, "pairs": 496, "full_matrix_cells": m["nominal"],
+            "gcups": round(m["gcups"], 2), "kernel_ms_median": round(m["kmed"], 4),
+            "kernel": "k_all_pairs<float, os=0, oe=0>", "schedule": m["sched"],
+            "roofline_frac": round(m["achieved"] / m["peak"], 4),
+            "pipeline_ms_median": round(m["pipeline_ms"], 3),
+            "pipeline": "perform_dba: z-norm, sort, all-vs-all, medoid, 3 DBA rounds, rescale"}
+
+
+def run_small32(args, N, WL):
+    m = small32_leg(args, N, WL, steps=max(args.steps, 20), warmup=max(args.warmup, 3))
     line = {
-        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1, "steps": steps,
-        "warmup": max(args.warmup, 3), "ms_per_step": round(wall * 1000.0 / steps, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(m["gcups"], 2), "unit": "GCUPS", "n_gpus": 1, "steps": m["steps"],
+        "warmup": m["warmup"], "ms_per_step": round(m["wall"] * 1000.0 / m["steps"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "plain small: 32 float32 random walks (step sd 1), lengths U[200,400], z-norm, global "
-                               "all-vs-all (496 pairs), then medoid + 3 DBA rounds",
-                   "pairs": 496, "full_matrix_cells": nominal, "mode": "global", "device": info["name"],
-                   "kernel_ms_median": round(kmed, 4),
-                   "schedule": sched, "tuned_from": tried, "tune_s": round(t_tune, 3),
-                   "pipeline_ms_median": round(float(np.median(tp)) * 1000.0, 3),
-                   "pipeline_ms_median_without_graphs": round(float(np.median(te)) * 1000.0, 3),
+        "config": {"workload": SMALL32_WORKLOAD,
+                   "pairs": 496, "full_matrix_cells": m["nominal"], "mode": "global", "device": m["info"]["name"],
+                   "kernel_ms_median": round(m["kmed"], 4),
+                   "schedule": m["sched"], "tuned_from": m["tried"], "tune_s": round(m["t_tune"], 3),
+                   "pipeline_ms_median": round(m["pipeline_ms"], 3),
+                   "pipeline_ms_median_without_graphs": round(m["pipeline_ms_eager"], 3),
                    "pipeline": "perform_dba: z-norm, sort, all-vs-all, medoid, 3 DBA rounds (graph replays "
                                "from the second), rescale"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "TOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "k_all_pairs<float, os=0, oe=0>", "kernel_ms": round(kmed, 4),
+        "roofline": {"bound": "valu", "achieved": round(m["achieved"], 3), "peak": round(m["peak"], 3),
+                     "unit": "TOP/s", "frac": round(m["achieved"] / m["peak"], 4), "traffic": None,
+                     "kernel": "k_all_pairs<float, os=0, oe=0>", "kernel_ms": round(m["kmed"], 4),
                      "ops_per_cell": VALU_OPS_PER_CELL},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(seqs, target_s=10.0, os_=0, oe=0)
+        line["cpu_baseline"] = cpu_baseline(m["seqs"], target_s=10.0, os_=0, oe=0)
     return line
 
 
@@ -246,31 +296,28 @@ def run_stripes(args, N, WL):
     }
 
 
-def run_serve(args, N, WL):
+def serve_leg(args, N, WL, jobs_n=32, scale=1.0, slots=8, max_rounds=250):
     """Serving: a seeded Poisson trace (mean 20 jobs/s) of DBA jobs with
-    16-512 series (scaled by --scale), lengths log-uniform 500-50k (scaled),
+    16-512 series (scaled by scale), lengths log-uniform 500-50k (scaled),
     float32 random walks or int16 squiggles as float, global or open-end;
     each job is z-norm -> all-vs-all -> medoid -> DBA to convergence. The same
     trace runs through the plain per-job path (one job after another) and
     through the serving loop; jobs/s = jobs / makespan, latency = completion
     - arrival."""
     from opendba_amd import serve as S
-    jobs = WL.poisson_jobs(n_jobs=args.jobs, rate=20.0, seed=4, scale=args.scale)
+    jobs = WL.poisson_jobs(n_jobs=jobs_n, rate=20.0, seed=4, scale=scale)
     cells = sum(WL.pair_cells([len(s) for s in j["seqs"]]) for j in jobs)
     log = lambda tag: (lambda k, o: print(f"{tag} job {k}: latency {o['latency']:.3f} s rounds {o['rounds']}",
                                           file=sys.stderr, flush=True))
-    plain = S.run_trace(jobs, "plain", max_rounds=args.max_rounds, progress=log("plain"))
-    served = S.run_trace(jobs, "served", slots=args.slots, max_rounds=args.max_rounds, progress=log("served"))
+    plain = S.run_trace(jobs, "plain", max_rounds=max_rounds, progress=log("plain"))
+    served = S.run_trace(jobs, "served", slots=slots, max_rounds=max_rounds, progress=log("served"))
     same = all(a["centroid"].tobytes() == b["centroid"].tobytes() and a["medoid"] == b["medoid"]
                for a, b in zip(plain["results"], served["results"]))
     return {
-        "metric": METRIC, "value": round(served["jobs_per_s"], 4), "unit": "jobs/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": round(served["makespan_s"] * 1000.0, 1), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"serving: {args.jobs} Poisson jobs (mean 20/s, seed 4), 16-512 series and lengths "
-                               f"500-50k scaled by {args.scale}, f32 walks / int16 squiggles, global / open_end, "
-                               f"z-norm -> all-vs-all -> medoid -> DBA (max {args.max_rounds} rounds)",
-                   "slots": args.slots, "all_pairs_cells": cells},
+        "workload": f"serving: {jobs_n} Poisson jobs (mean 20/s, seed 4), 16-512 series and lengths 500-50k scaled "
+                    f"by {scale}, f32 walks / int16 squiggles, global / open_end, z-norm -> all-vs-all -> medoid -> "
+                    f"DBA (max {max_rounds} rounds)",
+        "slots": slots, "all_pairs_cells": cells,
         "served": {"jobs_per_s": round(served["jobs_per_s"], 4), "p50_s": round(served["p50_s"], 4),
                    "p99_s": round(served["p99_s"], 4), "makespan_s": round(served["makespan_s"], 3)},
         "plain": {"jobs_per_s": round(plain["jobs_per_s"], 4), "p50_s": round(plain["p50_s"], 4),
@@ -280,7 +327,19 @@ def run_serve(args, N, WL):
     }
 
 
-def run_clusters(args, N, WL):
+def run_serve(args, N, WL):
+    leg = serve_leg(args, N, WL, jobs_n=args.jobs, scale=args.scale, slots=args.slots, max_rounds=args.max_rounds)
+    return {
+        "metric": METRIC, "value": leg["served"]["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(leg["served"]["makespan_s"] * 1000.0, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": leg["workload"], "slots": leg["slots"], "all_pairs_cells": leg["all_pairs_cells"]},
+        "served": leg["served"], "plain": leg["plain"], "speedup_jobs_per_s": leg["speedup_jobs_per_s"],
+        "results_identical": leg["results_identical"],
+    }
+
+
+def clusters_leg(args, N, WL):
     """Segment-then-cluster: int16 squiggles (levels N(90,12), dwell geom mean
     8, noise sd 2), lengths log-uniform 4k-40k, as reads of K molecules
     (n/K reads each, every read with its own dwell times and noise),
@@ -347,22 +406,31 @@ def run_clusters(args, N, WL):
     # bitwise: a centroid position no member maps to is NaN (0/0, dba.hpp:820-822)
     same = all(plain[k][0].tobytes() == cents[k].tobytes() and plain[k][1] == rounds[k] for k in multi)
     return {
-        "metric": METRIC, "value": round(len(multi) / packed_s, 3), "unit": "jobs/s (cluster DBA)", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": round(packed_s * 1000.0, 1), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"segment-then-cluster: {n} int16 squiggles as {reads_per} reads of each of "
-                               f"{args.clusters} molecules, lengths 4k-40k ({samples} samples), "
-                               f"k-seg minseg 8, z-norm, open (open start + open end) all-vs-all on events, "
-                               f"complete linkage into "
-                               f"K={args.clusters} clusters of 2+ members, per-cluster DBA (max {R} rounds)",
-                   "events": int(lens.sum()), "pairs": n * (n - 1) // 2, "all_pairs_cells": cells,
-                   "all_pairs_gcups": round(cells / (kms / 1000.0) / 1e9, 1), "clusters": K,
-                   "multi_member_clusters": len(multi), "rounds_total": int(sum(rounds[k] for k in multi)),
-                   **{k: round(v, 3) for k, v in stage.items()}},
+        "workload": f"segment-then-cluster: {n} int16 squiggles as {reads_per} reads of each of {args.clusters} "
+                    f"molecules, lengths 4k-40k ({samples} samples), k-seg minseg 8, z-norm, open (open start + "
+                    f"open end) all-vs-all on events, complete linkage into K={args.clusters} clusters of 2+ "
+                    f"members, per-cluster DBA (max {R} rounds)",
+        "events": int(lens.sum()), "pairs": n * (n - 1) // 2, "all_pairs_cells": cells,
+        "all_pairs_gcups": round(cells / (kms / 1000.0) / 1e9, 1), "clusters": K,
+        "multi_member_clusters": len(multi), "rounds_total": int(sum(rounds[k] for k in multi)),
+        **{k: round(v, 3) for k, v in stage.items()},
         "packed": {"jobs_per_s": round(len(multi) / packed_s, 3), "seconds": round(packed_s, 3)},
         "plain": {"jobs_per_s": round(len(multi) / plain_s, 3), "seconds": round(plain_s, 3)},
         "speedup_jobs_per_s": round(plain_s / packed_s, 3),
         "results_identical": bool(same),
+    }
+
+
+def run_clusters(args, N, WL):
+    leg = clusters_leg(args, N, WL)
+    return {
+        "metric": METRIC, "value": leg["packed"]["jobs_per_s"], "unit": "jobs/s (cluster DBA)", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(leg["packed"]["seconds"] * 1000.0, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {k: v for k, v in leg.items() if k not in ("packed", "plain", "speedup_jobs_per_s",
+                                                             "results_identical")},
+        "packed": leg["packed"], "plain": leg["plain"], "speedup_jobs_per_s": leg["speedup_jobs_per_s"],
+        "results_identical": leg["results_identical"],
     }
 
 
@@ -388,6 +456,10 @@ def main():
                     help="squiggles: DBA round limit of the perform_dba run after the timed all-vs-all (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the schedule autotuner")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="default run: wall-time budget; a leg runs only if its estimate still fits")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default run: only the headline all-vs-all (no small-method, serving or cluster legs)")
     ap.add_argument("--order", type=int, choices=[0, 1], default=None,
                     help="pin the pair packing order instead of tuning (0 built-in, 1 largest cells first)")
     args = ap.parse_args()
@@ -396,10 +468,11 @@ def main():
               f"pair cost across GPUs by design, but this build measures one GPU)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "serve":
-        # One hardware queue per slot stream (HIP reads this when it starts;
-        # with fewer queues than streams, slots share queues and their kernels
-        # run one after another).
+    if args.workload in ("serve", "squiggles"):
+        # One hardware queue per slot stream of the serving loop (HIP reads
+        # this when it starts; with fewer queues than streams, slots share
+        # queues and their kernels run one after another). The default run
+        # serves too, after its headline.
         if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < args.slots:
             os.environ["GPU_MAX_HW_QUEUES"] = str(args.slots)
     from opendba_amd import native as N
@@ -465,6 +538,7 @@ def main():
     value = nominal / (ms_per_step / 1000.0) / 1e9
     kavg = float(np.mean(kms))
     ev_cells = evaluated_cells(lens, flags)
+    evaluated_gcups = ev_cells / (ms_per_step / 1000.0) / 1e9
     achieved = ev_cells * VALU_OPS_PER_CELL / (kavg / 1000.0) / 1e12
     peak = FP32_VECTOR_TFLOPS / 2.0
     from opendba_amd import probe
@@ -473,6 +547,7 @@ def main():
         "metric": METRIC,
         "value": round(value, 2),
         "unit": "GCUPS",
+        "evaluated_gcups": round(evaluated_gcups, 2),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -517,7 +592,33 @@ def main():
         line["dba_to_convergence"] = dba
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(seqs)
+    if not args.no_legs:
+        legs_run(args, N, WL, line)
     print(json.dumps(line))
+
+
+def legs_run(args, N, WL, line):
+    """The other configurations the metric is quoted on, each as one object
+    of the default line, in order, each only if its estimated wall time still
+    fits the budget: the small method (its kernel, roofline fraction and
+    pipeline), the full-size serving trace (served vs plain jobs/s, p50,
+    p99, identical results) and the segment-then-cluster per-cluster DBA
+    (packed vs one cluster at a time)."""
+    legs = [("config1_small", 20.0, lambda: small32_summary(small32_leg(args, N, WL))),
+            ("serving", 240.0, lambda: serve_leg(args, N, WL, jobs_n=32, scale=1.0, slots=args.slots)),
+            ("segment_then_cluster", 120.0, lambda: clusters_leg(args, N, WL))]
+    skipped = []
+    for name, est, fn in legs:
+        used = time.perf_counter() - T_START
+        if used + est > args.budget_s:
+            skipped.append(f"{name} (estimated {est:.0f} s, {used:.0f} s of the {args.budget_s:.0f} s budget used)")
+            continue
+        t0 = time.perf_counter()
+        print(f"leg {name}: starting at {used:.0f} s", file=sys.stderr, flush=True)
+        line[name] = fn()
+        line[name]["leg_wall_s"] = round(time.perf_counter() - t0, 1)
+    if skipped:
+        line["legs_skipped"] = skipped
 
 
 if __name__ == "__main__":
