@@ -606,7 +606,7 @@ def legs_run(args, N, WL, line):
     (packed vs one cluster at a time)."""
     legs = [("config1_small", 20.0, lambda: small32_summary(small32_leg(args, N, WL))),
             ("serving", 240.0, lambda: serve_leg(args, N, WL, jobs_n=32, scale=1.0, slots=args.slots)),
-            ("segment_then_cluster", 120.0, lambda: clusters_leg(args, N, WL))]
+            ("segment_then_cluster", 30.0, lambda: clusters_leg(args, N, WL))]
     skipped = []
     for name, est, fn in legs:
         used = time.perf_counter() - T_START
