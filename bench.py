@@ -434,13 +434,44 @@ def run_clusters(args, N, WL):
     }
 
 
+def run_dba(args, N, WL):
+    """DBA rounds of the batched-uneven set: 2,000 int16 squiggles as float
+    (lengths log-uniform 2k-20k), perform_dba in the given mode with
+    --dba-rounds rounds at most (sanitize, sort, z-norm, all-vs-all, medoid,
+    rounds, rescale). In global mode every round evaluates
+    sum_i |S_i| * |centroid| cells (no open-end stop, no flip), which the line
+    reports per round for the path kernel's roofline."""
+    oe = 1 if args.mode == "open_end" else 0
+    raw = WL.squiggles(args.n, 2000, 20000, seed=1)
+    seqs = WL.short_to_float(raw)
+    with N.Context(0) as ctx:
+        info = ctx.device_info()
+        t0 = time.perf_counter()
+        r = ctx.perform_dba(seqs, 0, oe, norm=1, max_rounds=args.dba_rounds)
+        wall = time.perf_counter() - t0
+    lc = len(r["centroid"])
+    cells_round = float(sum(len(s) for s in seqs)) * lc
+    return {
+        "metric": METRIC, "value": round(r["rounds"] / wall, 3), "unit": "DBA rounds/s (incl. all-vs-all)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1000.0, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"DBA rounds: {args.n} int16 squiggles as float, lengths 2k-20k, {args.mode}, "
+                               f"perform_dba with at most {args.dba_rounds} rounds",
+                   "mode": args.mode, "rounds": r["rounds"], "medoid": int(r["medoid"]), "centroid_len": lc,
+                   "path_cells_per_round": cells_round if not oe else None,
+                   "path_cells_per_round_upper": cells_round, "device": info["name"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--workload", choices=["squiggles", "small32", "serve", "clusters", "stripes"], default="squiggles")
+    ap.add_argument("--workload", choices=["squiggles", "small32", "serve", "clusters", "stripes", "dba"],
+                    default="squiggles")
+    ap.add_argument("--mode", choices=["global", "open_end"], default="open_end", help="dba: alignment mode")
     ap.add_argument("--long-n", type=int, default=8, help="stripes: sequences")
     ap.add_argument("--long-min", type=int, default=1_000_000, help="stripes: shortest length")
     ap.add_argument("--long-max", type=int, default=4_000_000, help="stripes: longest length")
@@ -489,6 +520,9 @@ def main():
         return
     if args.workload == "stripes":
         print(json.dumps(run_stripes(args, N, WL)))
+        return
+    if args.workload == "dba":
+        print(json.dumps(run_dba(args, N, WL)))
         return
     raw = WL.squiggles(args.n, 2000, 20000, seed=1)
     seqs, _ = WL.stable_length_sort(WL.short_to_float(raw))

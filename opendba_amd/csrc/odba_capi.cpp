@@ -769,12 +769,25 @@ template <typename J> double mean_swaths(const std::vector<J>& jobs) {
   return t / (double)jobs.size();
 }
 
+// Longest job's row-swaths over the mean (1: even jobs). One wave per job
+// ends when its longest job does, so a large ratio means a tail.
+template <typename J> double path_tail(const std::vector<J>& jobs) {
+  if (jobs.empty()) return 0.0;
+  double t = 0, mx = 0;
+  for (const J& j : jobs) {
+    const double w = (double)j.ly * (double)((j.lx + ODBA_W - 1) / ODBA_W);
+    t += w;
+    mx = std::max(mx, w);
+  }
+  return t > 0 ? mx / (t / (double)jobs.size()) : 0.0;
+}
+
 // Path jobs: one wave per job, or (float, few long jobs) a workgroup of
 // waves per job sweeping its swaths as a pipeline. ODBA_PATH_SCHEDULE =
 // wave|strip overrides the choice.
 int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
                   uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0, int* err = nullptr,
-                  double mean_swaths = 0.0) {
+                  double mean_swaths = 0.0, double tail = 0.0) {
   const size_t es = dtype_size(dtype);
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
@@ -792,7 +805,11 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   // average eight or more swaths each (a long centroid on the columns):
   // then the longest runs alone on one SIMD for the whole round, and the
   // workgroup-per-job schedule (longest first, its waves busy) is faster.
-  bool strips = (njobs < 4 * c->cus || mean_swaths >= 8.0) && maxlx > 256 && minly >= 64;
+  // Uneven jobs: the longest holds at least twice the mean work, so one
+  // wave per job would end with that job alone on its SIMD (2,000 squiggles
+  // of 2k-20k against a 4,603-sample centroid: 41.5 ms per round as one wave
+  // per job, 19.7 ms as strips).
+  bool strips = (njobs < 4 * c->cus || mean_swaths >= 8.0 || tail >= 2.0) && maxlx > 256 && minly >= 64;
   bool quarters = kmax <= 4;
   const bool bands = false;
   // Few jobs (at most 1,024) of which the longest has more swaths than a
@@ -1391,7 +1408,7 @@ template <typename T> struct DbaState {
         HIPC(c, hipMemcpyAsync(jobs, pj.data(), nfull * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
         HIPC(c, hipMemcpyAsync(tbj, tb.data(), nfull * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
         rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, jobs, nfull, maxly, mv, mtop, abr, maxlx, minly, err,
-                           mean_swaths(pj));
+                           mean_swaths(pj), path_tail(pj));
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
@@ -1474,7 +1491,7 @@ template <typename T> struct DbaState {
       if (e == hipSuccess) e = launch_fill_u32(c->stream, dbits, 0u, 2);
       if (e != hipSuccess) rc = fail(c, ODBA_E_HIP, "graph capture: %s", hipGetErrorString(e));
       if (!rc) rc = run_path_jobs(c, dtype_of<T>(), minfirst, os, oe, gjobs[p], nj, g_maxly, mv, mtop, abr, g_maxlx, g_minly, err,
-                                  mean_swaths(g_pj));
+                                  mean_swaths(g_pj), path_tail(g_pj));
       if (!rc) {
         e = launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, lc, os, oe, err, nullptr);
         if (e == hipSuccess) e = launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs, runs, n, lc, acc, counts, 1);
@@ -1883,7 +1900,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
       HIPC(c, hipMemsetAsync(dbits, 0, nc * sizeof(unsigned long long), c->stream));
       HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
       rc = run_path_jobs(c, dtype_of<T>(), no_wrap, os, oe, jobs, nj, maxly, mv, mtop, abr, maxlx, minly, derr,
-                         mean_swaths(pj));
+                         mean_swaths(pj), path_tail(pj));
       if (rc) return rc;
       HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, 0, os, oe, derr, nullptr));
       HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, dslots, (int)slots.size(), pos, acc, counts, dbits, derr));
@@ -2024,7 +2041,7 @@ int prefix_chop_t(odba_ctx* c, const T* prefix_in, int P, T* const* seqs, const 
     HIPC(c, hipMemcpyAsync(jobs, pj.data(), nj * sizeof(PJ<T>), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(tbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
-    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P, derr, mean_swaths(pj));
+    rc = run_path_jobs(c, dtype_of<T>(), no_wrap, 0, 1, jobs, nj, P, mv, mtop, abr, bmaxlx, P, derr, mean_swaths(pj), path_tail(pj));
     if (rc) return rc;
     HIPC(c, launch_traceback(c->stream, tbj, nj, mv, mtop, abr, runs, P, 0, 1, derr, nullptr));
     int herr = 0;
