@@ -182,11 +182,20 @@ SMALL32_WORKLOAD = ("plain small: 32 float32 random walks (step sd 1), lengths U
                     "all-vs-all (496 pairs), then medoid + 3 DBA rounds")
 
 
+def pair_kernel_name(sched, os_=0, oe=0):
+    """The all-vs-all kernel a schedule (odba_schedule as a dict) runs."""
+    if sched.get("kind") == 2:
+        return f"k_all_pairs_small<C={sched['band_c']}, B={sched['ring']}, os={os_}, oe={oe}>"
+    if sched.get("kind") == 1:
+        return f"k_all_pairs_bands<float, C={sched['band_c']}, os={os_}, oe={oe}>"
+    return f"k_all_pairs<float, os={os_}, oe={oe}>"
+
+
 def small32_summary(m):
     """The small method's numbers as one object of the default line."""
     return {"workload": SMALL32_WORKLOAD, "pairs": 496, "full_matrix_cells": m["nominal"],
             "gcups": round(m["gcups"], 2), "kernel_ms_median": round(m["kmed"], 4),
-            "kernel": "k_all_pairs<float, os=0, oe=0>", "schedule": m["sched"],
+            "kernel": pair_kernel_name(m["sched"]), "schedule": m["sched"],
             "roofline_frac": round(m["achieved"] / m["peak"], 4),
             "pipeline_ms_median": round(m["pipeline_ms"], 3),
             "pipeline": "perform_dba: z-norm, sort, all-vs-all, medoid, 3 DBA rounds, rescale"}
@@ -208,7 +217,7 @@ def run_small32(args, N, WL):
                                "from the second), rescale"},
         "roofline": {"bound": "valu", "achieved": round(m["achieved"], 3), "peak": round(m["peak"], 3),
                      "unit": "TOP/s", "frac": round(m["achieved"] / m["peak"], 4), "traffic": None,
-                     "kernel": "k_all_pairs<float, os=0, oe=0>", "kernel_ms": round(m["kmed"], 4),
+                     "kernel": pair_kernel_name(m["sched"]), "kernel_ms": round(m["kmed"], 4),
                      "ops_per_cell": VALU_OPS_PER_CELL},
     }
     if not args.no_cpu_baseline:

@@ -42,14 +42,19 @@
  * threadblock per pair and swath). */
 #define ODBA_SCHED_WAVE 0  /* one wave per pair, persistent grid               */
 #define ODBA_SCHED_BANDS 1 /* bands of column strips, a workgroup per band     */
+#define ODBA_SCHED_SMALL 2 /* small pairs (float, lengths 64..1024): a workgroup
+                              of band_waves waves per pair, band_c (1, 2 or 4)
+                              columns per lane, strips in lock step, ring =
+                              steps per sub-block (16 or 32; 0 means 16)      */
 
 typedef struct odba_schedule {
-  int kind;       /* ODBA_SCHED_WAVE or ODBA_SCHED_BANDS                               */
+  int kind;       /* ODBA_SCHED_WAVE, ODBA_SCHED_BANDS or ODBA_SCHED_SMALL             */
   int order;      /* wave: pair packing order, 0 = second index descending then first   */
                   /* index descending, 1 = largest cell count first                     */
   int band_c;     /* bands: columns per lane, 1, 4 or 16 (strips of 64 * band_c columns) */
   int band_waves; /* bands: strips per band = waves per workgroup, 1..16                 */
-  int ring;       /* bands: rows of the in-workgroup edge ring (power of two >= 128)     */
+  int ring;       /* bands: rows of the in-workgroup edge ring (power of two >= 128);    */
+                  /* small: steps per sub-block (16 or 32, 0 means 16)                  */
 } odba_schedule;
 
 /* Return codes. 0 is success; 18..46 are the reference's exit codes

@@ -290,6 +290,8 @@ namespace {
 // ODBA_BAND_WAVES and ODBA_BAND_RING override both (tests).
 struct PairSchedule {
   bool strips = false;
+  bool small = false;  // lock-step strips of small pairs (C = 1, 2 or 4, `waves` waves per pair,
+                       // sub-blocks of `ring` = 16 or 32 steps)
   int order = 0;
   int C = 16, waves = 4, nblocks = 1, ring = 2048;
   // mixed: pairs of more than mixed_cells cells run first as bands of
@@ -314,6 +316,18 @@ bool bands_eligible(const odba_seqset* s) {
          set_minlen(s) >= 64;
 }
 
+// The lock-step small-pair kernel: float, every sequence 64..1024 samples
+// (one swath, so no swath-width rule of the open end applies).
+bool small_eligible(const odba_seqset* s) {
+  return s->dtype == ODBA_F32 && set_minlen(s) >= 64 && s->maxlen <= ODBA_W;
+}
+
+// Built-in small-pair shape: columns per lane and steps per sub-block.
+constexpr int kSmallC = 4, kSmallB = 32;
+
+// Waves per pair of the small-pair kernel at C columns per lane.
+int small_waves(const odba_seqset* s, int C) { return std::max(1, (s->maxlen + 64 * C - 1) / (64 * C)); }
+
 std::string shape_key(const odba_seqset* s, int os, int oe) {
   long long sum = 0;
   for (int L : s->h_lens) sum += L;
@@ -325,17 +339,19 @@ std::string shape_key(const odba_seqset* s, int os, int oe) {
 PairSchedule from_public(odba_ctx* c, const odba_schedule& u, long long npairs) {
   PairSchedule ps;
   ps.strips = u.kind == ODBA_SCHED_BANDS;
+  ps.small = u.kind == ODBA_SCHED_SMALL;
   ps.order = u.order;
   ps.C = u.band_c;
   ps.waves = u.band_waves;
-  ps.ring = u.ring;
-  if (!ps.strips) ps.nblocks = grid_blocks(c, npairs);
+  ps.ring = ps.small && u.ring == 0 ? 16 : u.ring;
+  if (ps.small) ps.nblocks = (int)std::max(1LL, std::min<long long>(npairs, (long long)c->cus * std::max(1, 16 / ps.waves)));
+  else if (!ps.strips) ps.nblocks = grid_blocks(c, npairs);
   return ps;
 }
 
 odba_schedule to_public(const PairSchedule& ps) {
   odba_schedule u;
-  u.kind = ps.strips ? ODBA_SCHED_BANDS : ODBA_SCHED_WAVE;
+  u.kind = ps.small ? ODBA_SCHED_SMALL : (ps.strips ? ODBA_SCHED_BANDS : ODBA_SCHED_WAVE);
   u.order = ps.order;
   u.band_c = ps.C;
   u.band_waves = ps.waves;
@@ -347,8 +363,17 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, 
   PairSchedule ps;
   const bool eligible = bands_eligible(s);
   auto t = c->tuned.find(shape_key(s, os, oe));
-  if (t != c->tuned.end() && (t->second.kind == ODBA_SCHED_WAVE || eligible)) {
+  const bool tuned_ok = t != c->tuned.end() &&
+                        (t->second.kind == ODBA_SCHED_WAVE || (t->second.kind == ODBA_SCHED_BANDS && eligible) ||
+                         (t->second.kind == ODBA_SCHED_SMALL && small_eligible(s)));
+  if (tuned_ok) {
     ps = from_public(c, t->second, npairs);
+  } else if (small_eligible(s) && npairs < 4LL * c->cus) {
+    // Fewer pairs than SIMDs, none longer than one swath: one wave per pair
+    // leaves each pair's sweep as the latency of a single wave; strips of
+    // 256 columns in lock step cut it, largest pair first.
+    odba_schedule u{ODBA_SCHED_SMALL, 1, kSmallC, small_waves(s, kSmallC), kSmallB};
+    ps = from_public(c, u, npairs);
   } else {
     // Fewer pairs than one wave per SIMD, and long enough that the strip
     // pipeline's fill (about 128 rows per strip) is small against |Y|.
@@ -372,7 +397,15 @@ PairSchedule choose_schedule(odba_ctx* c, const odba_seqset* s, int os, int oe, 
   // sweep every strip while a wave stops at the stop swath, so there the
   // largest pair must hold twice a share and the bands take the pairs
   // above one share.
-  if (!ps.strips && (t == c->tuned.end() || (env && !std::strcmp(env, "mixed"))) && eligible &&
+  if (env && !std::strcmp(env, "small") && small_eligible(s)) {
+    odba_schedule u{ODBA_SCHED_SMALL, 1, kSmallC, small_waves(s, kSmallC), kSmallB};
+    ps = from_public(c, u, npairs);
+  }
+  if (env && std::strcmp(env, "small") && ps.small) {  // another schedule asked for
+    ps = PairSchedule();
+    ps.nblocks = grid_blocks(c, npairs);
+  }
+  if (!ps.strips && !ps.small && (!tuned_ok || (env && !std::strcmp(env, "mixed"))) && eligible &&
       !(env && std::strcmp(env, "mixed"))) {
     long double sum = 0, sq = 0;
     std::vector<int> L(s->h_lens.begin(), s->h_lens.end());
@@ -534,6 +567,17 @@ int run_all_pairs(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, i
       HIPC(c, hipEventRecord(c->ev1, c->stream));
       HIPC(c, hipStreamSynchronize(c->stream));
     }
+  } else if (ps.small) {
+    if (ps.order == 1 && (rc = ensure_pair_order(c, s, npairs))) return rc;
+    unsigned long long* counter = (unsigned long long*)c->small;
+    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    HIPC(c, hipEventRecord(c->ev0, c->stream));
+    hipError_t e = launch_all_pairs_small(ps.C, ps.ring, ps.waves, os, oe, ps.nblocks, c->stream, (const float*)s->data, s->offs,
+                                          s->lens, s->n, npairs, counter, (float*)dist, flags,
+                                          ps.order == 1 ? s->order : nullptr);
+    if (e != hipSuccess) return fail(c, ODBA_E_HIP, "small-pair launch: %s", hipGetErrorString(e));
+    HIPC(c, hipEventRecord(c->ev1, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
   } else if (ps.strips) {
     rc = all_pairs_bands(c, s, ps, os, oe, npairs, dist, flags);
     if (rc) return rc;
@@ -647,6 +691,15 @@ int odba_tune_all_pairs(odba_ctx* c, odba_seqset* s, int os, int oe, int reps, o
     w.nblocks = grid_blocks(c, npairs);
     cand.push_back(w);
   }
+  if (small_eligible(s) && npairs < 16LL * c->cus) {
+    for (int C : {1, 2, 4})
+      for (int B : {16, 32})
+        for (int order = 0; order < 2; order++) {
+          if (small_waves(s, C) > 8 || 64 * C * small_waves(s, C) > ODBA_W) continue;
+          odba_schedule u{ODBA_SCHED_SMALL, order, C, small_waves(s, C), B};
+          cand.push_back(from_public(c, u, npairs));
+        }
+  }
   if (bands_eligible(s) && npairs < 16LL * c->cus) {
     for (int C : {1, 4, 16}) {
       const long long smax = (s->maxlen + 64LL * C - 1) / (64LL * C);
@@ -677,8 +730,9 @@ int odba_tune_all_pairs(odba_ctx* c, odba_seqset* s, int os, int oe, int reps, o
     std::sort(t.begin(), t.end());
     const double med = t[t.size() / 2];
     if (std::getenv("ODBA_TUNE_VERBOSE"))
-      std::fprintf(stderr, "odba tune: %s order %d C %d waves %d: %.4f ms\n", ps.strips ? "bands" : "wave", ps.order,
-                   ps.C, ps.waves, med);
+      std::fprintf(stderr, "odba tune: %s order %d C %d waves %d ring %d: %.4f ms\n",
+                   ps.small ? "small" : (ps.strips ? "bands" : "wave"), ps.order,
+                   ps.C, ps.waves, ps.ring, med);
     if (med < bms) { bms = med; bps = ps; }
   }
   hipFree(dist);
@@ -711,6 +765,13 @@ int odba_set_schedule(odba_ctx* c, odba_seqset* s, int os, int oe, const odba_sc
     if ((u->band_c != 1 && u->band_c != 4 && u->band_c != 16) || u->band_waves < 1 || u->band_waves > 16 ||
         u->ring < 128 || (u->ring & (u->ring - 1)))
       return fail(c, ODBA_E_INVALID, "band schedule (C %d, waves %d, ring %d)", u->band_c, u->band_waves, u->ring);
+  } else if (u->kind == ODBA_SCHED_SMALL) {
+    if (!small_eligible(s)) return fail(c, ODBA_E_INVALID, "the small-pair schedule needs float sequences of 64..1024 samples");
+    if ((u->band_c != 1 && u->band_c != 2 && u->band_c != 4) || u->band_waves != small_waves(s, u->band_c) ||
+        u->band_waves > 8 || 64 * u->band_c * u->band_waves > ODBA_W || (u->order != 0 && u->order != 1) ||
+        (u->ring != 0 && u->ring != 16 && u->ring != 32))
+      return fail(c, ODBA_E_INVALID, "small-pair schedule (C %d, waves %d, order %d, sub-block %d)", u->band_c,
+                  u->band_waves, u->order, u->ring);
   } else {
     return fail(c, ODBA_E_INVALID, "schedule kind %d", u->kind);
   }

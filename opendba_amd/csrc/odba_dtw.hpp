@@ -37,6 +37,12 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
                             const long long* offs, const int* lens, int n, long long npairs,
                             unsigned long long* counter, void* dist, void* esc, long long stride, int* flags,
                             const void* order);
+// Small pairs (float, every length 64..1024): a workgroup of nw waves per
+// pair, C = 1, 2 or 4 columns per lane, nw * 64 * C >= the longest length,
+// sub-blocks of B = 16 or 32 steps between barriers.
+hipError_t launch_all_pairs_small(int C, int B, int nw, int os, int oe, int nblocks, hipStream_t st,
+                                  const float* data, const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, float* dist, int* flags, const void* order);
 hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,
                             int njobs, unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                             long long stride);

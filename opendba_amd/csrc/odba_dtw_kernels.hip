@@ -1071,6 +1071,203 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Small pairs (float, every length 64..1024, so each pair is one swath and
+// none of the swath-width rules of the open end applies): a workgroup per
+// pair, wave k sweeping the pair's strip of 64*C columns [64*C*k, 64*C*(k+1)),
+// all waves in lock step: B-step sub-blocks separated by workgroup
+// barriers, strip k running 64 + B steps behind strip k-1. Its right-edge
+// value of row r (lane 63, 63 steps after lane 0 took the row) lands in an
+// LDS ring of 128 rows B + 1 steps before strip k+1's lane 0 needs it, in an
+// earlier sub-block, so strips hand over every sub-block instead of every 64
+// or 256 rows. Y is staged in LDS once per pair. The recurrence and its
+// initial values are run_swath's for a first swath, with C columns per lane;
+// a smaller C shortens each step's chain of dependent operations (2C + 1)
+// at the price of more strips. With open end the top row (a prefix minimum
+// along the row) runs strip by strip after the main rows. One strip per wave
+// needs C*64*waves >= |X|.
+// ---------------------------------------------------------------------------
+constexpr int kSmallMaxLen = 1024;  // longest sequence (one swath)
+constexpr int kSmallMaxWaves = 8;
+
+// One B-step sub-block of one strip. Every LDS read of the sub-block (its
+// rows of Y, and the left neighbour's edge rows, written in earlier
+// sub-blocks) is issued first, so their latency is paid once per sub-block;
+// the steps are then register-only, and the right-edge values of lane 63 go
+// to the ring afterwards. Strip 0 reads a ring of MAXV (no column -1).
+template <int C, int B, bool OS, bool OE>
+__device__ __forceinline__ void small_steps(int lb, int nrows, int r0, bool has_out, int lane,
+                                            const float* __restrict__ sy, const float* __restrict__ ring_in,
+                                            float* __restrict__ ring_out, const float (&x)[C], float (&D)[C],
+                                            float& diag_in) {
+  const int s0 = B * lb;
+  float yv[B], ev[B], ov[B];
+#pragma unroll
+  for (int u = 0; u < B; u++) yv[u] = sy[(r0 + s0 + u - lane) & (kSmallMaxLen - 1)];
+#pragma unroll
+  for (int u = 0; u < B; u++) ev[u] = ring_in[(r0 + s0 + u) & 127];
+  const bool steady = (s0 >= 63) && (s0 + B - 1 < nrows);
+  uint32_t code = 0;
+  if (steady) {
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const float left_in = Ar<float>::shr(ev[u], D[C - 1]);
+      float dg = diag_in;
+      diag_in = left_in;
+      float l = left_in;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const float up = D[q];
+        const float df = yv[u] - x[q];
+        const float v = cell<float, true, false>(up, l, dg, df * df, code);
+        dg = up;
+        D[q] = v;
+        l = v;
+      }
+      ov[u] = D[C - 1];
+    }
+    if (has_out && lane == 63) {
+#pragma unroll
+      for (int u = 0; u < B; u++) ring_out[(r0 + s0 + u - 63) & 127] = ov[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int s = s0 + u;
+      const bool active = (unsigned)(s - lane) < (unsigned)nrows;
+      const float left_in = Ar<float>::shr(ev[u], D[C - 1]);
+      float dg = diag_in;
+      diag_in = active ? left_in : diag_in;
+      float l = left_in;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const float up = D[q];
+        const float df = yv[u] - x[q];
+        const float v = cell<float, true, false>(up, l, dg, df * df, code);
+        dg = up;
+        D[q] = active ? v : up;
+        l = v;
+      }
+      ov[u] = D[C - 1];
+    }
+    if (has_out && lane == 63) {
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        const int row = s0 + u - 63;  // main-loop row index of lane 63 at this step
+        if ((unsigned)row < (unsigned)nrows) ring_out[(r0 + row) & 127] = ov[u];
+      }
+    }
+  }
+}
+
+template <int C, int B, bool OS, bool OE>
+__global__ void __launch_bounds__(64 * kSmallMaxWaves) k_all_pairs_small(
+    const float* __restrict__ data, const long long* __restrict__ offs, const int* __restrict__ lens, int n,
+    long long npairs, unsigned long long* __restrict__ counter, float* __restrict__ dist, int* __restrict__ flags,
+    const int2* __restrict__ order) {
+  constexpr int kLag = (64 + B) / B;  // sub-blocks between neighbouring strips
+  static_assert((64 + B) % B == 0, "sub-block length divides 64");
+  __shared__ float s_y[kSmallMaxLen];
+  __shared__ float s_ring[kSmallMaxWaves][128];
+  __shared__ float s_left[128];  // strip 0's left edge: column -1 does not exist
+  __shared__ float s_top[kSmallMaxWaves];
+  __shared__ unsigned long long s_p;
+  const float MAXV = Ar<float>::maxv();
+  const float RINF = Ar<float>::rowinf();
+  const int lane = threadIdx.x & 63;
+  const int k = uniform_i32((int)(threadIdx.x >> 6));
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) s_left[i] = MAXV;
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous pair
+    if (threadIdx.x == 0) s_p = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const long long p = (long long)uniform_u64(s_p);
+    if (p >= npairs) break;
+    int r, c;
+    if (order) {
+      r = uniform_i32(order[p].x);
+      c = uniform_i32(order[p].y);
+    } else {
+      decode_pair(p, npairs, r, c);
+    }
+    const int ly = uniform_i32(lens[r]), lx = uniform_i32(lens[c]);
+    const float* yp = data + offs[r];
+    for (int i = threadIdx.x; i < ly; i += blockDim.x) s_y[i] = yp[i];
+    const int S = (lx + 64 * C - 1) / (64 * C);
+    const int off = k * 64 * C;
+    const bool mine = k < S;
+    const bool first = (k == 0);
+    const bool has_out = k < S - 1;
+    float x[C], D[C];
+    const float* xp = data + offs[c] + off + lane * C;  // padded: in bounds
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      x[q] = xp[q];
+      D[q] = OS ? 0.0f : RINF;
+    }
+    float diag_in = (lane == 0 && first) ? (OS ? MAXV : 0.0f) : (OS ? 0.0f : RINF);
+    const int r0 = OS ? 1 : 0;
+    const int r1 = (OE && ly >= 2) ? ly - 2 : ly - 1;
+    const int nrows = r1 - r0 + 1;
+    const int nsub = nrows > 0 ? (nrows + 63 + B - 1) / B : 0;
+    const int G = nsub + (S - 1) * kLag;
+    const float* ring_in = first ? s_left : s_ring[k - 1];
+    float* ring_out = s_ring[k];
+    __syncthreads();  // Y staged
+    for (int g = 0; g < G; g++) {
+      const int lb = g - k * kLag;
+      if (mine && lb >= 0 && lb < nsub)
+        small_steps<C, B, OS, OE>(lb, nrows, r0, has_out, lane, s_y, ring_in, ring_out, x, D, diag_in);
+      __syncthreads();
+    }
+    if (OE && ly >= 2) {  // the top row, strip after strip (dtw.hpp:172-200)
+      for (int kk = 0; kk < S; kk++) {
+        if (k == kk) {
+          const int t = ly - 1;
+          const float yv = s_y[t];
+          const float lane0_diag = first ? MAXV : ring_in[(t - 1) & 127];
+          const float left0 = first ? MAXV : s_top[k - 1];
+          const float nb = Ar<float>::shr(lane0_diag, D[C - 1]);
+          const bool bare0 = (lane == 0) && first;  // the pair's column 0: no diagonal
+          float m[C];
+          float run = 0.0f;
+#pragma unroll
+          for (int q = 0; q < C; q++) {
+            const float df = yv - x[q];
+            const float dd = df * df;
+            const float U = D[q] + dd;
+            const float Gd = (q == 0 && bare0) ? nb : ((q == 0 ? nb : D[q - 1]) + dd);
+            const float mm = (Gd > U) ? U : Gd;
+            run = (q == 0) ? mm : Ar<float>::mn(run, mm);
+            m[q] = run;
+          }
+          const float pre = wave_excl_prefix_min<float>(m[C - 1], left0, lane);
+#pragma unroll
+          for (int q = 0; q < C; q++) D[q] = Ar<float>::mn(pre, m[q]);
+          const float top = __shfl(D[C - 1], 63, 64);
+          if (lane == 0) s_top[k] = top;
+        }
+        __syncthreads();
+      }
+    }
+    if (k == S - 1) {  // the pair's last column is in this strip
+      const int tl = lx - 1 - off;
+      const int ll = tl / C, qq = tl - ll * C;
+      float last = 0.0f;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const float v = __shfl(D[q], ll, 64);
+        if (q == qq) last = v;
+      }
+      if (lane == 0) {
+        const long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
+        dist[idx] = pair_distance<float>(last, ly, OE != OS);
+        if (flags) flags[idx] = 0;
+      }
+    }
+  }
+}
+
 __global__ void k_band_init(int* __restrict__ g_cnt, unsigned long long* __restrict__ g_free, int R,
                             int* __restrict__ slot_of, long long nclaims, int* __restrict__ abr, int ntab) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1899,6 +2096,44 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
       return launch_all_pairs_t<uint64_t, false>(os, oe, g, st, (const uint64_t*)data, offs, lens, n, npairs, counter, (uint64_t*)dist, (uint64_t*)esc, stride, flags, O);
   }
   return hipErrorInvalidValue;
+}
+
+template <int C, int B>
+static void launch_small_cb(int os, int oe, dim3 g, dim3 b, hipStream_t st, const float* data, const long long* offs,
+                            const int* lens, int n, long long npairs, unsigned long long* counter, float* dist,
+                            int* flags, const int2* order) {
+#define ODBA_SMALL(A, Z) hipLaunchKernelGGL((k_all_pairs_small<C, B, A, Z>), g, b, 0, st, data, offs, lens, n, npairs, counter, dist, flags, order)
+  if (os && oe) ODBA_SMALL(true, true); else if (os) ODBA_SMALL(true, false); else if (oe) ODBA_SMALL(false, true); else ODBA_SMALL(false, false);
+#undef ODBA_SMALL
+}
+
+hipError_t launch_all_pairs_small(int C, int B, int nw, int os, int oe, int nblocks, hipStream_t st,
+                                  const float* data, const long long* offs, const int* lens, int n, long long npairs,
+                                  unsigned long long* counter, float* dist, int* flags, const void* order) {
+  // every strip inside the first 1024 columns, which a padded sequence holds
+  if (nw < 1 || nw > kSmallMaxWaves || 64 * C * nw > kSmallMaxLen || nblocks < 1) return hipErrorInvalidValue;
+  const dim3 g(nblocks), b(64 * nw);
+  const int2* O = (const int2*)order;
+#define ODBA_SMALL_CB(CC, BB) launch_small_cb<CC, BB>(os, oe, g, b, st, data, offs, lens, n, npairs, counter, dist, flags, O)
+  if (B == 16) {
+    switch (C) {
+      case 1: ODBA_SMALL_CB(1, 16); break;
+      case 2: ODBA_SMALL_CB(2, 16); break;
+      case 4: ODBA_SMALL_CB(4, 16); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else if (B == 32) {
+    switch (C) {
+      case 1: ODBA_SMALL_CB(1, 32); break;
+      case 2: ODBA_SMALL_CB(2, 32); break;
+      case 4: ODBA_SMALL_CB(4, 32); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef ODBA_SMALL_CB
+  return hipGetLastError();
 }
 
 template <typename T, int C>

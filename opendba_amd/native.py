@@ -397,8 +397,9 @@ class SeqSet:
         if sched is None:
             self.ctx._check(self.ctx.L.odba_set_schedule(self.ctx.h, self.h, open_start, open_end, None))
             return
+        ring = 0 if int(sched.get("kind", 0)) == 2 else 2048  # small: 0 = 16-step sub-blocks
         u = Schedule(**{k: int(sched.get(k, d)) for k, d in
-                        (("kind", 0), ("order", 0), ("band_c", 16), ("band_waves", 4), ("ring", 2048))})
+                        (("kind", 0), ("order", 0), ("band_c", 16), ("band_waves", 4), ("ring", ring))})
         self.ctx._check(self.ctx.L.odba_set_schedule(self.ctx.h, self.h, open_start, open_end, C.byref(u)))
 
     def close(self):

@@ -7,7 +7,7 @@
  * smoke entry point and the benchmark's CPU-baseline leg load it; the product
  * library never links or calls it.
  *
- * Build: g++ -O2 -std=c++17 -ffp-contract=off -fwrapv (oracle/Makefile).
+ * Build: g++ -O3 -march=x86-64-v3 -std=c++17 -ffp-contract=off -fwrapv (oracle/Makefile).
  * -ffp-contract=off keeps `cost + diff*diff` as a multiply then an add, the
  * order the DTW recurrence is specified with; -fwrapv gives signed int
  * arithmetic the two's-complement wrap the GPU has.
