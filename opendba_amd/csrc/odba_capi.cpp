@@ -54,6 +54,8 @@ struct odba_seqset {
   int maxlen = 0;
   bool int_no_wrap = true;     // integer costs provably cannot wrap
   void* order = nullptr;       // device pair order (int2 per pair), largest cell count first
+  SmallPairRec* small_tab = nullptr;  // device pair records of the small-pair kernel
+  int small_tab_order = -1;           // the order they were built in
 };
 
 // Fixed device scratch of odba_dtw_all_pairs_noalloc: the pair counter in
@@ -272,6 +274,7 @@ int odba_seqset_create(odba_ctx* c, int dtype, const void* const* seqs, const ui
 int odba_seqset_destroy(odba_seqset* s) {
   if (!s) return ODBA_OK;
   if (s->order) hipFree(s->order);
+  if (s->small_tab) hipFree(s->small_tab);
   if (s->data) hipFree(s->data);
   if (s->offs) hipFree(s->offs);
   if (s->lens) hipFree(s->lens);
@@ -344,7 +347,9 @@ PairSchedule from_public(odba_ctx* c, const odba_schedule& u, long long npairs) 
   ps.C = u.band_c;
   ps.waves = u.band_waves;
   ps.ring = ps.small && u.ring == 0 ? 16 : u.ring;
-  if (ps.small) ps.nblocks = (int)std::max(1LL, std::min<long long>(npairs, (long long)c->cus * std::max(1, 16 / ps.waves)));
+  // small: a workgroup per pair, or, for more pairs than that holds, three
+  // persistent workgroups per CU (the LDS of one is about 41 KB)
+  if (ps.small) ps.nblocks = npairs <= 64LL * c->cus ? (int)std::max(1LL, npairs) : 3 * c->cus;
   else if (!ps.strips) ps.nblocks = grid_blocks(c, npairs);
   return ps;
 }
@@ -454,6 +459,37 @@ int ensure_pair_order(odba_ctx* c, odba_seqset* s, long long npairs) {
   }
   HIPC(c, hipMemcpyAsync(s->order, ord.data(), ord.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  return ODBA_OK;
+}
+
+// Pair records of the small-pair kernel in launch order (order 0: the
+// one-wave kernel's, second index descending then first index descending;
+// order 1: largest cell count first), built once per set and order.
+int ensure_small_tab(odba_ctx* c, odba_seqset* s, int order, long long npairs) {
+  if (s->small_tab && s->small_tab_order == order) return ODBA_OK;
+  std::vector<int2> ord;
+  ord.reserve((size_t)npairs);
+  for (int cc = s->n - 1; cc >= 1; cc--)
+    for (int r = cc - 1; r >= 0; r--) ord.push_back(make_int2(r, cc));
+  if (order == 1)
+    std::stable_sort(ord.begin(), ord.end(), [&](const int2& a, const int2& b) {
+      return (long long)s->h_lens[a.x] * s->h_lens[a.y] > (long long)s->h_lens[b.x] * s->h_lens[b.y];
+    });
+  std::vector<SmallPairRec> tab(ord.size());
+  const long long n = s->n;
+  for (size_t i = 0; i < ord.size(); i++) {
+    const int r = ord[i].x, cc = ord[i].y;
+    tab[i] = SmallPairRec{s->h_offs[r], s->h_offs[cc], (long long)r * (2 * n - r - 1) / 2 + (cc - r - 1),
+                          s->h_lens[r], s->h_lens[cc]};
+  }
+  if (!s->small_tab && hipMalloc(&s->small_tab, tab.size() * sizeof(SmallPairRec)) != hipSuccess) {
+    s->small_tab = nullptr;
+    return fail(c, ODBA_E_NOMEM, "cannot allocate the pair records");
+  }
+  HIPC(c, hipMemcpyAsync(s->small_tab, tab.data(), tab.size() * sizeof(SmallPairRec), hipMemcpyHostToDevice,
+                         c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  s->small_tab_order = order;
   return ODBA_OK;
 }
 
@@ -568,13 +604,15 @@ int run_all_pairs(odba_ctx* c, odba_seqset* s, const PairSchedule& ps, int os, i
       HIPC(c, hipStreamSynchronize(c->stream));
     }
   } else if (ps.small) {
-    if (ps.order == 1 && (rc = ensure_pair_order(c, s, npairs))) return rc;
-    unsigned long long* counter = (unsigned long long*)c->small;
-    HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    if ((rc = ensure_small_tab(c, s, ps.order, npairs))) return rc;
+    unsigned long long* counter = nullptr;  // one workgroup per pair: no claims
+    if (ps.nblocks < npairs) {
+      counter = (unsigned long long*)c->small;
+      HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    }
     HIPC(c, hipEventRecord(c->ev0, c->stream));
-    hipError_t e = launch_all_pairs_small(ps.C, ps.ring, ps.waves, os, oe, ps.nblocks, c->stream, (const float*)s->data, s->offs,
-                                          s->lens, s->n, npairs, counter, (float*)dist, flags,
-                                          ps.order == 1 ? s->order : nullptr);
+    hipError_t e = launch_all_pairs_small(ps.C, ps.ring, ps.waves, os, oe, ps.nblocks, c->stream,
+                                          (const float*)s->data, s->small_tab, npairs, counter, (float*)dist, flags);
     if (e != hipSuccess) return fail(c, ODBA_E_HIP, "small-pair launch: %s", hipGetErrorString(e));
     HIPC(c, hipEventRecord(c->ev1, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));

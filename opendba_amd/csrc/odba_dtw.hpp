@@ -39,10 +39,18 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
                             const void* order);
 // Small pairs (float, every length 64..1024): a workgroup of nw waves per
 // pair, C = 1, 2 or 4 columns per lane, nw * 64 * C >= the longest length,
-// sub-blocks of B = 16 or 32 steps between barriers.
+// sub-blocks of B = 16 or 32 steps between barriers. counter == nullptr:
+// one workgroup per pair (nblocks >= npairs); otherwise persistent
+// workgroups claim pairs from *counter (zeroed by the caller).
+// The pairs in launch order, one record each: element offsets of Y and X in
+// the packed data, the pair's index in the distance array, the lengths.
+struct SmallPairRec {
+  long long oy, ox, idx;
+  int ly, lx;
+};
 hipError_t launch_all_pairs_small(int C, int B, int nw, int os, int oe, int nblocks, hipStream_t st,
-                                  const float* data, const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, float* dist, int* flags, const void* order);
+                                  const float* data, const SmallPairRec* tab, long long npairs,
+                                  unsigned long long* counter, float* dist, int* flags);
 hipError_t launch_path_jobs(int dtype, bool minfirst, int os, int oe, int nblocks, hipStream_t st, const void* jobs,
                             int njobs, unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                             long long stride);

@@ -1078,37 +1078,66 @@ __global__ void __launch_bounds__(1024) k_all_pairs_bands(const T* __restrict__ 
 // all waves in lock step: B-step sub-blocks separated by workgroup
 // barriers, strip k running 64 + B steps behind strip k-1. Its right-edge
 // value of row r (lane 63, 63 steps after lane 0 took the row) lands in an
-// LDS ring of 128 rows B + 1 steps before strip k+1's lane 0 needs it, in an
-// earlier sub-block, so strips hand over every sub-block instead of every 64
-// or 256 rows. Y is staged in LDS once per pair. The recurrence and its
-// initial values are run_swath's for a first swath, with C columns per lane;
-// a smaller C shortens each step's chain of dependent operations (2C + 1)
-// at the price of more strips. With open end the top row (a prefix minimum
-// along the row) runs strip by strip after the main rows. One strip per wave
-// needs C*64*waves >= |X|.
+// LDS column of the strip B + 1 steps before strip k+1's lane 0 needs it, in
+// an earlier sub-block, so strips hand over every sub-block instead of
+// every 64 or 256 rows. Y is staged in LDS once per pair, between guard
+// rows, and each strip's edge column has a row of LDS per row of the pair:
+// no index wraps, so every LDS access of a sub-block is one base register
+// plus an immediate offset. The recurrence and its initial values are
+// run_swath's for a first swath, with C columns per lane; a smaller C
+// shortens each step's instruction stream (4C + 2) at the price of more
+// strips. With open end the top row (a prefix minimum along the row) runs
+// strip by strip after the main rows. One strip per wave needs
+// C*64*waves >= |X|.
 // ---------------------------------------------------------------------------
 constexpr int kSmallMaxLen = 1024;  // longest sequence (one swath)
 constexpr int kSmallMaxWaves = 8;
+constexpr int kSmallYPad = 64;  // guard rows before Y: lane l reads row s - l
+// Y with its guards: steps run to nrows + 63 + B - 1 (rounded up to a
+// sub-block), lane 0 reading row r0 + step.
+constexpr int kSmallYLen = kSmallYPad + kSmallMaxLen + 64 + 32;
+// Edge column of a strip: row r at index r + kSmallEdgePad, so lane 63's
+// rows of a sub-block (s0 - 63 .. s0 - 63 + B - 1, s0 a multiple of 16) start
+// 16-byte aligned and every row it sweeps, before the first (>= -63) or
+// past the last, has a slot; lane 0 of the next strip reads rows up to its
+// last step.
+constexpr int kSmallEdgePad = 67;
+constexpr int kSmallEdgeLen = kSmallEdgePad + kSmallMaxLen + 64 + 32 + 1;
 
 // One B-step sub-block of one strip. Every LDS read of the sub-block (its
 // rows of Y, and the left neighbour's edge rows, written in earlier
 // sub-blocks) is issued first, so their latency is paid once per sub-block;
 // the steps are then register-only, and the right-edge values of lane 63 go
-// to the ring afterwards. Strip 0 reads a ring of MAXV (no column -1).
+// to the strip's edge column afterwards, all B of them: a row before the
+// first lands in a guard slot nobody reads, a row past the last is read only
+// by a lane that is past its last row too. Strip 0 reads the last wave's
+// column, which no strip writes and holds MAXV (no column -1).
+//
+// Lane l takes row s - l at step s. Before its first row (s < l) a lane
+// must leave its state alone, and after its last row too. pro_free says the
+// first need not be enforced: without open start the row above the first
+// row is +inf in every column but the corner, and Y's guard rows before the
+// first row hold +inf, so with X finite (and not +inf) a lane's steps before
+// its first row compute min3(...) + inf = +inf, exactly the values of the row
+// above that it holds, and hand +inf on as left and diagonal. Then only a
+// sub-block that reaches past the last row masks, and only the state update.
 template <int C, int B, bool OS, bool OE>
-__device__ __forceinline__ void small_steps(int lb, int nrows, int r0, bool has_out, int lane,
-                                            const float* __restrict__ sy, const float* __restrict__ ring_in,
-                                            float* __restrict__ ring_out, const float (&x)[C], float (&D)[C],
+__device__ __forceinline__ void small_steps(int lb, int nrows, int r0, bool has_out, bool pro_free, int lane,
+                                            const float* __restrict__ sy, const float* __restrict__ edge_in,
+                                            float* __restrict__ edge_out, const float (&x)[C], float (&D)[C],
                                             float& diag_in) {
   const int s0 = B * lb;
   float yv[B], ev[B], ov[B];
+  const float* yb = sy + (kSmallYPad + r0 + s0 - lane);  // lane l at step s0 + u: row r0 + s0 + u - l
+  const float* eb = edge_in + (kSmallEdgePad + s0);      // lane 0 at step s0 + u: edge row s0 + u
 #pragma unroll
-  for (int u = 0; u < B; u++) yv[u] = sy[(r0 + s0 + u - lane) & (kSmallMaxLen - 1)];
+  for (int u = 0; u < B; u++) yv[u] = yb[u];
 #pragma unroll
-  for (int u = 0; u < B; u++) ev[u] = ring_in[(r0 + s0 + u) & 127];
-  const bool steady = (s0 >= 63) && (s0 + B - 1 < nrows);
+  for (int u = 0; u < B; u++) ev[u] = eb[u];
+  const bool head = s0 < 63;             // some lane is before its first row
+  const bool tail = s0 + B - 1 >= nrows;  // some lane passes its last row
   uint32_t code = 0;
-  if (steady) {
+  if (!tail && (!head || pro_free)) {
 #pragma unroll
     for (int u = 0; u < B; u++) {
       const float left_in = Ar<float>::shr(ev[u], D[C - 1]);
@@ -1126,9 +1155,24 @@ __device__ __forceinline__ void small_steps(int lb, int nrows, int r0, bool has_
       }
       ov[u] = D[C - 1];
     }
-    if (has_out && lane == 63) {
+  } else if (pro_free) {
 #pragma unroll
-      for (int u = 0; u < B; u++) ring_out[(r0 + s0 + u - 63) & 127] = ov[u];
+    for (int u = 0; u < B; u++) {
+      const bool live = lane > s0 + u - nrows;  // row s0 + u - lane is not past the last
+      const float left_in = Ar<float>::shr(ev[u], D[C - 1]);
+      float dg = diag_in;
+      diag_in = left_in;
+      float l = left_in;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const float up = D[q];
+        const float df = yv[u] - x[q];
+        const float v = cell<float, true, false>(up, l, dg, df * df, code);
+        dg = up;
+        D[q] = live ? v : up;
+        l = v;
+      }
+      ov[u] = D[C - 1];
     }
   } else {
 #pragma unroll
@@ -1150,82 +1194,87 @@ __device__ __forceinline__ void small_steps(int lb, int nrows, int r0, bool has_
       }
       ov[u] = D[C - 1];
     }
-    if (has_out && lane == 63) {
+  }
+  if (has_out && lane == 63) {
+    float* ob = edge_out + (kSmallEdgePad - 63 + s0);  // lane 63 at step s0 + u: row s0 + u - 63
 #pragma unroll
-      for (int u = 0; u < B; u++) {
-        const int row = s0 + u - 63;  // main-loop row index of lane 63 at this step
-        if ((unsigned)row < (unsigned)nrows) ring_out[(r0 + row) & 127] = ov[u];
-      }
-    }
+    for (int u = 0; u < B; u += 4) *reinterpret_cast<float4*>(ob + u) = make_float4(ov[u], ov[u + 1], ov[u + 2], ov[u + 3]);
   }
 }
 
+// counter == nullptr: the grid holds one workgroup per pair (pair =
+// blockIdx.x); otherwise workgroups claim pairs from *counter.
 template <int C, int B, bool OS, bool OE>
 __global__ void __launch_bounds__(64 * kSmallMaxWaves) k_all_pairs_small(
-    const float* __restrict__ data, const long long* __restrict__ offs, const int* __restrict__ lens, int n,
-    long long npairs, unsigned long long* __restrict__ counter, float* __restrict__ dist, int* __restrict__ flags,
-    const int2* __restrict__ order) {
+    const float* __restrict__ data, const SmallPairRec* __restrict__ tab, long long npairs,
+    unsigned long long* __restrict__ counter, float* __restrict__ dist, int* __restrict__ flags) {
   constexpr int kLag = (64 + B) / B;  // sub-blocks between neighbouring strips
   static_assert((64 + B) % B == 0, "sub-block length divides 64");
-  __shared__ float s_y[kSmallMaxLen];
-  __shared__ float s_ring[kSmallMaxWaves][128];
-  __shared__ float s_left[128];  // strip 0's left edge: column -1 does not exist
+  __shared__ float s_y[kSmallYLen];
+  __shared__ __attribute__((aligned(16))) float s_edge[kSmallMaxWaves][kSmallEdgeLen];
+  static_assert(kSmallEdgeLen % 4 == 0 && (kSmallEdgePad - 63) % 4 == 0, "aligned edge rows");
   __shared__ float s_top[kSmallMaxWaves];
   __shared__ unsigned long long s_p;
   const float MAXV = Ar<float>::maxv();
   const float RINF = Ar<float>::rowinf();
   const int lane = threadIdx.x & 63;
   const int k = uniform_i32((int)(threadIdx.x >> 6));
-  for (int i = threadIdx.x; i < 128; i += blockDim.x) s_left[i] = MAXV;
-  for (;;) {
-    __syncthreads();  // every wave is done with the previous pair
-    if (threadIdx.x == 0) s_p = atomicAdd(counter, 1ull);
-    __syncthreads();
-    const long long p = (long long)uniform_u64(s_p);
-    if (p >= npairs) break;
-    int r, c;
-    if (order) {
-      r = uniform_i32(order[p].x);
-      c = uniform_i32(order[p].y);
+  const int nw = (int)(blockDim.x >> 6);
+  for (int i = threadIdx.x; i < kSmallEdgeLen; i += blockDim.x) s_edge[nw - 1][i] = MAXV;
+  // Y's guard rows before the first row: +inf (the prologue without masks;
+  // without open start lanes read guard rows kSmallYPad - 63 .. kSmallYPad - 1)
+  for (int i = threadIdx.x; i < kSmallYPad; i += blockDim.x) s_y[i] = RINF;
+  for (long long it = 0;; ++it) {
+    long long p;
+    if (counter) {
+      __syncthreads();  // every wave is done with the previous pair
+      if (threadIdx.x == 0) s_p = atomicAdd(counter, 1ull);
+      __syncthreads();
+      p = (long long)uniform_u64(s_p);
     } else {
-      decode_pair(p, npairs, r, c);
+      p = it == 0 ? (long long)blockIdx.x : npairs;
     }
-    const int ly = uniform_i32(lens[r]), lx = uniform_i32(lens[c]);
-    const float* yp = data + offs[r];
-    for (int i = threadIdx.x; i < ly; i += blockDim.x) s_y[i] = yp[i];
+    if (p >= npairs) break;
+    const SmallPairRec P = tab[p];
+    const int ly = uniform_i32(P.ly), lx = uniform_i32(P.lx);
+    const float* yp = data + P.oy;
+    for (int i = threadIdx.x; i < ly; i += blockDim.x) s_y[kSmallYPad + i] = yp[i];
     const int S = (lx + 64 * C - 1) / (64 * C);
     const int off = k * 64 * C;
     const bool mine = k < S;
     const bool first = (k == 0);
     const bool has_out = k < S - 1;
     float x[C], D[C];
-    const float* xp = data + offs[c] + off + lane * C;  // padded: in bounds
+    const float* xp = data + P.ox + off + lane * C;  // padded: in bounds
+    bool xfin = true;
 #pragma unroll
     for (int q = 0; q < C; q++) {
       x[q] = xp[q];
       D[q] = OS ? 0.0f : RINF;
+      xfin = xfin && (x[q] < RINF);  // false for +inf and NaN
     }
+    const bool pro_free = !OS && __all(xfin);
     float diag_in = (lane == 0 && first) ? (OS ? MAXV : 0.0f) : (OS ? 0.0f : RINF);
     const int r0 = OS ? 1 : 0;
     const int r1 = (OE && ly >= 2) ? ly - 2 : ly - 1;
     const int nrows = r1 - r0 + 1;
     const int nsub = nrows > 0 ? (nrows + 63 + B - 1) / B : 0;
     const int G = nsub + (S - 1) * kLag;
-    const float* ring_in = first ? s_left : s_ring[k - 1];
-    float* ring_out = s_ring[k];
+    const float* edge_in = first ? s_edge[nw - 1] : s_edge[k - 1];
+    float* edge_out = s_edge[k];
     __syncthreads();  // Y staged
     for (int g = 0; g < G; g++) {
       const int lb = g - k * kLag;
       if (mine && lb >= 0 && lb < nsub)
-        small_steps<C, B, OS, OE>(lb, nrows, r0, has_out, lane, s_y, ring_in, ring_out, x, D, diag_in);
+        small_steps<C, B, OS, OE>(lb, nrows, r0, has_out, pro_free, lane, s_y, edge_in, edge_out, x, D, diag_in);
       __syncthreads();
     }
     if (OE && ly >= 2) {  // the top row, strip after strip (dtw.hpp:172-200)
       for (int kk = 0; kk < S; kk++) {
         if (k == kk) {
           const int t = ly - 1;
-          const float yv = s_y[t];
-          const float lane0_diag = first ? MAXV : ring_in[(t - 1) & 127];
+          const float yv = s_y[kSmallYPad + t];
+          const float lane0_diag = first ? MAXV : edge_in[kSmallEdgePad + nrows - 1];  // row t - 1 of the left strip
           const float left0 = first ? MAXV : s_top[k - 1];
           const float nb = Ar<float>::shr(lane0_diag, D[C - 1]);
           const bool bare0 = (lane == 0) && first;  // the pair's column 0: no diagonal
@@ -1260,11 +1309,11 @@ __global__ void __launch_bounds__(64 * kSmallMaxWaves) k_all_pairs_small(
         if (q == qq) last = v;
       }
       if (lane == 0) {
-        const long long idx = (long long)r * (2LL * n - r - 1) / 2 + (c - r - 1);
-        dist[idx] = pair_distance<float>(last, ly, OE != OS);
-        if (flags) flags[idx] = 0;
+        dist[P.idx] = pair_distance<float>(last, ly, OE != OS);
+        if (flags) flags[P.idx] = 0;
       }
     }
+    if (!counter) break;
   }
 }
 
@@ -2099,22 +2148,22 @@ hipError_t launch_all_pairs(int dtype, bool minfirst, int os, int oe, int nblock
 }
 
 template <int C, int B>
-static void launch_small_cb(int os, int oe, dim3 g, dim3 b, hipStream_t st, const float* data, const long long* offs,
-                            const int* lens, int n, long long npairs, unsigned long long* counter, float* dist,
-                            int* flags, const int2* order) {
-#define ODBA_SMALL(A, Z) hipLaunchKernelGGL((k_all_pairs_small<C, B, A, Z>), g, b, 0, st, data, offs, lens, n, npairs, counter, dist, flags, order)
+static void launch_small_cb(int os, int oe, dim3 g, dim3 b, hipStream_t st, const float* data,
+                            const SmallPairRec* tab, long long npairs, unsigned long long* counter, float* dist,
+                            int* flags) {
+#define ODBA_SMALL(A, Z) hipLaunchKernelGGL((k_all_pairs_small<C, B, A, Z>), g, b, 0, st, data, tab, npairs, counter, dist, flags)
   if (os && oe) ODBA_SMALL(true, true); else if (os) ODBA_SMALL(true, false); else if (oe) ODBA_SMALL(false, true); else ODBA_SMALL(false, false);
 #undef ODBA_SMALL
 }
 
 hipError_t launch_all_pairs_small(int C, int B, int nw, int os, int oe, int nblocks, hipStream_t st,
-                                  const float* data, const long long* offs, const int* lens, int n, long long npairs,
-                                  unsigned long long* counter, float* dist, int* flags, const void* order) {
+                                  const float* data, const SmallPairRec* tab, long long npairs,
+                                  unsigned long long* counter, float* dist, int* flags) {
   // every strip inside the first 1024 columns, which a padded sequence holds
   if (nw < 1 || nw > kSmallMaxWaves || 64 * C * nw > kSmallMaxLen || nblocks < 1) return hipErrorInvalidValue;
+  if (!counter && nblocks < npairs) return hipErrorInvalidValue;  // without claims, a workgroup per pair
   const dim3 g(nblocks), b(64 * nw);
-  const int2* O = (const int2*)order;
-#define ODBA_SMALL_CB(CC, BB) launch_small_cb<CC, BB>(os, oe, g, b, st, data, offs, lens, n, npairs, counter, dist, flags, O)
+#define ODBA_SMALL_CB(CC, BB) launch_small_cb<CC, BB>(os, oe, g, b, st, data, tab, npairs, counter, dist, flags)
   if (B == 16) {
     switch (C) {
       case 1: ODBA_SMALL_CB(1, 16); break;

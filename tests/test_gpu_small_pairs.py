@@ -111,3 +111,29 @@ def test_small_schedule_refusals(ctx):
                 s.set_schedule(cand)
         finally:
             s.close()
+
+
+@pytest.mark.parametrize("os_,oe", [(0, 0), (0, 1), (1, 1)])
+def test_small_schedule_infinite_values(ctx, os_, oe):
+    """Sequences holding +inf and -inf: without open start a strip whose
+    columns hold +inf keeps the masked prologue (its steps before a lane's
+    first row would compute inf - inf), one holding -inf does not. NaN
+    samples are outside the parity claim of every float kernel (min-first
+    drops NaN where the reference's comparisons keep it), so none is used."""
+    seqs = _set(33 + 2 * os_ + oe, [70, 130, 200, 257, 300, 390, 400, 512])
+    seqs[1][5] = np.inf
+    seqs[3][200] = np.inf
+    seqs[5][17] = -np.inf
+    exp = O.all_pairs(seqs, os_, oe)
+    s = ctx.seqset(seqs)
+    try:
+        for C, B in ((1, 16), (2, 32), (4, 16), (4, 32)):
+            cand = dict(kind=2, order=1, band_c=C, band_waves=-(-512 // (64 * C)), ring=B)
+            print(f"small schedule {cand} os={os_} oe={oe}", flush=True)
+            s.set_schedule(cand, os_, oe)
+            got, _, _ = s.all_pairs(os_, oe)
+            nan = np.isnan(exp)
+            assert np.array_equal(np.isnan(got), nan), cand
+            assert np.array_equal(got[~nan].view(np.uint32), exp[~nan].view(np.uint32)), cand
+    finally:
+        s.close()
