@@ -305,7 +305,7 @@ def run_stripes(args, N, WL):
     }
 
 
-def serve_leg(args, N, WL, jobs_n=32, scale=1.0, slots=8, max_rounds=250):
+def serve_leg(args, N, WL, jobs_n=32, scale=1.0, slots=4, max_rounds=250):
     """Serving: a seeded Poisson trace (mean 20 jobs/s) of DBA jobs with
     16-512 series (scaled by scale), lengths log-uniform 500-50k (scaled),
     float32 random walks or int16 squiggles as float, global or open-end;
@@ -490,7 +490,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="serve: series-count and length scale (1.0: the serving configuration as named)")
-    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent job slots (8 measured best at full size)")
+    ap.add_argument("--slots", type=int, default=4,
+                    help="serve: preparation slots; the rounds of prepared jobs are packed by the server's engines")
     ap.add_argument("--max-rounds", type=int, default=250, help="serve: DBA round limit")
     ap.add_argument("--dba-rounds", type=int, default=250,
                     help="squiggles: DBA round limit of the perform_dba run after the timed all-vs-all (0: skip)")
@@ -513,8 +514,9 @@ def main():
         # this when it starts; with fewer queues than streams, slots share
         # queues and their kernels run one after another). The default run
         # serves too, after its headline.
-        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < args.slots:
-            os.environ["GPU_MAX_HW_QUEUES"] = str(args.slots)
+        want = min(32, args.slots + 8)  # the slots' streams and two rounds engines' four streams each
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     from opendba_amd import native as N
     from opendba_amd import workloads as WL
 

@@ -271,6 +271,40 @@ int odba_kseg(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t*
 uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length);
 
 /*
+ * Packed DBA rounds across jobs (no counterpart in the reference; for
+ * servers that batch the rounds of concurrent jobs). odba_job_prepare runs
+ * the first half of odba_perform_dba for one job on ctx's stream -
+ * sanitize, stable length sort, upload, z-norm (norm != 0), all-vs-all,
+ * medoid - and keeps its sequences resident on ctx's device. A rounds
+ * engine holds prepared jobs of one value type and one (open start, open
+ * end) mode; each odba_rounds_step runs one DBA round (full path matrices)
+ * of every job it holds, packed into shared launches on the engine
+ * context's stream, in batches of whole jobs whose step words fit
+ * word_bytes. Each job keeps its own stop rule (delta 0, the open-mode
+ * repeat test, max_rounds) and its result (centroid rescaled by the
+ * medoid's mu/sigma when normalised, medoid index in sorted order, rounds)
+ * equals odba_perform_dba's with ODBA_PATH_FULL. Jobs that ended are
+ * returned by odba_rounds_step (up to `capacity` per call, the rest on
+ * later calls); n_active counts jobs held or not yet returned. An engine
+ * is used by one thread at a time; jobs may be prepared on other contexts
+ * of the same device. odba_job_destroy frees a job's device memory (also
+ * the engine's buffers for it) and must not be called while an engine
+ * holds the job.
+ */
+typedef struct odba_job odba_job;
+typedef struct odba_rounds odba_rounds;
+int odba_job_prepare(odba_ctx* ctx, int dtype, const void* const* seqs, const uint64_t* lens, int n,
+                     int use_open_start, int use_open_end, int norm, int max_rounds, odba_job** out);
+int odba_job_result(const odba_job* job, void* centroid_out, uint64_t capacity, uint64_t* centroid_len,
+                    int* medoid_sorted, int* rounds, int* job_rc);
+int odba_job_destroy(odba_job* job);
+int odba_rounds_create(odba_ctx* ctx, int dtype, int use_open_start, int use_open_end, uint64_t word_bytes,
+                       odba_rounds** out);
+int odba_rounds_add(odba_rounds* engine, odba_job* job);
+int odba_rounds_step(odba_rounds* engine, odba_job** finished, int capacity, int* n_finished, int* n_active);
+int odba_rounds_destroy(odba_rounds* engine);
+
+/*
  * Serving (no counterpart in the reference, which runs one job per process):
  * DBA jobs - the performDBA pipeline for one cluster, as odba_perform_dba
  * with the full path matrix - are submitted at any time, admitted in

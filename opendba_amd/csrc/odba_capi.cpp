@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -96,6 +97,17 @@ int fail(odba_ctx* c, int code, const char* fmt, ...) {
     hipError_t e_ = (call);                                                               \
     if (e_ != hipSuccess) return fail(ctx, ODBA_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
+
+#define ODBA_DISPATCH(dtype, CALL)                        \
+  switch (dtype) {                                        \
+    case ODBA_I32: { typedef int32_t T; return CALL; }    \
+    case ODBA_U32: { typedef uint32_t T; return CALL; }   \
+    case ODBA_U64: { typedef uint64_t T; return CALL; }   \
+    case ODBA_F32: { typedef float T; return CALL; }      \
+    case ODBA_F64: { typedef double T; return CALL; }     \
+    default: return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype); \
+  }
+
 
 int ensure_scratch(odba_ctx* c, size_t bytes) {
   if (bytes <= c->esc_bytes) return ODBA_OK;
@@ -927,7 +939,7 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   (void)err;
   hipError_t e;
   if (swath_bands) {
-    const int nw = std::max(1, std::min(16, env_int("ODBA_PATH_BAND_WAVES", 4)));
+    const int nw = std::max(1, std::min(kPathStripWaves, env_int("ODBA_PATH_BAND_WAVES", 4)));
     const int per_cu = std::max(1, env_int("ODBA_PATH_BANDS_PER_CU", 2));
     const long long max_claims = (long long)njobs * ((kmax + nw - 1) / nw);
     const int nblocks = (int)std::max(1LL, std::min(max_claims, (long long)per_cu * c->cus));
@@ -967,8 +979,8 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
     e = launch_path_bands(os, oe, nblocks, c->stream, jobs_dev, njobs, max_claims, counter64, mv, mtop, abr, base, ring,
                           base + ring_bytes, gstride, sync, R, err, limit);
   } else if (strips) {
-    const int waves = std::min(16, quarters ? 4 * kmax : kmax);
-    const int per_cu = std::max(1, 16 / waves);
+    const int waves = std::min(kPathStripWaves, quarters ? 4 * kmax : kmax);
+    const int per_cu = std::max(1, kPathStripWaves / waves);
     const int nblocks = std::max(1, std::min(njobs, per_cu * c->cus));
     int rc = ensure_scratch(c, (size_t)nblocks * 2 * waves * stride * es);
     if (rc) return rc;
@@ -1732,23 +1744,57 @@ template <typename T> int medoid_single(const T* d, const std::vector<uint64_t>&
   return 0;
 }
 
-// Device-resident pipeline of performDBA for one cluster: the sanitized,
-// sorted members are packed and uploaded once, z-normalised in place, and
-// that one buffer feeds the all-vs-all and every DBA round.
+// The first half of performDBA for one cluster (dba.hpp:863-990): the
+// sanitized members, stably sorted by length, are packed and uploaded once,
+// z-normalised in place, and that one buffer feeds the all-vs-all; the
+// medoid follows. The buffer stays resident for the DBA rounds, which run
+// either here (perform_dba_t) or packed with other jobs' rounds (RoundEngine).
+template <typename T> struct Prepared {
+  int device = 0;
+  int n = 0, os = 0, oe = 0, norm = 0;
+  T* d = nullptr;            // packed, padded, z-normalised members in sorted order
+  long long* o = nullptr;    // device offsets
+  int* ln = nullptr;         // device lengths
+  std::vector<long long> ho;
+  std::vector<int> hl;
+  std::vector<uint64_t> l;
+  std::vector<double> mu, sg;
+  std::vector<int> perm;     // sorted position -> input index
+  bool no_wrap = true;
+  int med = -1, lc = 0;
+  Prepared() = default;
+  Prepared(const Prepared&) = delete;
+  Prepared& operator=(const Prepared&) = delete;
+  void release() {
+    if (d || o || ln) hipSetDevice(device);
+    for (void* p : {(void*)d, (void*)o, (void*)ln})
+      if (p) hipFree(p);
+    d = nullptr;
+    o = nullptr;
+    ln = nullptr;
+  }
+  ~Prepared() { release(); }
+};
+
 template <typename T>
-int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
-                  int max_rounds, int policy, uint64_t free_mem, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
-                  T* dists_out) {
+int prepare_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
+              Prepared<T>& P, T* dists_out) {
   if (n < 2) return fail(c, ODBA_E_NOT_ENOUGH_SEQUENCES, "at least two sequences are needed, got %d", n);
+  P.device = c->device;
+  P.n = n; P.os = os; P.oe = oe; P.norm = norm;
   std::vector<uint64_t> lens(in_lens, in_lens + n);
   for (int i = 0; i < n; i++)
     if (lens[i] && sanitize_drop_last<T>(in[i][lens[i] - 1])) lens[i]--;  // dba.hpp:868-872
-  std::vector<int> perm(n);
+  std::vector<int>& perm = P.perm;
+  perm.resize(n);
   for (int i = 0; i < n; i++) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return lens[a] < lens[b]; });  // dba.hpp:876-885
-  std::vector<uint64_t> l(n);
-  std::vector<long long> ho(n);
-  std::vector<int> hl(n);
+  std::vector<uint64_t>& l = P.l;
+  std::vector<long long>& ho = P.ho;
+  std::vector<int>& hl = P.hl;
+  l.resize(n);
+  ho.resize(n);
+  hl.resize(n);
   long long total = 0;
   int maxlen = 0;
   for (int i = 0; i < n; i++) {
@@ -1761,37 +1807,38 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   }
   std::vector<T> hb((size_t)total, T(0));
   for (int i = 0; i < n; i++) std::memcpy(&hb[ho[i]], in[perm[i]], l[i] * sizeof(T));
-  struct Buf {
-    T* d = nullptr; long long* o = nullptr; int* ln = nullptr; double* mu = nullptr; double* sg = nullptr;
-    ~Buf() { for (void* p : {(void*)d, (void*)o, (void*)ln, (void*)mu, (void*)sg}) if (p) hipFree(p); }
-  } B;
-  HIPC(c, hipMalloc((void**)&B.d, hb.size() * sizeof(T)));
-  HIPC(c, hipMalloc((void**)&B.o, n * sizeof(long long)));
-  HIPC(c, hipMalloc((void**)&B.ln, n * sizeof(int)));
-  HIPC(c, hipMemcpyAsync(B.d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(B.o, ho.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(B.ln, hl.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  std::vector<double> mu(n, 0.0), sg(n, 1.0);
+  struct Stats {
+    double* mu = nullptr; double* sg = nullptr;
+    ~Stats() { for (void* p : {(void*)mu, (void*)sg}) if (p) hipFree(p); }
+  } S;
+  HIPC(c, hipMalloc((void**)&P.d, hb.size() * sizeof(T)));
+  HIPC(c, hipMalloc((void**)&P.o, n * sizeof(long long)));
+  HIPC(c, hipMalloc((void**)&P.ln, n * sizeof(int)));
+  HIPC(c, hipMemcpyAsync(P.d, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(P.o, ho.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(P.ln, hl.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  P.mu.assign(n, 0.0);
+  P.sg.assign(n, 1.0);
   if (norm) {
-    HIPC(c, hipMalloc((void**)&B.mu, n * sizeof(double)));
-    HIPC(c, hipMalloc((void**)&B.sg, n * sizeof(double)));
-    HIPC(c, launch_znorm(dtype_of<T>(), c->stream, B.d, B.o, B.ln, n, B.mu, B.sg));
-    HIPC(c, hipMemcpyAsync(mu.data(), B.mu, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipMemcpyAsync(sg.data(), B.sg, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMalloc((void**)&S.mu, n * sizeof(double)));
+    HIPC(c, hipMalloc((void**)&S.sg, n * sizeof(double)));
+    HIPC(c, launch_znorm(dtype_of<T>(), c->stream, P.d, P.o, P.ln, n, S.mu, S.sg));
+    HIPC(c, hipMemcpyAsync(P.mu.data(), S.mu, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(P.sg.data(), S.sg, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  bool no_wrap = true;
+  P.no_wrap = true;
   if (!is_float<T>()) {  // the wrap bound needs the (normalised) values
-    if (norm) HIPC(c, hipMemcpyAsync(hb.data(), B.d, hb.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    if (norm) HIPC(c, hipMemcpyAsync(hb.data(), P.d, hb.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     std::vector<const T*> sp(n);
     for (int i = 0; i < n; i++) sp[i] = &hb[ho[i]];
-    no_wrap = int_cannot_wrap<T>(sp.data(), l.data(), n);
+    P.no_wrap = int_cannot_wrap<T>(sp.data(), l.data(), n);
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   // the all-vs-all on the resident buffer (a seqset view that owns nothing)
   odba_seqset view;
-  view.ctx = c; view.dtype = dtype_of<T>(); view.n = n; view.data = B.d; view.offs = B.o; view.lens = B.ln;
-  view.h_offs = ho; view.h_lens = hl; view.maxlen = maxlen; view.int_no_wrap = no_wrap;
+  view.ctx = c; view.dtype = dtype_of<T>(); view.n = n; view.data = P.d; view.offs = P.o; view.lens = P.ln;
+  view.h_offs = ho; view.h_lens = hl; view.maxlen = maxlen; view.int_no_wrap = P.no_wrap;
   std::vector<T> d((size_t)n * (n - 1) / 2);
   int rc = odba_dtw_all_pairs_resident(c, &view, os, oe, d.data(), nullptr, nullptr);
   if (view.order) {  // the view owns nothing else
@@ -1800,36 +1847,65 @@ int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int 
   }
   if (rc) return rc;
   if (dists_out) std::memcpy(dists_out, d.data(), d.size() * sizeof(T));
-  const int med = medoid_single<T>(d.data(), l, n);
-  if (med < 0) return fail(c, ODBA_E_MEDOID, "no medoid found (all sums of squares at the type maximum)");
-  const int lc = hl[med];
+  P.med = medoid_single<T>(d.data(), l, n);
+  if (P.med < 0) return fail(c, ODBA_E_MEDOID, "no medoid found (all sums of squares at the type maximum)");
+  P.lc = hl[P.med];
+  return ODBA_OK;
+}
+
+// performDBA's stop rule for one cluster (dba.hpp:1084-1110), applied after
+// round r (0-based) produced nb: true when the loop ends here.
+template <typename T> struct StopRule {
+  std::vector<T> prev, prev2;
+  void init(int lc) {
+    prev.assign(lc, T(0));
+    prev2.assign(lc, T(0));
+  }
+  bool done(int r, double delta, const std::vector<T>& nb, int os, int oe) {
+    if (delta == 0) return true;
+    if (os || oe) {
+      if (r >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * nb.size())) return true;
+      if (r > 1) prev2 = prev;
+      if (r > 0) prev = nb;
+    }
+    return false;
+  }
+};
+
+// Device-resident pipeline of performDBA for one cluster: preparation, then
+// the rounds on the same context until the stop rule, then the rescale.
+template <typename T>
+int perform_dba_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, int os, int oe, int norm,
+                  int max_rounds, int policy, uint64_t free_mem, T* cen_out, uint64_t* cen_len, int* med_out, int* perm_out, int* rounds_out,
+                  T* dists_out) {
+  Prepared<T> P;
+  int rc = prepare_t<T>(c, in, in_lens, n, os, oe, norm, P, dists_out);
+  if (rc) return rc;
+  const int med = P.med, lc = P.lc;
   DbaState<T> st;
-  rc = st.init_shared(c, B.d, B.o, ho, hl, no_wrap, os, oe);
+  rc = st.init_shared(c, P.d, P.o, P.ho, P.hl, P.no_wrap, os, oe);
   if (rc) return rc;
   st.policy = policy;
   st.free_mem = auto_free_mem(c, policy, free_mem);
-  rc = st.set_centroid_device(B.d + ho[med], lc);
+  rc = st.set_centroid_device(P.d + P.ho[med], lc);
   if (rc) return rc;
-  std::vector<T> nb(lc), prev(lc, T(0)), prev2(lc, T(0));
+  std::vector<T> nb(lc);
+  StopRule<T> stop;
+  stop.init(lc);
   int rounds = 0;
   for (int r = 0; r < max_rounds; r++) {  // dba.hpp:1084-1110
     double delta = 0;
     rc = st.round(&delta, nb.data());
     if (rc) return rc;
     rounds = r + 1;
-    if (delta == 0) break;
-    if (os || oe) {
-      if (r >= 1 && !std::memcmp(nb.data(), prev2.data(), sizeof(T) * lc)) break;
-      if (r > 1) prev2 = prev;
-      if (r > 0) prev = nb;
-    }
+    if (stop.done(r, delta, nb, os, oe)) break;
   }
   if (norm)
-    for (int i = 0; i < lc; i++) nb[i] = host_d2t<T>(mu[med] + nb[i] * sg[med]);  // dba.hpp:1117-1125
+    for (int i = 0; i < lc; i++) nb[i] = host_d2t<T>(P.mu[med] + nb[i] * P.sg[med]);  // dba.hpp:1117-1125
   std::memcpy(cen_out, nb.data(), lc * sizeof(T));
   if (cen_len) *cen_len = lc;
   if (med_out) *med_out = med;
-  if (perm_out) std::memcpy(perm_out, perm.data(), n * sizeof(int));
+  if (perm_out) std::memcpy(perm_out, P.perm.data(), n * sizeof(int));
   if (rounds_out) *rounds_out = rounds;
   return ODBA_OK;
 }
@@ -2042,6 +2118,539 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
   return ODBA_OK;
 }
 
+}  // namespace
+
+// A prepared DBA job (odba_job_prepare) and its result once its rounds ended.
+struct odba_job {
+  int dtype = 0;
+  int device = 0;
+  int max_rounds = 250;
+  void* prep = nullptr;          // Prepared<T>*
+  std::vector<void*> dev;        // the rounds engine's per-job device buffers, freed with the job
+  std::vector<unsigned char> cen;
+  uint64_t clen = 0;
+  int med = -1, rounds = 0, rc = ODBA_OK;
+  bool done = false;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Packed DBA rounds across jobs (continuous batching of a server). Jobs of
+// one value type and alignment mode, each prepared on its own (sequences
+// resident, medoid found), run their rounds together: every step packs one
+// round of every active job into shared launches - path matrices,
+// tracebacks, then accumulation and mean/delta over all their centroid
+// positions - in batches of whole jobs whose step words fit the engine's
+// budget. Each job keeps its own centroid buffers, run records and stop
+// rule (dba.hpp:1084-1110), and the accumulation adds each job's values in
+// its own sequence order, so a job's result equals perform_dba_t's for the
+// same input whatever else is packed with it. Jobs join before a step and
+// leave when their stop rule ends them.
+// ---------------------------------------------------------------------------
+template <typename T> struct EJob {
+  odba_job* h = nullptr;
+  Prepared<T>* P = nullptr;
+  T* cb[2] = {nullptr, nullptr};  // centroid buffers (padded): a round reads cb[cur] and writes cb[cur ^ 1]
+  int cur = 0;
+  int4* runs = nullptr;           // run records [member][position]
+  int rounds = 0;
+  std::vector<T> nb;
+  StopRule<T> stop;
+};
+
+template <typename T> struct RoundEngine {
+  // Path jobs are launched in groups of similar swath count, each on its
+  // own stream with its traceback behind it: a strip workgroup is as wide
+  // as the widest job of its launch, so one-swath jobs packed with
+  // six-swath jobs would leave most of their workgroup's waves idle.
+  static constexpr int kGroups = 3;
+  static int group_of(int K) { return K <= 1 ? 0 : (K <= 8 ? 1 : 2); }
+
+  odba_ctx* c = nullptr;
+  odba_ctx* sub[kGroups] = {};
+  hipEvent_t ev_go = nullptr, ev_done[kGroups] = {};
+  int os = 0, oe = 0;
+  long long budget_words = 0;
+  std::vector<std::unique_ptr<EJob<T>>> act;
+  std::vector<odba_job*> finished;
+  bool dirty = true;
+  long long nstep = 0, s0 = 0;
+  struct Group {
+    int nj = 0;
+    int maxly = 1, maxlx = 0, minly = INT32_MAX;
+    double msw = 0, tail = 0;
+    bool minfirst = true;
+    size_t pj_at[2] = {0, 0}, tb_at = 0;  // byte offsets into the table arena
+  };
+  struct Batch {
+    Group g[kGroups];
+    int ns = 0;
+    long long npos = 0;
+    size_t sl_at[2] = {0, 0};
+  };
+  std::vector<Batch> batches;
+  std::vector<unsigned char> htab;  // host image of the table arena (kept until the next rebuild)
+  unsigned char* tab = nullptr;
+  size_t tab_cap = 0;
+  uint32_t* mv[kGroups] = {};
+  size_t mv_cap[kGroups] = {};
+  uint32_t* mtop[kGroups] = {};
+  size_t mtop_cap[kGroups] = {};
+  int* abr[kGroups] = {};
+  size_t abr_cap[kGroups] = {};
+  T* acc = nullptr;
+  size_t acc_cap = 0;
+  uint32_t* counts = nullptr;
+  size_t counts_cap = 0;
+  unsigned long long* dbits = nullptr;
+  size_t dbits_cap = 0;
+  int* derr = nullptr;
+  int* herr = nullptr;               // pinned
+  unsigned long long* hbits = nullptr;
+  size_t hbits_cap = 0;
+
+  int init(odba_ctx* ctx, int os_, int oe_, uint64_t word_bytes) {
+    c = ctx;
+    os = os_;
+    oe = oe_;
+    budget_words = (long long)std::max<uint64_t>(word_bytes / 4, 1);
+    for (int g = 0; g < kGroups; g++) {
+      int rc = odba_ctx_create(c->device, &sub[g]);
+      if (rc) return fail(c, rc, "cannot create a path stream: %s", sub[g] ? sub[g]->err.c_str() : "");
+      sub[g]->graphs = false;
+      HIPC(c, hipEventCreateWithFlags(&ev_done[g], hipEventDisableTiming));
+    }
+    HIPC(c, hipEventCreateWithFlags(&ev_go, hipEventDisableTiming));
+    return ODBA_OK;
+  }
+
+  ~RoundEngine() {
+    if (c) hipSetDevice(c->device);
+    for (int g = 0; g < kGroups; g++) {
+      for (void* p : {(void*)mv[g], (void*)mtop[g], (void*)abr[g]})
+        if (p) hipFree(p);
+      if (ev_done[g]) hipEventDestroy(ev_done[g]);
+      if (sub[g]) odba_ctx_destroy(sub[g]);
+    }
+    if (ev_go) hipEventDestroy(ev_go);
+    for (void* p : {(void*)tab, (void*)acc, (void*)counts, (void*)dbits, (void*)derr})
+      if (p) hipFree(p);
+    if (herr) hipHostFree(herr);
+    if (hbits) hipHostFree(hbits);
+  }
+
+  // Grow-only device buffers, with headroom: a shrinking and growing job
+  // mix would otherwise free (and so synchronise the device) every rebuild.
+  template <typename P> int grow(P*& p, size_t& cap, size_t need) {
+    if (need <= cap) return ODBA_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = need + need / 4;
+    if (hipMalloc((void**)&p, want * sizeof(P)) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %zu bytes", want * sizeof(P));
+    cap = want;
+    return ODBA_OK;
+  }
+
+  int add(odba_job* h) {
+    auto* P = (Prepared<T>*)h->prep;
+    if (P->os != os || P->oe != oe) return fail(c, ODBA_E_INVALID, "job mode differs from the engine's");
+    auto J = std::make_unique<EJob<T>>();
+    J->h = h;
+    J->P = P;
+    const long long pl = padded_len(P->lc);
+    for (int k = 0; k < 2; k++) {
+      HIPC(c, hipMalloc((void**)&J->cb[k], pl * sizeof(T)));
+      h->dev.push_back(J->cb[k]);
+      HIPC(c, hipMemsetAsync(J->cb[k], 0, pl * sizeof(T), c->stream));
+    }
+    HIPC(c, hipMalloc((void**)&J->runs, std::max<size_t>(1, (size_t)P->n * P->lc) * sizeof(int4)));
+    h->dev.push_back(J->runs);
+    HIPC(c, hipMemcpyAsync(J->cb[0], P->d + P->ho[P->med], P->lc * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    J->nb.assign(P->lc, T(0));
+    J->stop.init(P->lc);
+    act.push_back(std::move(J));
+    dirty = true;
+    return ODBA_OK;
+  }
+
+  void finish(size_t k, int rc) {
+    EJob<T>& J = *act[k];
+    odba_job* h = J.h;
+    Prepared<T>& P = *J.P;
+    if (P.norm)
+      for (int i = 0; i < P.lc; i++) J.nb[i] = host_d2t<T>(P.mu[P.med] + J.nb[i] * P.sg[P.med]);  // dba.hpp:1117-1125
+    h->cen.resize((size_t)P.lc * sizeof(T));
+    std::memcpy(h->cen.data(), J.nb.data(), h->cen.size());
+    h->clen = (uint64_t)P.lc;
+    h->med = P.med;
+    h->rounds = J.rounds;
+    h->rc = rc;
+    h->done = true;
+    finished.push_back(h);
+  }
+
+  // Geometry of member q of a job: flip, rows, columns (dba.hpp:497-501).
+  void shape(const Prepared<T>& P, int q, int& flip, int& ly, int& lx) const {
+    flip = (oe && P.lc < P.hl[q]) ? 1 : 0;
+    ly = flip ? P.lc : P.hl[q];
+    lx = flip ? P.hl[q] : P.lc;
+  }
+
+  // Batches of whole jobs whose step words fit the budget; inside a batch
+  // each group's path jobs are listed by ascending work, so the path
+  // kernels (which claim from the end of the list) start the longest first.
+  int rebuild() {
+    batches.clear();
+    htab.clear();
+    size_t mvmax[kGroups], topmax[kGroups], njmax[kGroups], posmax = 1;
+    for (int g = 0; g < kGroups; g++) mvmax[g] = topmax[g] = njmax[g] = 1;
+    struct Item { double cost; int j, q; };
+    auto align = [&]() { htab.resize((htab.size() + 255) & ~(size_t)255); return htab.size(); };
+    auto put = [&](const void* p, size_t bytes) {
+      const size_t at = align();
+      htab.insert(htab.end(), (const unsigned char*)p, (const unsigned char*)p + bytes);
+      return at;
+    };
+    size_t j0 = 0;
+    while (j0 < act.size()) {
+      long long mvw = 0;
+      size_t j1 = j0;
+      for (; j1 < act.size(); j1++) {
+        const Prepared<T>& P = *act[j1]->P;
+        long long need = 0;
+        for (int q = 0; q < P.n; q++) {
+          int flip, ly, lx;
+          shape(P, q, flip, ly, lx);
+          need += (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        }
+        if (j1 > j0 && mvw + need > budget_words) break;
+        mvw += need;
+      }
+      Batch b;
+      std::vector<Item> items[kGroups];
+      for (size_t j = j0; j < j1; j++) {
+        const Prepared<T>& P = *act[j]->P;
+        for (int q = 0; q < P.n; q++) {
+          int flip, ly, lx;
+          shape(P, q, flip, ly, lx);
+          const int K = (lx + ODBA_W - 1) / ODBA_W;
+          items[group_of(K)].push_back(Item{(double)ly * (double)K, (int)j, q});
+        }
+      }
+      for (int g = 0; g < kGroups; g++) {
+        Group& G = b.g[g];
+        std::stable_sort(items[g].begin(), items[g].end(), [](const Item& a, const Item& z) { return a.cost < z.cost; });
+        std::vector<PJ<T>> pj[2];
+        std::vector<TbJob> tb;
+        long long gw = 0, tw = 0;
+        for (const Item& it : items[g]) {
+          EJob<T>& J = *act[it.j];
+          const Prepared<T>& P = *J.P;
+          int flip, ly, lx;
+          shape(P, it.q, flip, ly, lx);
+          long long mo, to;
+          plan_job(ly, lx, os, oe, gw, tw, mo, to);
+          const T* sq = P.d + P.ho[it.q];
+          for (int v = 0; v < 2; v++) {
+            const T* cp = J.cb[J.cur ^ v];
+            pj[v].push_back(PJ<T>{flip ? cp : sq, flip ? sq : cp, ly, lx, mo, to});
+          }
+          TbJob t;
+          t.ly = ly; t.lx = lx; t.flip = flip; t.seq = it.q; t.mv_off = mo; t.top_off = to; t.path_off = -1;
+          t.runs_off = 0;
+          t.rp = J.runs + (size_t)it.q * P.lc;
+          tb.push_back(t);
+          G.maxly = std::max(G.maxly, ly);
+          G.maxlx = std::max(G.maxlx, lx);
+          G.minly = std::min(G.minly, ly);
+          G.minfirst = G.minfirst && P.no_wrap;
+        }
+        G.nj = (int)tb.size();
+        if (!G.nj) continue;
+        G.msw = mean_swaths(pj[0]);
+        G.tail = path_tail(pj[0]);
+        for (int v = 0; v < 2; v++) G.pj_at[v] = put(pj[v].data(), pj[v].size() * sizeof(PJ<T>));
+        G.tb_at = put(tb.data(), tb.size() * sizeof(TbJob));
+        mvmax[g] = std::max(mvmax[g], (size_t)gw);
+        topmax[g] = std::max(topmax[g], (size_t)tw);
+        njmax[g] = std::max(njmax[g], (size_t)G.nj);
+      }
+      std::vector<MultiSlot> sl[2];
+      long long pos = 0;
+      for (size_t j = j0; j < j1; j++) {
+        EJob<T>& J = *act[j];
+        for (int v = 0; v < 2; v++) {
+          MultiSlot m;
+          m.pos0 = pos; m.data = J.P->d; m.moffs = J.P->o; m.runs = J.runs;
+          m.cen = J.cb[J.cur ^ v]; m.newc = J.cb[J.cur ^ v ^ 1];
+          m.lc = J.P->lc; m.nmem = J.P->n; m.slot = (int)j; m.pad = 0;
+          sl[v].push_back(m);
+        }
+        pos += J.P->lc;
+      }
+      b.ns = (int)(j1 - j0);
+      b.npos = pos;
+      for (int v = 0; v < 2; v++) b.sl_at[v] = put(sl[v].data(), sl[v].size() * sizeof(MultiSlot));
+      posmax = std::max(posmax, (size_t)pos);
+      batches.push_back(b);
+      j0 = j1;
+    }
+    int rc;
+    if ((rc = grow(tab, tab_cap, std::max<size_t>(htab.size(), 256))) || (rc = grow(acc, acc_cap, posmax)) ||
+        (rc = grow(counts, counts_cap, posmax)) || (rc = grow(dbits, dbits_cap, act.size())))
+      return rc;
+    for (int g = 0; g < kGroups; g++)
+      if ((rc = grow(mv[g], mv_cap[g], mvmax[g])) || (rc = grow(mtop[g], mtop_cap[g], topmax[g])) ||
+          (rc = grow(abr[g], abr_cap[g], njmax[g])))
+        return rc;
+    if (!derr) HIPC(c, hipMalloc((void**)&derr, sizeof(int)));
+    if (!herr) HIPC(c, hipHostMalloc((void**)&herr, sizeof(int)));
+    if (hbits_cap < act.size()) {
+      if (hbits) hipHostFree(hbits);
+      hbits = nullptr;
+      hbits_cap = 0;
+      HIPC(c, hipHostMalloc((void**)&hbits, act.size() * 2 * sizeof(unsigned long long)));
+      hbits_cap = act.size() * 2;
+    }
+    HIPC(c, hipMemcpyAsync(tab, htab.data(), htab.size(), hipMemcpyHostToDevice, c->stream));
+    return ODBA_OK;
+  }
+
+  // One round of every active job. Jobs whose stop rule ends them move to
+  // `finished` with their results; an error fails every active job.
+  int step() {
+    if (act.empty()) return ODBA_OK;
+    int rc = ODBA_OK;
+    if (dirty) {
+      rc = rebuild();
+      if (rc) return fail_all(rc);
+      s0 = nstep;
+      dirty = false;
+    }
+    const int v = (int)((nstep - s0) & 1);
+    rc = launch_round(v);
+    if (rc) return fail_all(rc);
+    const int herr_v = *herr;
+    if (herr_v) {
+      if (herr_v & 8) rc = fail(c, ODBA_E_HIP, "path band edge hand-off timed out");
+      else if (herr_v & 1) rc = fail(c, ODBA_E_TRACEBACK, "traceback left the matrix before reaching (0,0)");
+      else if (herr_v & 4) rc = fail(c, ODBA_E_TRACEBACK, "traceback produced a third counted run for one centroid position");
+      else rc = fail(c, ODBA_E_DIV_ZERO, "integer centroid position with zero count");
+      return fail_all(rc);
+    }
+    std::vector<std::unique_ptr<EJob<T>>> keep;
+    for (size_t k = 0; k < act.size(); k++) {
+      EJob<T>& J = *act[k];
+      J.cur ^= 1;
+      const int r = J.rounds++;
+      double d;
+      std::memcpy(&d, &hbits[k], sizeof(d));
+      if (J.stop.done(r, d, J.nb, os, oe) || J.rounds >= J.h->max_rounds) {
+        finish(k, ODBA_OK);
+        dirty = true;
+      } else {
+        keep.push_back(std::move(act[k]));
+      }
+    }
+    act.swap(keep);
+    nstep++;
+    return ODBA_OK;
+  }
+
+  int launch_round(int v) {
+    HIPC(c, launch_fill_u32(c->stream, derr, 0u, 1));
+    HIPC(c, launch_fill_u32(c->stream, dbits, 0u, 2 * (long long)act.size()));
+    for (const Batch& b : batches) {
+      HIPC(c, hipEventRecord(ev_go, c->stream));
+      for (int g = 0; g < kGroups; g++) {
+        const Group& G = b.g[g];
+        if (!G.nj) continue;
+        odba_ctx* sc = sub[g];
+        HIPC(c, hipStreamWaitEvent(sc->stream, ev_go, 0));
+        int rc = run_path_jobs(sc, dtype_of<T>(), G.minfirst, os, oe, tab + G.pj_at[v], G.nj, G.maxly, mv[g], mtop[g],
+                               abr[g], G.maxlx, G.minly, derr, G.msw, G.tail);
+        if (rc) return fail(c, rc, "%s", sc->err.c_str());
+        HIPC(c, launch_traceback(sc->stream, (const TbJob*)(tab + G.tb_at), G.nj, mv[g], mtop[g], abr[g], nullptr, 0, os,
+                                 oe, derr, nullptr));
+        HIPC(c, hipEventRecord(ev_done[g], sc->stream));
+        HIPC(c, hipStreamWaitEvent(c->stream, ev_done[g], 0));
+      }
+      HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, (const MultiSlot*)(tab + b.sl_at[v]), b.ns, b.npos, acc,
+                                counts, dbits, derr));
+    }
+    HIPC(c, hipMemcpyAsync(herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(hbits, dbits, act.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    for (auto& J : act)
+      HIPC(c, hipMemcpyAsync(J->nb.data(), J->cb[J->cur ^ 1], J->P->lc * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return ODBA_OK;
+  }
+
+  int fail_all(int rc) {
+    for (size_t k = 0; k < act.size(); k++) finish(k, rc);
+    act.clear();
+    dirty = true;
+    return rc;
+  }
+};
+
+}  // namespace
+
+struct odba_rounds {
+  odba_ctx* ctx = nullptr;
+  int dtype = 0;
+  void* eng = nullptr;  // RoundEngine<T>*
+};
+
+namespace {
+template <typename T> int job_prepare_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int n, int os, int oe,
+                                        int norm, odba_job* h) {
+  auto* P = new Prepared<T>();
+  h->prep = P;
+  return prepare_t<T>(c, seqs, lens, n, os, oe, norm, *P, nullptr);
+}
+template <typename T> void job_free_t(odba_job* h) { delete (Prepared<T>*)h->prep; }
+template <typename T> int rounds_create_t(odba_ctx* c, int os, int oe, uint64_t word_bytes, void** out) {
+  auto* e = new RoundEngine<T>();
+  const int rc = e->init(c, os, oe, word_bytes);
+  if (rc) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return ODBA_OK;
+}
+template <typename T> int rounds_add_t(void* e, odba_job* h) { return ((RoundEngine<T>*)e)->add(h); }
+template <typename T> int rounds_step_t(void* e, odba_job** fin, int cap, int* nfin, int* nact) {
+  auto* E = (RoundEngine<T>*)e;
+  const int rc = E->step();
+  int k = 0;
+  while (k < cap && !E->finished.empty()) {
+    fin[k++] = E->finished.front();
+    E->finished.erase(E->finished.begin());
+  }
+  if (nfin) *nfin = k;
+  if (nact) *nact = (int)E->act.size() + (int)E->finished.size();
+  return rc;
+}
+template <typename T> void rounds_free_t(void* e) { delete (RoundEngine<T>*)e; }
+}  // namespace
+
+extern "C" {
+
+int odba_job_prepare(odba_ctx* c, int dtype, const void* const* seqs, const uint64_t* lens, int n, int use_open_start,
+                     int use_open_end, int norm, int max_rounds, odba_job** out) {
+  if (!c || !seqs || !lens || !out || max_rounds < 1) return fail(c, ODBA_E_INVALID, "invalid job arguments");
+  if (dtype_size(dtype) == 0) return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  auto* h = new odba_job();
+  h->dtype = dtype;
+  h->device = c->device;
+  h->max_rounds = max_rounds;
+  int rc;
+  switch (dtype) {
+    case ODBA_I32: rc = job_prepare_t<int32_t>(c, (const int32_t* const*)seqs, lens, n, use_open_start, use_open_end, norm, h); break;
+    case ODBA_U32: rc = job_prepare_t<uint32_t>(c, (const uint32_t* const*)seqs, lens, n, use_open_start, use_open_end, norm, h); break;
+    case ODBA_U64: rc = job_prepare_t<uint64_t>(c, (const uint64_t* const*)seqs, lens, n, use_open_start, use_open_end, norm, h); break;
+    case ODBA_F32: rc = job_prepare_t<float>(c, (const float* const*)seqs, lens, n, use_open_start, use_open_end, norm, h); break;
+    default: rc = job_prepare_t<double>(c, (const double* const*)seqs, lens, n, use_open_start, use_open_end, norm, h); break;
+  }
+  if (rc) {
+    odba_job_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return ODBA_OK;
+}
+
+int odba_job_result(const odba_job* h, void* centroid_out, uint64_t capacity, uint64_t* centroid_len, int* medoid,
+                    int* rounds, int* job_rc) {
+  if (!h) return ODBA_E_INVALID;
+  if (!h->done) return ODBA_E_BUSY;
+  if (centroid_out) {
+    if (capacity < h->clen) return ODBA_E_INVALID;
+    std::memcpy(centroid_out, h->cen.data(), h->cen.size());
+  }
+  if (centroid_len) *centroid_len = h->clen;
+  if (medoid) *medoid = h->med;
+  if (rounds) *rounds = h->rounds;
+  if (job_rc) *job_rc = h->rc;
+  return ODBA_OK;
+}
+
+int odba_job_destroy(odba_job* h) {
+  if (!h) return ODBA_OK;
+  hipSetDevice(h->device);
+  for (void* p : h->dev)
+    if (p) hipFree(p);
+  switch (h->dtype) {
+    case ODBA_I32: job_free_t<int32_t>(h); break;
+    case ODBA_U32: job_free_t<uint32_t>(h); break;
+    case ODBA_U64: job_free_t<uint64_t>(h); break;
+    case ODBA_F32: job_free_t<float>(h); break;
+    case ODBA_F64: job_free_t<double>(h); break;
+  }
+  delete h;
+  return ODBA_OK;
+}
+
+int odba_rounds_create(odba_ctx* c, int dtype, int use_open_start, int use_open_end, uint64_t word_bytes,
+                       odba_rounds** out) {
+  if (!c || !out) return fail(c, ODBA_E_INVALID, "invalid rounds-engine arguments");
+  if (dtype_size(dtype) == 0) return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype);
+  auto* r = new odba_rounds();
+  r->ctx = c;
+  r->dtype = dtype;
+  int rc;
+  switch (dtype) {
+    case ODBA_I32: rc = rounds_create_t<int32_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+    case ODBA_U32: rc = rounds_create_t<uint32_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+    case ODBA_U64: rc = rounds_create_t<uint64_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+    case ODBA_F32: rc = rounds_create_t<float>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+    default: rc = rounds_create_t<double>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+  }
+  if (rc) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return ODBA_OK;
+}
+
+int odba_rounds_add(odba_rounds* r, odba_job* job) {
+  if (!r || !job || job->dtype != r->dtype || job->done) return ODBA_E_INVALID;
+  odba_ctx* c = r->ctx;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(r->dtype, rounds_add_t<T>(r->eng, job))
+}
+
+int odba_rounds_step(odba_rounds* r, odba_job** finished, int capacity, int* n_finished, int* n_active) {
+  if (!r || (capacity > 0 && !finished)) return ODBA_E_INVALID;
+  odba_ctx* c = r->ctx;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, ODBA_E_HIP, "hipSetDevice");
+  ODBA_DISPATCH(r->dtype, rounds_step_t<T>(r->eng, finished, capacity, n_finished, n_active))
+}
+
+int odba_rounds_destroy(odba_rounds* r) {
+  if (!r) return ODBA_OK;
+  hipSetDevice(r->ctx->device);
+  switch (r->dtype) {
+    case ODBA_I32: rounds_free_t<int32_t>(r->eng); break;
+    case ODBA_U32: rounds_free_t<uint32_t>(r->eng); break;
+    case ODBA_U64: rounds_free_t<uint64_t>(r->eng); break;
+    case ODBA_F32: rounds_free_t<float>(r->eng); break;
+    case ODBA_F64: rounds_free_t<double>(r->eng); break;
+  }
+  delete r;
+  return ODBA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
 // chopPrefixFromSequences (dba.hpp:1192-1395): every sequence is aligned
 // against the prefix (prefix on Y, open end, W = 1024) as one batch of path
 // jobs, and the DBA traceback with the prefix as the "centroid" on Y gives,
@@ -2163,16 +2772,6 @@ int prefix_chop_t(odba_ctx* c, const T* prefix_in, int P, T* const* seqs, const 
 }
 
 }  // namespace
-
-#define ODBA_DISPATCH(dtype, CALL)                        \
-  switch (dtype) {                                        \
-    case ODBA_I32: { typedef int32_t T; return CALL; }    \
-    case ODBA_U32: { typedef uint32_t T; return CALL; }   \
-    case ODBA_U64: { typedef uint64_t T; return CALL; }   \
-    case ODBA_F32: { typedef float T; return CALL; }      \
-    case ODBA_F64: { typedef double T; return CALL; }     \
-    default: return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype); \
-  }
 
 extern "C" {
 

@@ -40,6 +40,7 @@ __device__ __forceinline__ Geo make_geo(int ly, int lx, int os, int oe, int ab) 
   return g;
 }
 
+
 // Reference step code of cell (i, j) (dtw.hpp:10-16). Cells the reference
 // never writes (below the top row of an abrogated swath) come back as 0.
 __device__ __forceinline__ uint8_t step_code(const Geo& g, const uint32_t* __restrict__ mv,
@@ -61,7 +62,7 @@ __device__ __forceinline__ uint8_t step_code(const Geo& g, const uint32_t* __res
   uint32_t w;
   if (addr == caddr) w = cword;
   else { w = mv[addr]; caddr = addr; cword = w; }
-  uint32_t c = (w >> (2 * q)) & 3u;
+  const uint32_t c = plane_code(w, q);
   return c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
 }
 
@@ -151,7 +152,7 @@ struct ColCursor {
 };
 
 // Even bits of w packed down (bit 2q -> bit q): the low or high bits of 16
-// two-bit step codes as a 16-bit mask.
+// two-bit step codes (open-end top-row words) as a 16-bit mask.
 __device__ __forceinline__ uint32_t even_bits(uint32_t x) {
   x &= 0x55555555u;
   x = (x | (x >> 1)) & 0x33333333u;
@@ -199,9 +200,11 @@ struct MoveTile {
     uint64_t r = 0, u = 0;
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-      const uint32_t lo = even_bits(w[d]) & cm, hi = even_bits(w[d] >> 1) & cm;
-      r |= (uint64_t)lo << (d * C);        // codes 1 and 3 step RIGHT
-      u |= (uint64_t)(hi & ~lo) << (d * C);  // code 2 steps UP
+      // reversed, column q's bits land at 16 + q (not DIAG) and q (UP on a tie)
+      const uint32_t rv = __builtin_bitreverse32(w[d]);
+      const uint32_t nd = (rv >> 16) & cm, up = rv & cm;
+      r |= (uint64_t)(nd & ~up) << (d * C);  // steps RIGHT
+      u |= (uint64_t)(nd & up) << (d * C);   // steps UP
     }
     rm = r; um = u;
     i0 = i; cb = j - 3 * C - q;
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
       tile.i0 = wave_uniform(tile.i0); tile.l0 = wave_uniform(tile.l0); tile.k0 = wave_uniform(tile.k0);
       rr.lo1 = wave_uniform(rr.lo1); rr.hi1 = wave_uniform(rr.hi1); rr.lo2 = wave_uniform(rr.lo2);
       rr.hi2 = wave_uniform(rr.hi2); rr.gap = wave_uniform(rr.gap);
-      const uint32_t c2 = (tile.get(M, g.S64, g.r0, cc.k, i, cc.l, lane) >> (2 * cc.q)) & 3u;  // 0 DIAG 1 RIGHT 2 UP
+      const uint32_t c2 = plane_code(tile.get(M, g.S64, g.r0, cc.k, i, cc.l, lane), cc.q);  // 0 DIAG 1 RIGHT 2 UP
       if (REC) {
         if (P && lane == 0) P[1 + np] = pack_step(i, j, c2 == 0 ? ODBA_DIAGONAL : (c2 == 2 ? ODBA_UP : ODBA_RIGHT));
         np++;
@@ -545,7 +548,7 @@ __global__ void __launch_bounds__(256) k_ckpt_trace(const CkJobD* __restrict__ j
       curc = j;
       rr.reset();
     }
-    const uint32_t c2 = (tile.get(M, S64, 0, cc.k - k0, i - bR, cc.l, lane) >> (2 * cc.q)) & 3u;  // 0 DIAG 1 RIGHT 2 UP
+    const uint32_t c2 = plane_code(tile.get(M, S64, 0, cc.k - k0, i - bR, cc.l, lane), cc.q);  // 0 DIAG 1 RIGHT 2 UP
     if (REC) {
       if (P && lane == 0) P[1 + np] = pack_step(i, j, c2 == 0 ? ODBA_DIAGONAL : (c2 == 2 ? ODBA_UP : ODBA_RIGHT));
       np++;
@@ -640,7 +643,7 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
         const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
       } else {
-        const uint32_t c = (tile.get(M, 0, r0, 0, i, l, lane) >> (2 * q)) & 3u;
+        const uint32_t c = plane_code(tile.get(M, 0, r0, 0, i, l, lane), q);
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
       }
     }
@@ -959,22 +962,43 @@ template <typename T> struct IsFloat { static constexpr bool v = false; };
 template <> struct IsFloat<float> { static constexpr bool v = true; };
 template <> struct IsFloat<double> { static constexpr bool v = true; };
 
+// The max delta of a round as the bits of a non-negative double (their
+// integer order is the numeric order). NaN deltas are skipped, as the
+// reference's comparison skips them. With `uniform` (every lane of the
+// wave targets the same word) the wave's maximum is formed first and lane 0
+// issues the atomic; otherwise each lane does (positions of several
+// clusters in one wave).
+// d < 0: this lane has no delta. Every lane of the wave must call it.
+__device__ __forceinline__ void delta_max(unsigned long long* word, double d, bool uniform) {
+  const bool has = d == d && d >= 0.0;
+  if (uniform) {
+    double w = has ? d : -1.0;
+    for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o));
+    if (w >= 0.0 && (threadIdx.x & 63) == 0) atomicMax(word, (unsigned long long)__double_as_longlong(w));
+  } else if (has) {
+    atomicMax(word, (unsigned long long)__double_as_longlong(d));
+  }
+}
+
 template <typename T>
 __global__ void k_mean_delta(const T* __restrict__ acc, const uint32_t* __restrict__ counts,
                              const T* __restrict__ oldc, T* __restrict__ newc, int lc,
                              unsigned long long* __restrict__ delta_bits, int* __restrict__ err) {
+  // No early exit: the delta's wave reduction needs every lane.
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= lc) return;
-  T m;
-  if (!IsFloat<T>::v && counts[c] == 0) {
-    atomicOr(err, 2);
-    m = T(0);
-  } else {
-    m = div_count<T>(acc[c], counts[c]);
+  double d = -1.0;  // no delta
+  if (c < lc) {
+    T m;
+    if (!IsFloat<T>::v && counts[c] == 0) {
+      atomicOr(err, 2);
+      m = T(0);
+    } else {
+      m = div_count<T>(acc[c], counts[c]);
+    }
+    newc[c] = m;
+    d = fabs((double)(T)(oldc[c] - m));  // dba.hpp:830-836
   }
-  newc[c] = m;
-  const double d = fabs((double)(T)(oldc[c] - m));  // dba.hpp:830-836
-  if (d == d) atomicMax(delta_bits, (unsigned long long)__double_as_longlong(d));
+  delta_max(delta_bits, d, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,20 +1046,25 @@ template <typename T>
 __global__ void k_mean_delta_multi(const T* __restrict__ acc, const uint32_t* __restrict__ counts,
                                    const MultiSlot* __restrict__ sl, int ns, long long total,
                                    unsigned long long* __restrict__ delta_bits, int* __restrict__ err) {
+  // No early exit: lanes past total take the last position's cluster and
+  // no delta, so the delta's wave reduction sees every lane.
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const MultiSlot S = sl[slot_of<T>(sl, ns, g)];
-  const int c = (int)(g - S.pos0);
-  T m;
-  if (!IsFloat<T>::v && counts[g] == 0) {
-    atomicOr(err, 2);
-    m = T(0);
-  } else {
-    m = div_count<T>(acc[g], counts[g]);
+  const bool on = g < total;
+  const MultiSlot S = sl[slot_of<T>(sl, ns, on ? g : total - 1)];
+  double d = -1.0;  // no delta
+  if (on) {
+    const int c = (int)(g - S.pos0);
+    T m;
+    if (!IsFloat<T>::v && counts[g] == 0) {
+      atomicOr(err, 2);
+      m = T(0);
+    } else {
+      m = div_count<T>(acc[g], counts[g]);
+    }
+    ((T*)S.newc)[c] = m;
+    d = fabs((double)(T)(((const T*)S.cen)[c] - m));  // dba.hpp:830-836
   }
-  ((T*)S.newc)[c] = m;
-  const double d = fabs((double)(T)(((const T*)S.cen)[c] - m));  // dba.hpp:830-836
-  if (d == d) atomicMax(&delta_bits[S.slot], (unsigned long long)__double_as_longlong(d));
+  delta_max(&delta_bits[S.slot], d, wave_min_i(S.slot) == wave_max_i(S.slot));
 }
 
 // ---------------------------------------------------------------------------

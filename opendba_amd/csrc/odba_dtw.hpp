@@ -14,6 +14,25 @@ namespace odba {
 // 1024-wide column block and the row window 128 entries past the end.
 inline long long padded_len(long long L) { return ((L + 63) / 64) * 64 + 1024; }
 
+// Waves of a path-strip or swath-band workgroup at most. A path cell holds
+// a step word and more temporaries than the distance-only sweep: about 165
+// VGPRs at 16 columns per lane. Workgroups of at most 12 waves (three per
+// SIMD) keep all of it in registers; 16-wave workgroups capped the kernels
+// at 128 VGPRs and spilled 46-74 of them to scratch inside the sweep.
+constexpr int kPathStripWaves = 12;
+
+// Path step words. A lane's word for one step holds its C <= 16 cells in
+// two bit planes: bit 15 - q of the low half is set when column q did not
+// take the diagonal, bit 31 - q of the high half when its up candidate
+// holds the minimum. The reference picks DIAG, then UP, then RIGHT
+// (dtw.hpp:202-227), so the code is DIAG when the low bit is clear, else UP
+// when the high bit is set, else RIGHT. Returns 0 DIAG, 1 RIGHT, 2 UP.
+// (The open-end top row's own words keep 2-bit codes at bits 2q, 2q + 1.)
+__host__ __device__ inline uint32_t plane_code(uint32_t w, int q) {
+  const uint32_t a = (w >> (15 - q)) & 1u, b = (w >> (31 - q)) & 1u;
+  return a ? (b ? 2u : 1u) : 0u;
+}
+
 inline int last_swath_c_host(int wcols) {
   if (wcols <= 64) return 1;
   if (wcols <= 128) return 2;

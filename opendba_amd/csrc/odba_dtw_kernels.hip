@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
+#include <type_traits>
 #include <float.h>
 #include <limits.h>
 
@@ -210,6 +211,37 @@ __device__ __forceinline__ T cell_wrap(T up, T left, T diag, T dd, CellFlags f, 
   return v;
 }
 
+
+// Path cell with its step bits in the two planes of plane_code
+// (odba_dtw.hpp): each plane is built by shifting the lane's word left and
+// adding the compare's lane bit as the carry, one v_addc per plane and cell
+// (packing 2-bit codes took two selects and a shift-or per cell).
+template <typename T, bool MINFIRST>
+__device__ __forceinline__ T cell_planes(T up, T left, T diag, T dd, uint32_t& wa, uint32_t& wb) {
+  const T U = up + dd, R = left + dd, G = diag + dd;
+  const T v = Ar<T>::mn3(U, R, G);
+  if constexpr (std::is_same<T, float>::value) {
+    const unsigned long long na = __builtin_amdgcn_fcmpf(G, v, 14);  // unordered or not equal: not DIAG
+    const unsigned long long nb = __builtin_amdgcn_fcmpf(U, v, 1);   // ordered and equal: UP on a tie
+    unsigned long long co;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(wa), "=s"(co) : "v"(wa), "s"(na));
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(wb), "=s"(co) : "v"(wb), "s"(nb));
+  } else {
+    wa = (wa << 1) | ((G != v) ? 1u : 0u);
+    wb = (wb << 1) | ((U == v) ? 1u : 0u);
+  }
+  return v;
+}
+
+// The lane's step word from its planes after C cells (column q at bit C-1-q
+// of each plane): columns moved to bits 15 - q and 31 - q, or, for a quarter
+// strip's lane k of a group of four (columns kC .. kC+C-1 of the word), to
+// bits 15 - kC - q and 31 - kC - q.
+template <int C> __device__ __forceinline__ uint32_t plane_word(uint32_t wa, uint32_t wb, int k = 0) {
+  const int sh = 16 - (k + 1) * C;
+  return (wb << (16 + sh)) | (wa << sh);
+}
+
 // ---------------------------------------------------------------------------
 // Per-wave LDS staging window.
 // ---------------------------------------------------------------------------
@@ -326,10 +358,13 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
 
   T runmin = MAXV;
   uint32_t h1 = 0, h2 = 0, h3 = 0;  // QW: this lane's fragments 1, 2, 3 steps back
+  // QW: `word` is this lane's fragment, already at its bits of the whole
+  // word (plane_word with k = lane & 3), so the group's last lane ORs the
+  // three earlier fragments of the same row into its own.
   auto put_word = [&](int s, uint32_t word, bool act) {
     if constexpr (QW) {
       const uint32_t f1 = row_shr_u32<1>(h1), f2 = row_shr_u32<2>(h2), f3 = row_shr_u32<3>(h3);
-      const uint32_t full = f3 | (f2 << (2 * C)) | (f1 << (4 * C)) | (word << (6 * C));
+      const uint32_t full = f3 | f2 | f1 | word;
       h3 = h2;
       h2 = h1;
       h1 = word;
@@ -399,7 +434,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
         T dg = diag_in;
         diag_in = left_in;
         T l = left_in;
-        uint32_t word = 0;
+        uint32_t wa = 0, wb = 0;
 #pragma unroll
         for (int q = 0; q < C; q++) {
           T up = D[q];
@@ -408,19 +443,23 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           uint32_t code = 0;
           T v;
           if (MINFIRST || Ar<T>::kFloat) {
-            v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
+            if (PATH) v = cell_planes<T, MINFIRST>(up, l, dg, dd, wa, wb);
+            else v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
           } else {
             const int row = r0 + s - lane;
             CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
             T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
             v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
+            if (PATH) {
+              wa = (wa << 1) | (code != 0u ? 1u : 0u);
+              wb = (wb << 1) | (code == 2u ? 1u : 0u);
+            }
           }
-          if (PATH) word |= code << (2 * q);
           dg = up;
           D[q] = v;
           l = v;
         }
-        if (PATH) put_word(s, word, true);
+        if (PATH) put_word(s, plane_word<C>(wa, wb, QW ? (lane & 3) : 0), true);
         if (CKPT) ckpt(r0 + s - lane, true);
         W->o[u - lane + 64] = D[C - 1];
       }
@@ -438,7 +477,7 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
         T dg = diag_in;
         diag_in = active ? left_in : diag_in;
         T l = left_in;
-        uint32_t word = 0;
+        uint32_t wa = 0, wb = 0;
 #pragma unroll
         for (int q = 0; q < C; q++) {
           T up = D[q];
@@ -447,18 +486,22 @@ __device__ SwathRes<T> run_swath(const SwathIO<T>& a, const WaveWin<T>* W, int l
           uint32_t code = 0;
           T v;
           if (MINFIRST || Ar<T>::kFloat) {
-            v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
+            if (PATH) v = cell_planes<T, MINFIRST>(up, l, dg, dd, wa, wb);
+            else v = cell<T, MINFIRST, PATH>(up, l, dg, dd, code);
           } else {
             CellFlags f{row > 0, (q > 0) || (lane > 0) || !first_swath || row == 0, (row > 0) && ((q > 0) || (lane > 0) || !first_swath)};
             T lv = (q == 0 && lane == 0 && first_swath) ? T(0) : l;
             v = cell_wrap<T, PATH>(up, lv, dg, dd, f, code);
+            if (PATH) {
+              wa = (wa << 1) | (code != 0u ? 1u : 0u);
+              wb = (wb << 1) | (code == 2u ? 1u : 0u);
+            }
           }
-          if (PATH) word |= code << (2 * q);
           dg = up;
           D[q] = active ? v : up;
           l = v;
         }
-        if (PATH) put_word(s, word, active);
+        if (PATH) put_word(s, plane_word<C>(wa, wb, QW ? (lane & 3) : 0), active);
         if (CKPT) ckpt(row, active);
         W->o[u - lane + 64] = D[C - 1];
       }
@@ -1393,16 +1436,16 @@ __global__ void __launch_bounds__(256) k_path_jobs(const PathJob<T>* __restrict_
 // swaths (a long sequence against a short centroid) still spreads over four
 // times as many waves; the step words land in the same layout.
 template <typename T, bool OS, bool OE, bool QW>
-__global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(64 * kPathStripWaves) k_path_strips(const PathJob<T>* __restrict__ jobs, int njobs,
                                                       unsigned int* __restrict__ counter, uint32_t* __restrict__ mv,
                                                       uint32_t* __restrict__ mtop, int* __restrict__ abrogated,
                                                       T* __restrict__ escratch, long long stride) {
-  __shared__ T s_y[kMaxStripWaves][128];
-  __shared__ T s_e[kMaxStripWaves][64];
-  __shared__ T s_o[kMaxStripWaves][128];
-  __shared__ int s_owner[2 * kMaxStripWaves];
-  __shared__ int s_cnt[2 * kMaxStripWaves];
-  __shared__ int s_free[2 * kMaxStripWaves];
+  __shared__ T s_y[kPathStripWaves][128];
+  __shared__ T s_e[kPathStripWaves][64];
+  __shared__ T s_o[kPathStripWaves][128];
+  __shared__ int s_owner[2 * kPathStripWaves];
+  __shared__ int s_cnt[2 * kPathStripWaves];
+  __shared__ int s_free[2 * kPathStripWaves];
   __shared__ int s_job;
   __shared__ int s_abr;
   const int lane = threadIdx.x & 63;
@@ -1417,7 +1460,7 @@ __global__ void __launch_bounds__(1024) k_path_strips(const PathJob<T>* __restri
       s_job = (int)atomicAdd(counter, 1u);
       s_abr = INT_MAX;
     }
-    if (threadIdx.x < 2 * kMaxStripWaves) {
+    if (threadIdx.x < 2 * kPathStripWaves) {
       s_owner[threadIdx.x] = -1;
       s_cnt[threadIdx.x] = 0;
       s_free[threadIdx.x] = 1;
@@ -1667,7 +1710,7 @@ __global__ void __launch_bounds__(256) k_path_bands(const PathJob<float>* __rest
 // band0[p] = first claim of the p-th job in claim order; band0[njobs] = all.
 // ---------------------------------------------------------------------------
 template <bool OS, bool OE>
-__global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>* __restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const PathJob<float>* __restrict__ jobs, int njobs,
                                                            const long long* __restrict__ band0,
                                                            unsigned long long* __restrict__ counter,
                                                            uint32_t* __restrict__ mv, uint32_t* __restrict__ mtop,
@@ -1677,11 +1720,11 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
                                                            int R, int* __restrict__ slot_of, int* __restrict__ abr,
                                                            int* __restrict__ err, long long limit) {
   using T = float;
-  __shared__ T s_y[kMaxStripWaves][128];
-  __shared__ T s_e[kMaxStripWaves][64];
-  __shared__ T s_o[kMaxStripWaves][128];
-  __shared__ int s_cnt[kMaxStripWaves];   // rows published into ring w
-  __shared__ int s_used[kMaxStripWaves];  // rows of ring w read by wave w+1
+  __shared__ T s_y[kPathStripWaves][128];
+  __shared__ T s_e[kPathStripWaves][64];
+  __shared__ T s_o[kPathStripWaves][128];
+  __shared__ int s_cnt[kPathStripWaves];   // rows published into ring w
+  __shared__ int s_used[kPathStripWaves];  // rows of ring w read by wave w+1
   __shared__ unsigned long long s_q;
   const int lane = threadIdx.x & 63;
   const int wid = uniform_i32((int)(threadIdx.x >> 6));
@@ -1692,7 +1735,7 @@ __global__ void __launch_bounds__(1024) k_path_swath_bands(const PathJob<float>*
   for (;;) {
     __syncthreads();  // every wave is done with the previous claim
     if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
-    if (threadIdx.x < kMaxStripWaves) {
+    if (threadIdx.x < kPathStripWaves) {
       s_cnt[threadIdx.x] = 0;
       s_used[threadIdx.x] = 0;
     }
@@ -2277,7 +2320,7 @@ hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStrea
                                    long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
                                    int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
                                    int* err, long long tick_limit) {
-  if (njobs < 1 || njobs > 1024 || nw < 1 || nw > kMaxStripWaves || ring < 128 || (ring & (ring - 1)) ||
+  if (njobs < 1 || njobs > 1024 || nw < 1 || nw > kPathStripWaves || ring < 128 || (ring & (ring - 1)) ||
       R < 2 * nblocks + 2)
     return hipErrorInvalidValue;
   // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1] (8-aligned)
@@ -2334,7 +2377,7 @@ size_t ckpt_state_size() { return sizeof(CkState); }
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                               long long stride, int quarters) {
-  if (waves < 1 || waves > kMaxStripWaves) return hipErrorInvalidValue;
+  if (waves < 1 || waves > kPathStripWaves) return hipErrorInvalidValue;
   dim3 g(nblocks), b(64 * waves);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
   float* E = (float*)esc;

@@ -1,21 +1,31 @@
 // Serving loop: DBA jobs (z-norm -> all-vs-all -> medoid -> DBA rounds to
 // convergence, the performDBA pipeline for one cluster) submitted at any
-// time, admitted in arrival order as soon as a slot and enough device
-// memory are free, and run concurrently.
+// time, admitted in arrival order as soon as a preparation slot and enough
+// device memory are free, and run concurrently.
 //
-// Each slot is a worker thread with its own odba_ctx, hence its own HIP
-// stream: the kernels of admitted jobs overlap on the device. A job that
-// cannot fill the GPU (a few hundred pairs, or the one-wave-per-sequence
-// DBA kernels) leaves compute units idle that the other jobs' persistent
-// launches take up. Admission compares an estimate of the job's device
-// working set (resident sequences, distance matrix, edge scratch, DBA
-// traceback state and its path-word budget) with what is left of the
-// server's HBM budget; each slot's context plans its DBA batches within
-// budget / slots. A job's result does not depend on what runs beside it:
-// every kernel is deterministic and the path mode is fixed (full matrix).
+// Continuous batching (the default): each of `slots` worker threads owns a
+// context (a HIP stream) and prepares admitted jobs - upload, z-norm,
+// all-vs-all, medoid (odba_job_prepare) - and hands each prepared job to
+// the rounds engine of its value type and mode. An engine is a thread with
+// its own context that runs one DBA round of every job it holds per step,
+// packed into shared launches (odba_rounds_step); a job joins at the next
+// step after its preparation and leaves when its own stop rule ends it, so
+// jobs that arrive while others are on the device are batched with them.
+// Preparations and the engines' rounds overlap on the device. A job's
+// result does not depend on what runs beside it: every kernel is
+// deterministic and each job's sums are formed in its own sequence order.
+//
+// With ODBA_SERVE_PACKED=0 in the environment at creation, each slot
+// instead runs whole jobs (odba_perform_dba) one after another, and the
+// slots' streams overlap.
+//
+// Admission compares an estimate of the job's device working set
+// (resident sequences, distances, edge scratch, run records, centroids)
+// with what the engines' step-word budgets leave of the HBM budget.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -53,6 +63,17 @@ struct Job {
   std::vector<unsigned char> centroid;
   uint64_t clen = 0;
   int medoid = -1, rounds = 0;
+  odba_job* handle = nullptr;  // prepared job while its rounds run
+};
+
+// Rounds engine of one (value type, open start, open end) class.
+struct Engine {
+  odba_ctx* ctx = nullptr;
+  odba_rounds* eng = nullptr;
+  std::thread th;
+  std::deque<Job*> pending;   // prepared, not yet added
+  int held = 0;               // jobs added and not yet returned
+  uint64_t words = 0;         // step-word budget (bytes)
 };
 
 }  // namespace
@@ -62,13 +83,18 @@ struct odba_server {
   uint64_t budget = 0;
   int slots = 1;
   int cus = 256;
+  bool packed = true;
   std::vector<odba_ctx*> ctxs;
   std::vector<std::thread> workers;
   std::mutex mu;
-  std::condition_variable cv_admit, cv_done;
+  std::condition_variable cv_admit, cv_done, cv_engine, cv_reap;
   std::deque<Job*> queue;
   std::map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::map<int, std::unique_ptr<Engine>> engines;
+  std::deque<odba_job*> reap;
+  std::thread reaper;
   uint64_t in_use = 0;
+  uint64_t reserved = 0;  // the engines' step-word budgets
   int running = 0;
   bool stop = false;
   uint64_t next_id = 1;
@@ -79,7 +105,8 @@ struct odba_server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // Device bytes one job plans to hold while it runs.
+  // Device bytes one job holds while it runs, apart from step words (the
+  // engines' budgets; with whole-job slots, the slot's share).
   uint64_t footprint(const Job& j) const {
     const size_t es = elem_size(j.dtype);
     const long long n = (long long)j.lens.size();
@@ -94,18 +121,37 @@ struct odba_server {
     const uint64_t edge = (uint64_t)waves * 2 * (maxlen + 256) * es;
     const uint64_t dists = (uint64_t)npairs * es;
     const uint64_t runs = (uint64_t)n * maxlen * 16 + (uint64_t)padded(maxlen) * es * 4;
-    const uint64_t words = (uint64_t)sumlen * (uint64_t)maxlen / 4 + (uint64_t)n * 64 * 4 * (maxlen / 1024 + 1);
-    const uint64_t word_cap = budget / (uint64_t)slots / 4;
-    return (uint64_t)seq * es * 2 + dists + edge + runs + std::min(words, word_cap);
+    uint64_t words = 0;
+    if (!packed) {
+      words = (uint64_t)sumlen * (uint64_t)maxlen / 4 + (uint64_t)n * 64 * 4 * (maxlen / 1024 + 1);
+      words = std::min(words, budget / (uint64_t)slots / 4);
+    }
+    return (uint64_t)seq * es * 2 + dists + edge + runs + words;
   }
 
+  // Whole-job slots bound the jobs in flight; with packed rounds they bound
+  // the preparations only, and every prepared job joins its engine.
   bool can_admit() const {
-    if (queue.empty() || running >= slots) return false;
+    if (queue.empty() || (packed ? prep_slots_busy >= slots : running >= slots)) return false;
     const Job* j = queue.front();
-    return running == 0 || in_use + j->footprint <= budget;
+    return running == 0 || in_use + reserved + j->footprint <= budget;
   }
 
-  void worker(int slot) {
+  static int class_key(const Job& j) { return j.dtype * 4 + (j.os ? 2 : 0) + (j.oe ? 1 : 0); }
+
+  // Records a finished job's outcome (lock held).
+  void complete(Job* j, int rc, const std::string& msg) {
+    in_use -= j->footprint;
+    j->rc = rc;
+    if (rc) j->err = msg;
+    j->data.clear();
+    j->data.shrink_to_fit();
+    j->t_done = now();
+    j->state = rc ? ODBA_JOB_FAILED : ODBA_JOB_DONE;
+  }
+
+  // Whole-job slots: odba_perform_dba per admitted job.
+  void worker_whole(int slot) {
     odba_ctx* c = ctxs[slot];
     for (;;) {
       Job* j = nullptr;
@@ -133,23 +179,164 @@ struct odba_server {
                                       nullptr);
       {
         std::lock_guard<std::mutex> lk(mu);
-        in_use -= j->footprint;
         running--;
-        j->rc = rc;
-        if (rc) j->err = odba_last_error(c);
         j->centroid.assign(cen.begin(), cen.begin() + clen * elem_size(j->dtype));
         j->clen = clen;
         j->medoid = med;
         j->rounds = rounds;
-        j->data.clear();
-        j->data.shrink_to_fit();
-        j->t_done = now();
-        j->state = rc ? ODBA_JOB_FAILED : ODBA_JOB_DONE;
+        complete(j, rc, rc ? odba_last_error(c) : "");
       }
       cv_done.notify_all();
       cv_admit.notify_all();
     }
   }
+
+  // Packed rounds: a slot prepares admitted jobs and hands them on.
+  void worker_prepare(int slot) {
+    odba_ctx* c = ctxs[slot];
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_admit.wait(lk, [&] { return stop || can_admit(); });
+        if (stop) return;
+        j = queue.front();
+        queue.pop_front();
+        in_use += j->footprint;
+        running++;
+        prep_slots_busy++;
+        j->state = ODBA_JOB_RUNNING;
+        j->t_admit = now();
+      }
+      const int n = (int)j->lens.size();
+      std::vector<const void*> ptrs(n);
+      for (int i = 0; i < n; i++) ptrs[i] = j->data[i].data();
+      odba_job* h = nullptr;
+      const int rc = odba_job_prepare(c, j->dtype, ptrs.data(), j->lens.data(), n, j->os, j->oe, j->norm,
+                                      j->max_rounds, &h);
+      std::string msg = rc ? odba_last_error(c) : "";
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        prep_slots_busy--;
+        if (rc) {
+          running--;
+          complete(j, rc, msg);
+        } else {
+          j->handle = h;
+          j->data.clear();
+          j->data.shrink_to_fit();
+          Engine* e = engine_for(*j);
+          if (!e) {
+            running--;
+            reap.push_back(h);
+            j->handle = nullptr;
+            complete(j, ODBA_E_NOMEM, err);
+          } else {
+            e->pending.push_back(j);
+          }
+        }
+      }
+      cv_engine.notify_all();
+      cv_reap.notify_all();
+      cv_done.notify_all();
+      cv_admit.notify_all();
+    }
+  }
+  int prep_slots_busy = 0;
+
+  // The engine of a job's class, created on first use (lock held).
+  Engine* engine_for(const Job& j) {
+    const int key = class_key(j);
+    auto it = engines.find(key);
+    if (it != engines.end()) return it->second.get();
+    auto e = std::make_unique<Engine>();
+    if (odba_ctx_create(device, &e->ctx)) {
+      err = "cannot create a rounds-engine context";
+      return nullptr;
+    }
+    odba_ctx_set_graphs(e->ctx, 0);
+    e->words = budget / 4;
+    if (odba_rounds_create(e->ctx, j.dtype, j.os, j.oe, e->words, &e->eng)) {
+      err = odba_last_error(e->ctx);
+      odba_ctx_destroy(e->ctx);
+      return nullptr;
+    }
+    reserved += e->words;
+    Engine* ep = e.get();
+    e->th = std::thread([this, ep] { engine_loop(ep); });
+    engines[key] = std::move(e);
+    return ep;
+  }
+
+  void engine_loop(Engine* e) {
+    std::vector<odba_job*> fin(64);
+    std::map<odba_job*, Job*> owner;
+    for (;;) {
+      std::vector<Job*> add;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_engine.wait(lk, [&] { return (stop && preps_done) || !e->pending.empty() || e->held > 0; });
+        if (stop && preps_done && e->held == 0 && e->pending.empty()) return;
+        add.assign(e->pending.begin(), e->pending.end());
+        e->pending.clear();
+      }
+      for (Job* j : add) {
+        const int rc = odba_rounds_add(e->eng, j->handle);
+        std::lock_guard<std::mutex> lk(mu);
+        if (rc) {
+          running--;
+          reap.push_back(j->handle);
+          j->handle = nullptr;
+          complete(j, rc, odba_last_error(e->ctx));
+        } else {
+          owner[j->handle] = j;
+          e->held++;
+        }
+      }
+      int nf = 0, na = 0;
+      odba_rounds_step(e->eng, fin.data(), (int)fin.size(), &nf, &na);
+      if (nf == 0) continue;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (int k = 0; k < nf; k++) {
+          odba_job* h = fin[k];
+          Job* j = owner[h];
+          owner.erase(h);
+          e->held--;
+          running--;
+          int rc = ODBA_OK;
+          uint64_t clen = 0;
+          odba_job_result(h, nullptr, 0, &clen, &j->medoid, &j->rounds, &rc);
+          j->centroid.resize(clen * elem_size(j->dtype));
+          odba_job_result(h, j->centroid.data(), clen, &j->clen, nullptr, nullptr, nullptr);
+          j->handle = nullptr;
+          reap.push_back(h);
+          complete(j, rc, rc ? odba_last_error(e->ctx) : "");
+        }
+      }
+      cv_reap.notify_all();
+      cv_done.notify_all();
+      cv_admit.notify_all();
+    }
+  }
+
+  // Device memory of finished jobs is freed here: hipFree waits for the
+  // work queued on the device, which would stall an engine or a slot.
+  void reap_loop() {
+    for (;;) {
+      odba_job* h = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_reap.wait(lk, [&] { return !reap.empty() || (stop && workers_done); });
+        if (reap.empty()) return;
+        h = reap.front();
+        reap.pop_front();
+      }
+      odba_job_destroy(h);
+    }
+  }
+  bool workers_done = false;
+  bool preps_done = false;
 };
 
 extern "C" {
@@ -159,6 +346,8 @@ int odba_server_create(int device, uint64_t hbm_budget, int slots, odba_server**
   auto* s = new odba_server();
   s->device = device;
   s->slots = slots;
+  const char* pk = std::getenv("ODBA_SERVE_PACKED");
+  s->packed = !(pk && !std::strcmp(pk, "0"));
   for (int k = 0; k < slots; k++) {
     odba_ctx* c = nullptr;
     const int rc = odba_ctx_create(device, &c);
@@ -179,7 +368,12 @@ int odba_server_create(int device, uint64_t hbm_budget, int slots, odba_server**
     odba_ctx_set_mem_cap(c, s->budget / (uint64_t)slots);
     odba_ctx_set_graphs(c, 0);  // graph replays serialise the slots' streams (measured)
   }
-  for (int k = 0; k < slots; k++) s->workers.emplace_back([s, k] { s->worker(k); });
+  if (s->packed) {
+    s->reaper = std::thread([s] { s->reap_loop(); });
+    for (int k = 0; k < slots; k++) s->workers.emplace_back([s, k] { s->worker_prepare(k); });
+  } else {
+    for (int k = 0; k < slots; k++) s->workers.emplace_back([s, k] { s->worker_whole(k); });
+  }
   *out = s;
   return ODBA_OK;
 }
@@ -196,7 +390,27 @@ int odba_server_destroy(odba_server* s) {
     s->queue.clear();
   }
   s->cv_admit.notify_all();
-  for (auto& t : s->workers) t.join();
+  s->cv_done.notify_all();
+  for (auto& t : s->workers) t.join();  // preparations in flight are handed to their engines
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->preps_done = true;
+  }
+  s->cv_engine.notify_all();
+  for (auto& kv : s->engines) {  // engines finish the jobs they hold
+    Engine* e = kv.second.get();
+    if (e->th.joinable()) e->th.join();
+    odba_rounds_destroy(e->eng);
+    odba_ctx_destroy(e->ctx);
+  }
+  if (s->reaper.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->workers_done = true;
+    }
+    s->cv_reap.notify_all();
+    s->reaper.join();
+  }
   for (odba_ctx* c : s->ctxs) odba_ctx_destroy(c);
   delete s;
   return ODBA_OK;

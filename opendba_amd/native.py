@@ -26,7 +26,8 @@ EXPORTS = [
     "odba_server_create", "odba_server_destroy", "odba_server_submit", "odba_server_wait", "odba_server_result",
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
     "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
-    "odba_ctx_set_graphs", "odba_dtw_all_pairs_noalloc",
+    "odba_ctx_set_graphs", "odba_dtw_all_pairs_noalloc", "odba_job_prepare", "odba_job_result", "odba_job_destroy",
+    "odba_rounds_create", "odba_rounds_add", "odba_rounds_step", "odba_rounds_destroy",
 ]
 SCHED_WAVE, SCHED_BANDS = 0, 1
 
@@ -101,6 +102,13 @@ def load():
     L.odba_cluster.argtypes = [i32, vp, i32, vp, C.c_double, vp, vp, C.POINTER(i32)]
     L.odba_hclust_complete.argtypes = [i32, vp, vp, vp]
     L.odba_cutree_k.argtypes = [i32, vp, i32, vp]
+    L.odba_job_prepare.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, C.POINTER(vp)]
+    L.odba_job_result.argtypes = [vp, vp, u64, C.POINTER(u64), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.odba_job_destroy.argtypes = [vp]
+    L.odba_rounds_create.argtypes = [vp, i32, i32, i32, u64, C.POINTER(vp)]
+    L.odba_rounds_add.argtypes = [vp, vp]
+    L.odba_rounds_step.argtypes = [vp, vp, i32, C.POINTER(i32), C.POINTER(i32)]
+    L.odba_rounds_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -406,3 +414,67 @@ class SeqSet:
         if self.h:
             self.ctx.L.odba_seqset_destroy(self.h)
             self.h = C.c_void_p()
+
+
+class Job:
+    """A prepared DBA job (odba_job_prepare): sanitized, sorted, z-normalised
+    sequences resident on the device, all-vs-all done, medoid found."""
+
+    def __init__(self, ctx, seqs, open_start=0, open_end=0, norm=1, max_rounds=250):
+        self.L = ctx.L
+        self.ctx = ctx
+        seqs = [np.ascontiguousarray(s) for s in seqs]
+        self.dtype = seqs[0].dtype
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        self.maxlen = int(lens.max())
+        self.h = C.c_void_p()
+        ctx._check(self.L.odba_job_prepare(ctx.h, CODE[self.dtype], _ptrs(seqs), _p(lens), len(seqs), open_start,
+                                           open_end, norm, max_rounds, C.byref(self.h)))
+
+    def result(self):
+        cen = np.zeros(self.maxlen, dtype=self.dtype)
+        clen = C.c_uint64()
+        med, rounds, jrc = C.c_int(), C.c_int(), C.c_int()
+        rc = self.L.odba_job_result(self.h, _p(cen), self.maxlen, C.byref(clen), C.byref(med), C.byref(rounds),
+                                    C.byref(jrc))
+        if rc:
+            raise OdbaError(rc, "odba_job_result failed")
+        return dict(centroid=cen[: clen.value], medoid=med.value, rounds=rounds.value, rc=jrc.value)
+
+    def close(self):
+        if self.h:
+            self.L.odba_job_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Rounds:
+    """Packed DBA rounds over prepared jobs of one value type and mode
+    (odba_rounds_*): each step runs one round of every job held."""
+
+    def __init__(self, ctx, dtype, open_start=0, open_end=0, word_bytes=1 << 34):
+        self.L = ctx.L
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx._check(self.L.odba_rounds_create(ctx.h, CODE[np.dtype(dtype)], open_start, open_end, word_bytes,
+                                             C.byref(self.h)))
+
+    def add(self, job):
+        self.ctx._check(self.L.odba_rounds_add(self.h, job.h))
+
+    def step(self, capacity=256):
+        """One packed round; returns (handles of jobs that ended, jobs still held)."""
+        fin = (C.c_void_p * capacity)()
+        nf, na = C.c_int(), C.c_int()
+        self.ctx._check(self.L.odba_rounds_step(self.h, fin, capacity, C.byref(nf), C.byref(na)))
+        return [fin[k] for k in range(nf.value)], na.value
+
+    def close(self):
+        if self.h:
+            self.L.odba_rounds_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
