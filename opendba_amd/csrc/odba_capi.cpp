@@ -896,9 +896,11 @@ template <typename J> double path_tail(const std::vector<J>& jobs) {
 // Path jobs: one wave per job, or (float, few long jobs) a workgroup of
 // waves per job sweeping its swaths as a pipeline. ODBA_PATH_SCHEDULE =
 // wave|strip overrides the choice.
+// bands_per_cu > 0: take the swath-band schedule (float, with an error
+// word) whatever the job count, with that many workgroups per CU.
 int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const void* jobs_dev, int njobs, int maxly,
                   uint32_t* mv, uint32_t* mtop, int* abr, int maxlx = 0, int minly = 0, int* err = nullptr,
-                  double mean_swaths = 0.0, double tail = 0.0) {
+                  double mean_swaths = 0.0, double tail = 0.0, int bands_per_cu = 0) {
   const size_t es = dtype_size(dtype);
   const long long stride = maxly + 256;
   unsigned int* counter = (unsigned int*)((char*)c->small + 64);
@@ -927,8 +929,9 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   // strip workgroup has waves (16): its swaths would run one after another
   // on one CU. Bands of whole swaths spread them over several workgroups.
   bool swath_bands = njobs < 4 * c->cus && njobs <= 1024 && kmax > 16 && minly >= 64;
+  if (bands_per_cu > 0) swath_bands = maxlx > 0 && minly >= 64;
   if (env && std::strcmp(env, "swathband")) swath_bands = false;
-  if (env && !std::strcmp(env, "swathband")) swath_bands = njobs <= 1024 && maxlx > 0;
+  if (env && !std::strcmp(env, "swathband")) swath_bands = maxlx > 0;
   swath_bands = swath_bands && dtype == ODBA_F32 && err != nullptr;  // err: the hand-off time-out word
   if (env && !std::strcmp(env, "band"))
     return fail(c, ODBA_E_INVALID, "the path band schedule is switched off (an open-end run faulted; cause not found)");
@@ -940,7 +943,7 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   hipError_t e;
   if (swath_bands) {
     const int nw = std::max(1, std::min(kPathStripWaves, env_int("ODBA_PATH_BAND_WAVES", 4)));
-    const int per_cu = std::max(1, env_int("ODBA_PATH_BANDS_PER_CU", 2));
+    const int per_cu = std::max(1, bands_per_cu > 0 ? bands_per_cu : env_int("ODBA_PATH_BANDS_PER_CU", 2));
     const long long max_claims = (long long)njobs * ((kmax + nw - 1) / nw);
     const int nblocks = (int)std::max(1LL, std::min(max_claims, (long long)per_cu * c->cus));
     const int R = 2 * nblocks + 2;
@@ -2164,8 +2167,12 @@ template <typename T> struct RoundEngine {
   // own stream with its traceback behind it: a strip workgroup is as wide
   // as the widest job of its launch, so one-swath jobs packed with
   // six-swath jobs would leave most of their workgroup's waves idle.
-  static constexpr int kGroups = 3;
-  static int group_of(int K) { return K <= 1 ? 0 : (K <= 8 ? 1 : 2); }
+  // Jobs of more than 16 swaths (more than a strip workgroup's waves) go to
+  // the swath-band schedule, so the longest pair of a step spreads over
+  // CUs instead of setting the step's length on one workgroup.
+  static constexpr int kGroups = 4;
+  static int group_of(int K) { return K <= 1 ? 0 : (K <= 8 ? 1 : (K <= 16 ? 2 : 3)); }
+  static int bands_per_cu(int g) { return g == 3 ? 4 : 0; }
 
   odba_ctx* c = nullptr;
   odba_ctx* sub[kGroups] = {};
@@ -2470,7 +2477,7 @@ template <typename T> struct RoundEngine {
         odba_ctx* sc = sub[g];
         HIPC(c, hipStreamWaitEvent(sc->stream, ev_go, 0));
         int rc = run_path_jobs(sc, dtype_of<T>(), G.minfirst, os, oe, tab + G.pj_at[v], G.nj, G.maxly, mv[g], mtop[g],
-                               abr[g], G.maxlx, G.minly, derr, G.msw, G.tail);
+                               abr[g], G.maxlx, G.minly, derr, G.msw, G.tail, bands_per_cu(g));
         if (rc) return fail(c, rc, "%s", sc->err.c_str());
         HIPC(c, launch_traceback(sc->stream, (const TbJob*)(tab + G.tb_at), G.nj, mv[g], mtop[g], abr[g], nullptr, 0, os,
                                  oe, derr, nullptr));

@@ -14,12 +14,11 @@ namespace odba {
 // 1024-wide column block and the row window 128 entries past the end.
 inline long long padded_len(long long L) { return ((L + 63) / 64) * 64 + 1024; }
 
-// Waves of a path-strip or swath-band workgroup at most. A path cell holds
-// a step word and more temporaries than the distance-only sweep: about 165
-// VGPRs at 16 columns per lane. Workgroups of at most 12 waves (three per
-// SIMD) keep all of it in registers; 16-wave workgroups capped the kernels
-// at 128 VGPRs and spilled 46-74 of them to scratch inside the sweep.
-constexpr int kPathStripWaves = 12;
+// Waves of a path-strip or swath-band workgroup at most: 16 caps a path
+// kernel at 128 VGPRs, four waves per SIMD. With the step bits in two
+// planes (plane_code) the whole-swath variants fit that without spills
+// (2-bit code packing needed 160-168 VGPRs and spilled 46-74 at this cap).
+constexpr int kPathStripWaves = 16;
 
 // Path step words. A lane's word for one step holds its C <= 16 cells in
 // two bit planes: bit 15 - q of the low half is set when column q did not

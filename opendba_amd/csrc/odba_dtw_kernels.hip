@@ -1558,11 +1558,19 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_strips(const Path
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
                                                      long long* __restrict__ band0, int per_band) {
+  // One workgroup: thread t sums the bands of its chunk of claim-order jobs,
+  // the chunk sums are scanned, then each thread writes its chunk's prefixes.
   __shared__ long long s[1024];
   const int t = threadIdx.x;
-  const int K = (t < njobs) ? (jobs[njobs - 1 - t].lx + ODBA_W - 1) / ODBA_W : 0;  // swaths of the job
-  const long long v = (long long)((K + per_band - 1) / per_band);                  // its bands
-  s[t] = v;
+  const int per = (njobs + 1023) / 1024;
+  const int p0 = min(njobs, t * per), p1 = min(njobs, p0 + per);
+  auto bands = [&](int p) -> long long {
+    const int K = (jobs[njobs - 1 - p].lx + ODBA_W - 1) / ODBA_W;  // swaths of the job
+    return (long long)((K + per_band - 1) / per_band);
+  };
+  long long mine = 0;
+  for (int p = p0; p < p1; p++) mine += bands(p);
+  s[t] = mine;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
     const long long a = t >= o ? s[t - o] : 0;
@@ -1570,8 +1578,12 @@ __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __res
     s[t] += a;
     __syncthreads();
   }
-  if (t < njobs) band0[t] = s[t] - v;
-  if (t == njobs - 1) band0[njobs] = s[t];
+  long long run = s[t] - mine;
+  for (int p = p0; p < p1; p++) {
+    band0[p] = run;
+    run += bands(p);
+  }
+  if (t == 1023) band0[njobs] = s[t];
 }
 
 __global__ void k_abr_index(int* __restrict__ abr, int n) {
@@ -2320,7 +2332,7 @@ hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStrea
                                    long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
                                    int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
                                    int* err, long long tick_limit) {
-  if (njobs < 1 || njobs > 1024 || nw < 1 || nw > kPathStripWaves || ring < 128 || (ring & (ring - 1)) ||
+  if (njobs < 1 || nw < 1 || nw > kPathStripWaves || ring < 128 || (ring & (ring - 1)) ||
       R < 2 * nblocks + 2)
     return hipErrorInvalidValue;
   // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1] (8-aligned)

@@ -135,3 +135,22 @@ def test_packed_rounds_stop_rule_and_leaving_jobs():
     assert ended[0] == [0]
     for p, q in zip(plain, res):
         _same(p, q)
+
+
+@pytest.mark.parametrize("oe", [0, 1])
+def test_packed_rounds_long_jobs_take_swath_bands(oe):
+    """Jobs whose pairs span more than 16 swaths (sequences of 17k-21k
+    samples: flipped onto the columns in open end, a long medoid in global
+    mode) run their path jobs as swath bands inside the packed step; results
+    equal the plain path, with a short job packed beside them."""
+    rng = np.random.default_rng(61 + oe)
+    long_job = [np.cumsum(rng.normal(size=int(L))).astype(np.float32) for L in rng.integers(17000, 21000, size=4)]
+    short_job = [np.cumsum(rng.normal(size=int(L))).astype(np.float32) for L in rng.integers(300, 2000, size=5)]
+    mixed_job = long_job[:2] + short_job[:3]
+    jobs = [long_job, short_job, mixed_job]
+    print(f"packed rounds with long jobs oe={oe}", flush=True)
+    with N.Context(0) as ctx, N.Context(0) as ectx:
+        plain = [ctx.perform_dba(s, 0, oe, norm=1, max_rounds=4) for s in jobs]
+        packed = _run_packed(ctx, ectx, jobs, 0, oe, 4, 1 << 34, join_every=1)
+    for p, q in zip(plain, packed):
+        _same(p, q)

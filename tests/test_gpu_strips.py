@@ -331,3 +331,23 @@ def test_mixed_schedule_integers(ctx, dtype):
         got, _ = _run(ctx, seqs, os_, oe, "mixed")
         exp = O.all_pairs(seqs, os_, oe)
         assert np.array_equal(got, exp), (os_, oe)
+
+
+@pytest.mark.parametrize("oe", [0, 1])
+def test_path_swath_bands_more_than_1024_jobs(ctx, oe):
+    """The swath-band schedule's claim table takes any number of jobs (the
+    packed rounds of a server hand it thousands): a DBA round of 1,100 short
+    sequences against a two-swath centroid, forced onto swath bands, against
+    the oracle."""
+    rng = np.random.default_rng(95 + oe)
+    lens = rng.integers(80, 200, size=1100)
+    seqs = O.znorm(_walks(rng, [int(L) for L in lens]))[0]
+    cen = O.znorm(_walks(rng, [1500]))[0][0]
+    print(f"swath bands over 1100 jobs oe={oe}", flush=True)
+    with path_schedule("swathband"):
+        got, counts, delta = ctx.dba_round(cen, seqs, 0, oe)
+    rc, exp, ecounts, edelta = O.dba_update(cen, seqs, os_=0, oe=oe)
+    assert rc == 0
+    assert np.array_equal(counts, ecounts)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert delta == edelta
