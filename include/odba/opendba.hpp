@@ -5,6 +5,8 @@
  *
  *   readSequenceTextFiles / readSequenceBinaryFiles / readSequenceTSVFiles
  *                                     cpu_utils.hpp:60-134, 381-456, 553-624
+ *   readSequenceSLOW5Files / writeSlow5Output (odba_slow5.hpp)
+ *                                     cpu_utils.hpp:458-500, io_utils.hpp:363-452
  *   normalizeSequences<T>             gpu_utils.hpp:131-175
  *   approximateMedoidIndices<T>       dba.hpp:36-365
  *   DBAUpdate<T>                      dba.hpp:432-848
@@ -34,6 +36,7 @@
 #include <vector>
 
 #include "odba.h"
+#include "odba_slow5.hpp"
 
 #define TEXT_READ_MODE 0
 #define BINARY_READ_MODE 1
@@ -600,7 +603,7 @@ template <typename T>
 void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char** sequence_names, int use_open_start,
                 int use_open_end, char* output_prefix, int norm_sequences, double cdist, char** series_file_names,
                 int num_series, int read_mode, bool is_segmented, int algo_mode, hipStream_t stream = 0) {
-  (void)series_file_names; (void)num_series; (void)read_mode; (void)is_segmented; (void)stream;
+  (void)stream;
   const int n = num_sequences;
   for (int i = 0; i < n; i++)  // dba.hpp:868-872
     if (sequence_lengths[i] && (double)sequences[i][sequence_lengths[i] - 1] >= std::sqrt((double)std::numeric_limits<T>::max()))
@@ -653,6 +656,12 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
     std::cerr << "Cannot open sequence averages file " << prefix << ".avg.txt for writing" << std::endl;
     std::exit(CANNOT_WRITE_DBA_AVG);
   }
+  // A single signal file's consensus sequences are also written back into a
+  // copy of its records, as <prefix>.avg.blow5 (dba.hpp:1170-1185).
+  const bool want_blow5 = !is_segmented && read_mode == SLOW5_READ_MODE && num_series == 1;
+  std::vector<std::vector<short>> avg_short(want_blow5 ? nclusters : 0);
+  std::vector<char*> avg_names(want_blow5 ? nclusters : 0, nullptr);
+  std::vector<size_t> avg_lens(want_blow5 ? nclusters : 0, 0);
   for (int cc = first; cc < nclusters; cc++) {
     const int med = medoids[cc];
     const size_t lc = sequence_lengths[med];
@@ -704,12 +713,36 @@ void performDBA(T** sequences, int num_sequences, size_t* sequence_lengths, char
       std::cerr << "Outputting singleton sequence " << sequence_names[med] << " as-is (a.k.a. cluster " << (cc + 1) << "/"
                 << nclusters << ")." << std::endl;
     }
+    if (want_blow5) {  // dba.hpp:1008-1037, 1133-1139
+      avg_short[cc].resize(lc);
+      avg_names[cc] = sequence_names[med];
+      avg_lens[cc] = lc;
+      if (cs.size() == 1 && norm_sequences)  // a singleton's short comes from the double rescale itself
+        for (size_t i = 0; i < lc; i++) avg_short[cc][i] = (short)(mu[med] + nb[i] * sg[med]);
+    }
     if (norm_sequences)  // dba.hpp:1017-1023, 1117-1125
       for (size_t i = 0; i < lc; i++) nb[i] = (T)(mu[med] + nb[i] * sg[med]);
+    if (want_blow5 && !(cs.size() == 1 && norm_sequences))  // templateToShort of the (rescaled) T values
+      for (size_t i = 0; i < lc; i++) avg_short[cc][i] = (short)nb[i];
     avgs << sequence_names[med];
     for (size_t i = 0; i < lc; ++i) avgs << "\t" << nb[i];
     avgs << std::endl;
     avgs.flush();
+  }
+  if (want_blow5 && first > 0) {
+    // The reference's resume leaves the skipped cluster's buffers unset;
+    // this layer writes no partial file.
+    std::cerr << "Not writing " << prefix << ".avg.blow5 after a resumed run (cluster 1 came from the checkpoint)"
+              << std::endl;
+  } else if (want_blow5) {
+    std::cerr << "Writing medoids to new slow5 file..." << std::endl;
+    std::vector<short*> sp(nclusters);
+    for (int cc = 0; cc < nclusters; cc++) sp[cc] = avg_short[cc].data();
+    if (writeSlow5Output(series_file_names[0], (prefix + ".avg.blow5").c_str(), avg_names.data(), sp.data(),
+                         avg_lens.data(), nclusters) == 1) {
+      std::cerr << "Cannot write updated sequences to new Fast5 file " << prefix << ".avg.blow5, aborting." << std::endl;
+      std::exit(CANNOT_WRITE_UPDATED_SLOW5);
+    }
   }
   delete[] medoids;
 }
@@ -763,6 +796,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
   int n;
   if (read_mode == BINARY_READ_MODE) n = readSequenceBinaryFiles<T>(series_file_names, num_series, &seqs, &names, &lens, is_short);
   else if (read_mode == TSV_READ_MODE) n = readSequenceTSVFiles<T>(series_file_names, num_series, &seqs, &names, &lens);
+  else if (read_mode == SLOW5_READ_MODE) n = readSequenceSLOW5Files<T>(series_file_names, num_series, &seqs, &names, &lens);
   else n = readSequenceTextFiles<T>(series_file_names, num_series, &seqs, &names, &lens);
   if (n < 2) {
     std::cerr << "At least two sequences must be provided to calculate an average, but found " << n << ", aborting" << std::endl;
@@ -790,6 +824,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
     std::free(pre_len);
   }
   const int n_raw = n;  // unsegmented sequences held (the two-stage flow keeps them)
+  int reloaded = -1;    // sequences read again for the consensus (signal files)
   if (minseg > 0) {  // openDBA.cuh:85-177
     T** seg = nullptr;
     size_t* seglens = nullptr;
@@ -824,6 +859,20 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
       for (int i = 0; i < n; i++) std::free(seg[i]);
       std::free(seg);
       std::free(seglens);
+      // A prefix-chopped signal file is read again for a consensus without
+      // segmentation, so the consensus keeps the leader as a raw signal
+      // would (openDBA.cuh:135-167).
+      if (seqprefix_file_name != nullptr && min2 == 0 && read_mode == SLOW5_READ_MODE) {
+        std::cerr << "Restoring raw signals (no prefix chop) before FAST5/SLOW5 consensus generation without segmentation"
+                  << std::endl;
+        for (int i = 0; i < n_raw; i++) std::free(seqs[i]);
+        for (int i = 0; i < n; i++) std::free(names[i]);
+        std::free(seqs);
+        std::free(names);
+        std::free(lens);
+        n = readSequenceSLOW5Files<T>(series_file_names, num_series, &seqs, &names, &lens);
+        reloaded = n;
+      }
     } else {
       std::free(seqs);
       std::free(lens);
@@ -840,7 +889,7 @@ void setupAndRun(char* seqprefix_file_name, char** series_file_names, int num_se
     performDBA<T>(seqs, n, lens, names, use_open_start, use_open_end, output_prefix, norm_sequences, cdist,
                   series_file_names, num_series, read_mode, minseg > 1, CLUSTER_AND_CONSENSUS);
   }
-  const int held = (min2 != -1 || minseg <= 0) ? n_raw : n;
+  const int held = reloaded >= 0 ? reloaded : ((min2 != -1 || minseg <= 0) ? n_raw : n);
   for (int i = 0; i < held; i++) std::free(seqs[i]);
   for (int i = 0; i < n; i++) std::free(names[i]);
   (void)n_read;

@@ -26,7 +26,7 @@ int main(int argc, char** argv) {
     if (c == 'n') norm_sequences = 0;
   }
   if (argc < 9) {
-    std::cout << "Usage: " << argv[0] << " <binary|text|tsv> <short|int|uint|ulong|float|double> "
+    std::cout << "Usage: " << argv[0] << " <binary|text|tsv|slow5> <short|int|uint|ulong|float|double> "
               << "<global|open_start|open_end|open> <output files prefix> <minimum unimodal segment length for "
                  "clustering[,for consensus generation]> <prefix sequence to remove|/dev/null> <clustering threshold> "
                  "<series.tsv|<series1> <series2> [series3...]>\n";
@@ -37,6 +37,7 @@ int main(int argc, char** argv) {
   int read_mode = TEXT_READ_MODE;
   if (!std::strcmp(argv[1], "binary")) read_mode = BINARY_READ_MODE;
   else if (!std::strcmp(argv[1], "tsv")) read_mode = TSV_READ_MODE;
+  else if (!std::strcmp(argv[1], "slow5")) read_mode = SLOW5_READ_MODE;
   else if (std::strcmp(argv[1], "text")) {
     std::cerr << "First argument (" << argv[1] << ") is neither 'binary' nor 'text'" << std::endl;
     std::exit(1);
