@@ -279,8 +279,7 @@ uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length)
  * engine holds prepared jobs of one value type and one (open start, open
  * end) mode; each odba_rounds_step runs one DBA round (full path matrices)
  * of every job it holds, packed into shared launches on the engine
- * context's stream, in batches of whole jobs whose step words fit
- * word_bytes. Each job keeps its own stop rule (delta 0, the open-mode
+ * context's stream. Each job keeps its own stop rule (delta 0, the open-mode
  * repeat test, max_rounds) and its result (centroid rescaled by the
  * medoid's mu/sigma when normalised, medoid index in sorted order, rounds)
  * equals odba_perform_dba's with ODBA_PATH_FULL. Jobs that ended are
@@ -290,6 +289,12 @@ uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length)
  * of the same device. odba_job_destroy frees a job's device memory (also
  * the engine's buffers for it) and must not be called while an engine
  * holds the job.
+ * A job's step words (2 bits per path-matrix cell, every swath of every
+ * member, and the open-end top-row words) lie in a buffer of its own that
+ * lives as long as the job: odba_job_words gives its size in bytes and,
+ * with allocate != 0, allocates it on the calling thread (a server does
+ * this on its preparing thread once the bytes fit its memory budget);
+ * odba_rounds_add allocates it when that has not been done.
  */
 typedef struct odba_job odba_job;
 typedef struct odba_rounds odba_rounds;
@@ -297,9 +302,9 @@ int odba_job_prepare(odba_ctx* ctx, int dtype, const void* const* seqs, const ui
                      int use_open_start, int use_open_end, int norm, int max_rounds, odba_job** out);
 int odba_job_result(const odba_job* job, void* centroid_out, uint64_t capacity, uint64_t* centroid_len,
                     int* medoid_sorted, int* rounds, int* job_rc);
+int odba_job_words(odba_job* job, int allocate, uint64_t* word_bytes);
 int odba_job_destroy(odba_job* job);
-int odba_rounds_create(odba_ctx* ctx, int dtype, int use_open_start, int use_open_end, uint64_t word_bytes,
-                       odba_rounds** out);
+int odba_rounds_create(odba_ctx* ctx, int dtype, int use_open_start, int use_open_end, odba_rounds** out);
 int odba_rounds_add(odba_rounds* engine, odba_job* job);
 int odba_rounds_step(odba_rounds* engine, odba_job** finished, int capacity, int* n_finished, int* n_active);
 int odba_rounds_destroy(odba_rounds* engine);

@@ -2129,7 +2129,9 @@ struct odba_job {
   int device = 0;
   int max_rounds = 250;
   void* prep = nullptr;          // Prepared<T>*
-  std::vector<void*> dev;        // the rounds engine's per-job device buffers, freed with the job
+  std::vector<void*> dev;        // the job's round buffers (step words, centroids, run records), freed with the job
+  uint32_t* words = nullptr;     // step words of one round (odba_job_words)
+  uint64_t word_bytes = 0;
   std::vector<unsigned char> cen;
   uint64_t clen = 0;
   int med = -1, rounds = 0, rc = ODBA_OK;
@@ -2151,9 +2153,40 @@ namespace {
 // same input whatever else is packed with it. Jobs join before a step and
 // leave when their stop rule ends them.
 // ---------------------------------------------------------------------------
+// Geometry of member q of a prepared job: flip, rows, columns (dba.hpp:497-501).
+template <typename T> void member_shape(const Prepared<T>& P, int q, int& flip, int& ly, int& lx) {
+  flip = (P.oe && P.lc < P.hl[q]) ? 1 : 0;
+  ly = flip ? P.lc : P.hl[q];
+  lx = flip ? P.hl[q] : P.lc;
+}
+
+// Step words of one round of a prepared job: every member's path step
+// words (one swath_words block per 1024-column swath) in member order, then
+// every member's open-end top-row words (64 per swath). mo / to (optional)
+// receive each member's word offsets; returns the total in words.
+template <typename T> long long job_word_layout(const Prepared<T>& P, std::vector<long long>* mo,
+                                                std::vector<long long>* to) {
+  long long mvw = 0, topw = 0;
+  if (mo) mo->assign(P.n, 0);
+  if (to) to->assign(P.n, 0);
+  for (int q = 0; q < P.n; q++) {
+    int flip, ly, lx;
+    member_shape(P, q, flip, ly, lx);
+    const int K = (lx + ODBA_W - 1) / ODBA_W;
+    if (mo) (*mo)[q] = mvw;
+    if (to) (*to)[q] = topw;
+    mvw += (long long)K * swath_words(ly, P.os, P.oe);
+    topw += (long long)K * 64;
+  }
+  if (to)
+    for (long long& t : *to) t += mvw;
+  return mvw + topw;
+}
+
 template <typename T> struct EJob {
   odba_job* h = nullptr;
   Prepared<T>* P = nullptr;
+  std::vector<long long> mo, to;  // member word offsets in the job's word buffer
   T* cb[2] = {nullptr, nullptr};  // centroid buffers (padded): a round reads cb[cur] and writes cb[cur ^ 1]
   int cur = 0;
   int4* runs = nullptr;           // run records [member][position]
@@ -2178,7 +2211,6 @@ template <typename T> struct RoundEngine {
   odba_ctx* sub[kGroups] = {};
   hipEvent_t ev_go = nullptr, ev_done[kGroups] = {};
   int os = 0, oe = 0;
-  long long budget_words = 0;
   std::vector<std::unique_ptr<EJob<T>>> act;
   std::vector<odba_job*> finished;
   bool dirty = true;
@@ -2187,23 +2219,19 @@ template <typename T> struct RoundEngine {
     int nj = 0;
     int maxly = 1, maxlx = 0, minly = INT32_MAX;
     double msw = 0, tail = 0;
+    double est = 0;  // expected length: the longest member's row-swaths, or all of them over four waves per CU
     bool minfirst = true;
     size_t pj_at[2] = {0, 0}, tb_at = 0;  // byte offsets into the table arena
   };
-  struct Batch {
-    Group g[kGroups];
-    int ns = 0;
-    long long npos = 0;
-    size_t sl_at[2] = {0, 0};
-  };
-  std::vector<Batch> batches;
+  Group grp[kGroups];
+  int order[kGroups] = {0, 1, 2, 3};  // launch order: longest expected group first
+  int ns = 0;
+  long long npos = 0;
+  size_t sl_at[2] = {0, 0};
+  uint32_t* base = nullptr;          // lowest job word buffer: the launches' word base
   std::vector<unsigned char> htab;  // host image of the table arena (kept until the next rebuild)
   unsigned char* tab = nullptr;
   size_t tab_cap = 0;
-  uint32_t* mv[kGroups] = {};
-  size_t mv_cap[kGroups] = {};
-  uint32_t* mtop[kGroups] = {};
-  size_t mtop_cap[kGroups] = {};
   int* abr[kGroups] = {};
   size_t abr_cap[kGroups] = {};
   T* acc = nullptr;
@@ -2217,11 +2245,10 @@ template <typename T> struct RoundEngine {
   unsigned long long* hbits = nullptr;
   size_t hbits_cap = 0;
 
-  int init(odba_ctx* ctx, int os_, int oe_, uint64_t word_bytes) {
+  int init(odba_ctx* ctx, int os_, int oe_) {
     c = ctx;
     os = os_;
     oe = oe_;
-    budget_words = (long long)std::max<uint64_t>(word_bytes / 4, 1);
     for (int g = 0; g < kGroups; g++) {
       int rc = odba_ctx_create(c->device, &sub[g]);
       if (rc) return fail(c, rc, "cannot create a path stream: %s", sub[g] ? sub[g]->err.c_str() : "");
@@ -2235,8 +2262,7 @@ template <typename T> struct RoundEngine {
   ~RoundEngine() {
     if (c) hipSetDevice(c->device);
     for (int g = 0; g < kGroups; g++) {
-      for (void* p : {(void*)mv[g], (void*)mtop[g], (void*)abr[g]})
-        if (p) hipFree(p);
+      if (abr[g]) hipFree(abr[g]);
       if (ev_done[g]) hipEventDestroy(ev_done[g]);
       if (sub[g]) odba_ctx_destroy(sub[g]);
     }
@@ -2263,9 +2289,12 @@ template <typename T> struct RoundEngine {
   int add(odba_job* h) {
     auto* P = (Prepared<T>*)h->prep;
     if (P->os != os || P->oe != oe) return fail(c, ODBA_E_INVALID, "job mode differs from the engine's");
+    if (!h->words && odba_job_words(h, 1, nullptr) != ODBA_OK)
+      return fail(c, ODBA_E_NOMEM, "cannot allocate %llu bytes of step words", (unsigned long long)h->word_bytes);
     auto J = std::make_unique<EJob<T>>();
     J->h = h;
     J->P = P;
+    job_word_layout(*P, &J->mo, &J->to);
     const long long pl = padded_len(P->lc);
     for (int k = 0; k < 2; k++) {
       HIPC(c, hipMalloc((void**)&J->cb[k], pl * sizeof(T)));
@@ -2298,21 +2327,19 @@ template <typename T> struct RoundEngine {
     finished.push_back(h);
   }
 
-  // Geometry of member q of a job: flip, rows, columns (dba.hpp:497-501).
-  void shape(const Prepared<T>& P, int q, int& flip, int& ly, int& lx) const {
-    flip = (oe && P.lc < P.hl[q]) ? 1 : 0;
-    ly = flip ? P.lc : P.hl[q];
-    lx = flip ? P.hl[q] : P.lc;
-  }
-
-  // Batches of whole jobs whose step words fit the budget; inside a batch
-  // each group's path jobs are listed by ascending work, so the path
-  // kernels (which claim from the end of the list) start the longest first.
+  // Tables of one step: inside each group the path jobs are listed by
+  // ascending work, so the path kernels (which claim from the end of the
+  // list) start the longest first. Every job's step words lie in its own
+  // buffer; the launches take the lowest of those buffers as their word
+  // base and each member's word offsets from it.
   int rebuild() {
-    batches.clear();
     htab.clear();
-    size_t mvmax[kGroups], topmax[kGroups], njmax[kGroups], posmax = 1;
-    for (int g = 0; g < kGroups; g++) mvmax[g] = topmax[g] = njmax[g] = 1;
+    size_t njmax[kGroups];
+    for (int g = 0; g < kGroups; g++) {
+      njmax[g] = 1;
+      grp[g] = Group();
+      order[g] = g;
+    }
     struct Item { double cost; int j, q; };
     auto align = [&]() { htab.resize((htab.size() + 255) & ~(size_t)255); return htab.size(); };
     auto put = [&](const void* p, size_t bytes) {
@@ -2320,98 +2347,84 @@ template <typename T> struct RoundEngine {
       htab.insert(htab.end(), (const unsigned char*)p, (const unsigned char*)p + bytes);
       return at;
     };
-    size_t j0 = 0;
-    while (j0 < act.size()) {
-      long long mvw = 0;
-      size_t j1 = j0;
-      for (; j1 < act.size(); j1++) {
-        const Prepared<T>& P = *act[j1]->P;
-        long long need = 0;
-        for (int q = 0; q < P.n; q++) {
-          int flip, ly, lx;
-          shape(P, q, flip, ly, lx);
-          need += (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
-        }
-        if (j1 > j0 && mvw + need > budget_words) break;
-        mvw += need;
+    base = nullptr;
+    for (const auto& J : act)
+      if (!base || (uintptr_t)J->h->words < (uintptr_t)base) base = J->h->words;
+    std::vector<Item> items[kGroups];
+    for (size_t j = 0; j < act.size(); j++) {
+      const Prepared<T>& P = *act[j]->P;
+      for (int q = 0; q < P.n; q++) {
+        int flip, ly, lx;
+        member_shape(P, q, flip, ly, lx);
+        const int K = (lx + ODBA_W - 1) / ODBA_W;
+        items[group_of(K)].push_back(Item{(double)ly * (double)K, (int)j, q});
       }
-      Batch b;
-      std::vector<Item> items[kGroups];
-      for (size_t j = j0; j < j1; j++) {
-        const Prepared<T>& P = *act[j]->P;
-        for (int q = 0; q < P.n; q++) {
-          int flip, ly, lx;
-          shape(P, q, flip, ly, lx);
-          const int K = (lx + ODBA_W - 1) / ODBA_W;
-          items[group_of(K)].push_back(Item{(double)ly * (double)K, (int)j, q});
-        }
-      }
-      for (int g = 0; g < kGroups; g++) {
-        Group& G = b.g[g];
-        std::stable_sort(items[g].begin(), items[g].end(), [](const Item& a, const Item& z) { return a.cost < z.cost; });
-        std::vector<PJ<T>> pj[2];
-        std::vector<TbJob> tb;
-        long long gw = 0, tw = 0;
-        for (const Item& it : items[g]) {
-          EJob<T>& J = *act[it.j];
-          const Prepared<T>& P = *J.P;
-          int flip, ly, lx;
-          shape(P, it.q, flip, ly, lx);
-          long long mo, to;
-          plan_job(ly, lx, os, oe, gw, tw, mo, to);
-          const T* sq = P.d + P.ho[it.q];
-          for (int v = 0; v < 2; v++) {
-            const T* cp = J.cb[J.cur ^ v];
-            pj[v].push_back(PJ<T>{flip ? cp : sq, flip ? sq : cp, ly, lx, mo, to});
-          }
-          TbJob t;
-          t.ly = ly; t.lx = lx; t.flip = flip; t.seq = it.q; t.mv_off = mo; t.top_off = to; t.path_off = -1;
-          t.runs_off = 0;
-          t.rp = J.runs + (size_t)it.q * P.lc;
-          tb.push_back(t);
-          G.maxly = std::max(G.maxly, ly);
-          G.maxlx = std::max(G.maxlx, lx);
-          G.minly = std::min(G.minly, ly);
-          G.minfirst = G.minfirst && P.no_wrap;
-        }
-        G.nj = (int)tb.size();
-        if (!G.nj) continue;
-        G.msw = mean_swaths(pj[0]);
-        G.tail = path_tail(pj[0]);
-        for (int v = 0; v < 2; v++) G.pj_at[v] = put(pj[v].data(), pj[v].size() * sizeof(PJ<T>));
-        G.tb_at = put(tb.data(), tb.size() * sizeof(TbJob));
-        mvmax[g] = std::max(mvmax[g], (size_t)gw);
-        topmax[g] = std::max(topmax[g], (size_t)tw);
-        njmax[g] = std::max(njmax[g], (size_t)G.nj);
-      }
-      std::vector<MultiSlot> sl[2];
-      long long pos = 0;
-      for (size_t j = j0; j < j1; j++) {
-        EJob<T>& J = *act[j];
-        for (int v = 0; v < 2; v++) {
-          MultiSlot m;
-          m.pos0 = pos; m.data = J.P->d; m.moffs = J.P->o; m.runs = J.runs;
-          m.cen = J.cb[J.cur ^ v]; m.newc = J.cb[J.cur ^ v ^ 1];
-          m.lc = J.P->lc; m.nmem = J.P->n; m.slot = (int)j; m.pad = 0;
-          sl[v].push_back(m);
-        }
-        pos += J.P->lc;
-      }
-      b.ns = (int)(j1 - j0);
-      b.npos = pos;
-      for (int v = 0; v < 2; v++) b.sl_at[v] = put(sl[v].data(), sl[v].size() * sizeof(MultiSlot));
-      posmax = std::max(posmax, (size_t)pos);
-      batches.push_back(b);
-      j0 = j1;
     }
+    for (int g = 0; g < kGroups; g++) {
+      Group& G = grp[g];
+      std::stable_sort(items[g].begin(), items[g].end(), [](const Item& a, const Item& z) { return a.cost < z.cost; });
+      std::vector<PJ<T>> pj[2];
+      std::vector<TbJob> tb;
+      double total = 0, longest = 0;
+      for (const Item& it : items[g]) {
+        total += it.cost;
+        longest = std::max(longest, it.cost);
+        EJob<T>& J = *act[it.j];
+        const Prepared<T>& P = *J.P;
+        int flip, ly, lx;
+        member_shape(P, it.q, flip, ly, lx);
+        const long long at = (long long)(((uintptr_t)J.h->words - (uintptr_t)base) / sizeof(uint32_t));
+        const long long mo = at + J.mo[it.q], to = at + J.to[it.q];
+        const T* sq = P.d + P.ho[it.q];
+        for (int v = 0; v < 2; v++) {
+          const T* cp = J.cb[J.cur ^ v];
+          pj[v].push_back(PJ<T>{flip ? cp : sq, flip ? sq : cp, ly, lx, mo, to});
+        }
+        TbJob t;
+        t.ly = ly; t.lx = lx; t.flip = flip; t.seq = it.q; t.mv_off = mo; t.top_off = to; t.path_off = -1;
+        t.runs_off = 0;
+        t.rp = J.runs + (size_t)it.q * P.lc;
+        tb.push_back(t);
+        G.maxly = std::max(G.maxly, ly);
+        G.maxlx = std::max(G.maxlx, lx);
+        G.minly = std::min(G.minly, ly);
+        G.minfirst = G.minfirst && P.no_wrap;
+      }
+      G.nj = (int)tb.size();
+      if (!G.nj) continue;
+      G.est = std::max(longest, total / (4.0 * std::max(1, c->cus)));
+      G.msw = mean_swaths(pj[0]);
+      G.tail = path_tail(pj[0]);
+      for (int v = 0; v < 2; v++) G.pj_at[v] = put(pj[v].data(), pj[v].size() * sizeof(PJ<T>));
+      G.tb_at = put(tb.data(), tb.size() * sizeof(TbJob));
+      njmax[g] = std::max(njmax[g], (size_t)G.nj);
+    }
+    std::vector<MultiSlot> sl[2];
+    long long pos = 0;
+    for (size_t j = 0; j < act.size(); j++) {
+      EJob<T>& J = *act[j];
+      for (int v = 0; v < 2; v++) {
+        MultiSlot m;
+        m.pos0 = pos; m.data = J.P->d; m.moffs = J.P->o; m.runs = J.runs;
+        m.cen = J.cb[J.cur ^ v]; m.newc = J.cb[J.cur ^ v ^ 1];
+        m.lc = J.P->lc; m.nmem = J.P->n; m.slot = (int)j; m.pad = 0;
+        sl[v].push_back(m);
+      }
+      pos += J.P->lc;
+    }
+    // The group that takes longest starts first: a group launched earlier
+    // takes CUs that a later, longer group then waits for.
+    std::stable_sort(order, order + kGroups, [&](int a, int z) { return grp[a].est > grp[z].est; });
+    ns = (int)act.size();
+    npos = pos;
+    for (int v = 0; v < 2; v++) sl_at[v] = put(sl[v].data(), sl[v].size() * sizeof(MultiSlot));
     int rc;
-    if ((rc = grow(tab, tab_cap, std::max<size_t>(htab.size(), 256))) || (rc = grow(acc, acc_cap, posmax)) ||
-        (rc = grow(counts, counts_cap, posmax)) || (rc = grow(dbits, dbits_cap, act.size())))
+    if ((rc = grow(tab, tab_cap, std::max<size_t>(htab.size(), 256))) ||
+        (rc = grow(acc, acc_cap, std::max<size_t>(pos, 1))) || (rc = grow(counts, counts_cap, std::max<size_t>(pos, 1))) ||
+        (rc = grow(dbits, dbits_cap, act.size())))
       return rc;
     for (int g = 0; g < kGroups; g++)
-      if ((rc = grow(mv[g], mv_cap[g], mvmax[g])) || (rc = grow(mtop[g], mtop_cap[g], topmax[g])) ||
-          (rc = grow(abr[g], abr_cap[g], njmax[g])))
-        return rc;
+      if ((rc = grow(abr[g], abr_cap[g], njmax[g]))) return rc;
     if (!derr) HIPC(c, hipMalloc((void**)&derr, sizeof(int)));
     if (!herr) HIPC(c, hipHostMalloc((void**)&herr, sizeof(int)));
     if (hbits_cap < act.size()) {
@@ -2469,24 +2482,23 @@ template <typename T> struct RoundEngine {
   int launch_round(int v) {
     HIPC(c, launch_fill_u32(c->stream, derr, 0u, 1));
     HIPC(c, launch_fill_u32(c->stream, dbits, 0u, 2 * (long long)act.size()));
-    for (const Batch& b : batches) {
-      HIPC(c, hipEventRecord(ev_go, c->stream));
-      for (int g = 0; g < kGroups; g++) {
-        const Group& G = b.g[g];
-        if (!G.nj) continue;
-        odba_ctx* sc = sub[g];
-        HIPC(c, hipStreamWaitEvent(sc->stream, ev_go, 0));
-        int rc = run_path_jobs(sc, dtype_of<T>(), G.minfirst, os, oe, tab + G.pj_at[v], G.nj, G.maxly, mv[g], mtop[g],
-                               abr[g], G.maxlx, G.minly, derr, G.msw, G.tail, bands_per_cu(g));
-        if (rc) return fail(c, rc, "%s", sc->err.c_str());
-        HIPC(c, launch_traceback(sc->stream, (const TbJob*)(tab + G.tb_at), G.nj, mv[g], mtop[g], abr[g], nullptr, 0, os,
-                                 oe, derr, nullptr));
-        HIPC(c, hipEventRecord(ev_done[g], sc->stream));
-        HIPC(c, hipStreamWaitEvent(c->stream, ev_done[g], 0));
-      }
-      HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, (const MultiSlot*)(tab + b.sl_at[v]), b.ns, b.npos, acc,
-                                counts, dbits, derr));
+    HIPC(c, hipEventRecord(ev_go, c->stream));
+    for (int gi = 0; gi < kGroups; gi++) {
+      const int g = order[gi];
+      const Group& G = grp[g];
+      if (!G.nj) continue;
+      odba_ctx* sc = sub[g];
+      HIPC(c, hipStreamWaitEvent(sc->stream, ev_go, 0));
+      int rc = run_path_jobs(sc, dtype_of<T>(), G.minfirst, os, oe, tab + G.pj_at[v], G.nj, G.maxly, base, base, abr[g],
+                             G.maxlx, G.minly, derr, G.msw, G.tail, bands_per_cu(g));
+      if (rc) return fail(c, rc, "%s", sc->err.c_str());
+      HIPC(c, launch_traceback(sc->stream, (const TbJob*)(tab + G.tb_at), G.nj, base, base, abr[g], nullptr, 0, os, oe,
+                               derr, nullptr));
+      HIPC(c, hipEventRecord(ev_done[g], sc->stream));
+      HIPC(c, hipStreamWaitEvent(c->stream, ev_done[g], 0));
     }
+    HIPC(c, launch_multi_mean(dtype_of<T>(), c->stream, (const MultiSlot*)(tab + sl_at[v]), ns, npos, acc, counts,
+                              dbits, derr));
     HIPC(c, hipMemcpyAsync(herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipMemcpyAsync(hbits, dbits, act.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     for (auto& J : act)
@@ -2519,9 +2531,9 @@ template <typename T> int job_prepare_t(odba_ctx* c, const T* const* seqs, const
   return prepare_t<T>(c, seqs, lens, n, os, oe, norm, *P, nullptr);
 }
 template <typename T> void job_free_t(odba_job* h) { delete (Prepared<T>*)h->prep; }
-template <typename T> int rounds_create_t(odba_ctx* c, int os, int oe, uint64_t word_bytes, void** out) {
+template <typename T> int rounds_create_t(odba_ctx* c, int os, int oe, void** out) {
   auto* e = new RoundEngine<T>();
-  const int rc = e->init(c, os, oe, word_bytes);
+  const int rc = e->init(c, os, oe);
   if (rc) {
     delete e;
     return rc;
@@ -2587,6 +2599,30 @@ int odba_job_result(const odba_job* h, void* centroid_out, uint64_t capacity, ui
   return ODBA_OK;
 }
 
+int odba_job_words(odba_job* h, int allocate, uint64_t* word_bytes) {
+  if (!h || !h->prep) return ODBA_E_INVALID;
+  if (!h->word_bytes) {
+    long long w = 0;
+    switch (h->dtype) {
+      case ODBA_I32: w = job_word_layout(*(Prepared<int32_t>*)h->prep, nullptr, nullptr); break;
+      case ODBA_U32: w = job_word_layout(*(Prepared<uint32_t>*)h->prep, nullptr, nullptr); break;
+      case ODBA_U64: w = job_word_layout(*(Prepared<uint64_t>*)h->prep, nullptr, nullptr); break;
+      case ODBA_F32: w = job_word_layout(*(Prepared<float>*)h->prep, nullptr, nullptr); break;
+      default: w = job_word_layout(*(Prepared<double>*)h->prep, nullptr, nullptr); break;
+    }
+    h->word_bytes = (uint64_t)std::max(1LL, w) * 4;
+  }
+  if (word_bytes) *word_bytes = h->word_bytes;
+  if (allocate && !h->words) {
+    if (hipSetDevice(h->device) != hipSuccess) return ODBA_E_HIP;
+    void* p = nullptr;
+    if (hipMalloc(&p, h->word_bytes) != hipSuccess) return ODBA_E_NOMEM;
+    h->words = (uint32_t*)p;
+    h->dev.push_back(p);
+  }
+  return ODBA_OK;
+}
+
 int odba_job_destroy(odba_job* h) {
   if (!h) return ODBA_OK;
   hipSetDevice(h->device);
@@ -2603,8 +2639,7 @@ int odba_job_destroy(odba_job* h) {
   return ODBA_OK;
 }
 
-int odba_rounds_create(odba_ctx* c, int dtype, int use_open_start, int use_open_end, uint64_t word_bytes,
-                       odba_rounds** out) {
+int odba_rounds_create(odba_ctx* c, int dtype, int use_open_start, int use_open_end, odba_rounds** out) {
   if (!c || !out) return fail(c, ODBA_E_INVALID, "invalid rounds-engine arguments");
   if (dtype_size(dtype) == 0) return fail(c, ODBA_E_INVALID, "unknown dtype %d", dtype);
   auto* r = new odba_rounds();
@@ -2612,11 +2647,11 @@ int odba_rounds_create(odba_ctx* c, int dtype, int use_open_start, int use_open_
   r->dtype = dtype;
   int rc;
   switch (dtype) {
-    case ODBA_I32: rc = rounds_create_t<int32_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
-    case ODBA_U32: rc = rounds_create_t<uint32_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
-    case ODBA_U64: rc = rounds_create_t<uint64_t>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
-    case ODBA_F32: rc = rounds_create_t<float>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
-    default: rc = rounds_create_t<double>(c, use_open_start, use_open_end, word_bytes, &r->eng); break;
+    case ODBA_I32: rc = rounds_create_t<int32_t>(c, use_open_start, use_open_end, &r->eng); break;
+    case ODBA_U32: rc = rounds_create_t<uint32_t>(c, use_open_start, use_open_end, &r->eng); break;
+    case ODBA_U64: rc = rounds_create_t<uint64_t>(c, use_open_start, use_open_end, &r->eng); break;
+    case ODBA_F32: rc = rounds_create_t<float>(c, use_open_start, use_open_end, &r->eng); break;
+    default: rc = rounds_create_t<double>(c, use_open_start, use_open_end, &r->eng); break;
   }
   if (rc) {
     delete r;

@@ -21,7 +21,12 @@
 //
 // Admission compares an estimate of the job's device working set
 // (resident sequences, distances, edge scratch, run records, centroids)
-// with what the engines' step-word budgets leave of the HBM budget.
+// with what the jobs already admitted leave of the HBM budget. A prepared
+// job then counts what it keeps (sequences and run records) and its step
+// words (a buffer of its own for the job's life, sized once the medoid is
+// known), which its preparing slot allocates as soon as they fit the
+// budget (at once when no prepared job holds words), so the engines never
+// allocate or free them on their steps.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -56,6 +61,7 @@ struct Job {
   std::vector<uint64_t> lens;
   int os = 0, oe = 0, norm = 1, max_rounds = 250;
   uint64_t footprint = 0;
+  bool worded = false;  // step words counted in the server's in_use
   int state = ODBA_JOB_QUEUED;
   int rc = 0;
   std::string err;
@@ -73,7 +79,6 @@ struct Engine {
   std::thread th;
   std::deque<Job*> pending;   // prepared, not yet added
   int held = 0;               // jobs added and not yet returned
-  uint64_t words = 0;         // step-word budget (bytes)
 };
 
 }  // namespace
@@ -94,7 +99,7 @@ struct odba_server {
   std::deque<odba_job*> reap;
   std::thread reaper;
   uint64_t in_use = 0;
-  uint64_t reserved = 0;  // the engines' step-word budgets
+  int worded = 0;         // jobs whose step words are counted in in_use
   int running = 0;
   bool stop = false;
   uint64_t next_id = 1;
@@ -105,9 +110,11 @@ struct odba_server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // Device bytes one job holds while it runs, apart from step words (the
-  // engines' budgets; with whole-job slots, the slot's share).
-  uint64_t footprint(const Job& j) const {
+  // Device bytes one job holds while it is prepared or runs, apart from
+  // step words (a prepared job's own buffer; with whole-job slots, the
+  // slot's share). With `held`, what a prepared job keeps: its sequences
+  // and run records.
+  uint64_t footprint(const Job& j, bool held = false) const {
     const size_t es = elem_size(j.dtype);
     const long long n = (long long)j.lens.size();
     long long seq = 0, maxlen = 0, sumlen = 0;
@@ -126,6 +133,7 @@ struct odba_server {
       words = (uint64_t)sumlen * (uint64_t)maxlen / 4 + (uint64_t)n * 64 * 4 * (maxlen / 1024 + 1);
       words = std::min(words, budget / (uint64_t)slots / 4);
     }
+    if (held) return (uint64_t)seq * es * 2 + runs;
     return (uint64_t)seq * es * 2 + dists + edge + runs + words;
   }
 
@@ -134,7 +142,7 @@ struct odba_server {
   bool can_admit() const {
     if (queue.empty() || (packed ? prep_slots_busy >= slots : running >= slots)) return false;
     const Job* j = queue.front();
-    return running == 0 || in_use + reserved + j->footprint <= budget;
+    return running == 0 || in_use + j->footprint <= budget;
   }
 
   static int class_key(const Job& j) { return j.dtype * 4 + (j.os ? 2 : 0) + (j.oe ? 1 : 0); }
@@ -142,6 +150,8 @@ struct odba_server {
   // Records a finished job's outcome (lock held).
   void complete(Job* j, int rc, const std::string& msg) {
     in_use -= j->footprint;
+    if (j->worded) worded--;
+    j->worded = false;
     j->rc = rc;
     if (rc) j->err = msg;
     j->data.clear();
@@ -212,14 +222,29 @@ struct odba_server {
       std::vector<const void*> ptrs(n);
       for (int i = 0; i < n; i++) ptrs[i] = j->data[i].data();
       odba_job* h = nullptr;
-      const int rc = odba_job_prepare(c, j->dtype, ptrs.data(), j->lens.data(), n, j->os, j->oe, j->norm,
-                                      j->max_rounds, &h);
+      int rc = odba_job_prepare(c, j->dtype, ptrs.data(), j->lens.data(), n, j->os, j->oe, j->norm, j->max_rounds,
+                                &h);
       std::string msg = rc ? odba_last_error(c) : "";
+      uint64_t wb = 0;
+      if (!rc && (rc = odba_job_words(h, 0, &wb))) msg = "cannot size the job's step words";
+      if (!rc) {
+        std::unique_lock<std::mutex> lk(mu);
+        in_use -= j->footprint;
+        j->footprint = footprint(*j, true);
+        in_use += j->footprint;
+        cv_admit.wait(lk, [&] { return stop || worded == 0 || in_use + wb <= budget; });
+        in_use += wb;
+        j->footprint += wb;
+        j->worded = true;
+        worded++;
+      }
+      if (!rc && (rc = odba_job_words(h, 1, &wb))) msg = "cannot allocate the job's step words";
       {
         std::lock_guard<std::mutex> lk(mu);
         prep_slots_busy--;
         if (rc) {
           running--;
+          if (h) reap.push_back(h);
           complete(j, rc, msg);
         } else {
           j->handle = h;
@@ -255,13 +280,11 @@ struct odba_server {
       return nullptr;
     }
     odba_ctx_set_graphs(e->ctx, 0);
-    e->words = budget / 4;
-    if (odba_rounds_create(e->ctx, j.dtype, j.os, j.oe, e->words, &e->eng)) {
+    if (odba_rounds_create(e->ctx, j.dtype, j.os, j.oe, &e->eng)) {
       err = odba_last_error(e->ctx);
       odba_ctx_destroy(e->ctx);
       return nullptr;
     }
-    reserved += e->words;
     Engine* ep = e.get();
     e->th = std::thread([this, ep] { engine_loop(ep); });
     engines[key] = std::move(e);

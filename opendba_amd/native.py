@@ -27,6 +27,7 @@ EXPORTS = [
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
     "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
     "odba_ctx_set_graphs", "odba_dtw_all_pairs_noalloc", "odba_job_prepare", "odba_job_result", "odba_job_destroy",
+    "odba_job_words",
     "odba_rounds_create", "odba_rounds_add", "odba_rounds_step", "odba_rounds_destroy",
 ]
 SCHED_WAVE, SCHED_BANDS = 0, 1
@@ -105,7 +106,8 @@ def load():
     L.odba_job_prepare.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, C.POINTER(vp)]
     L.odba_job_result.argtypes = [vp, vp, u64, C.POINTER(u64), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.odba_job_destroy.argtypes = [vp]
-    L.odba_rounds_create.argtypes = [vp, i32, i32, i32, u64, C.POINTER(vp)]
+    L.odba_job_words.argtypes = [vp, i32, C.POINTER(u64)]
+    L.odba_rounds_create.argtypes = [vp, i32, i32, i32, C.POINTER(vp)]
     L.odba_rounds_add.argtypes = [vp, vp]
     L.odba_rounds_step.argtypes = [vp, vp, i32, C.POINTER(i32), C.POINTER(i32)]
     L.odba_rounds_destroy.argtypes = [vp]
@@ -441,6 +443,15 @@ class Job:
             raise OdbaError(rc, "odba_job_result failed")
         return dict(centroid=cen[: clen.value], medoid=med.value, rounds=rounds.value, rc=jrc.value)
 
+    def words(self, allocate=False):
+        """Bytes of the job's step words (odba_job_words); with allocate, the
+        buffer is allocated now."""
+        wb = C.c_uint64()
+        rc = self.L.odba_job_words(self.h, 1 if allocate else 0, C.byref(wb))
+        if rc:
+            raise OdbaError(rc, "odba_job_words failed")
+        return wb.value
+
     def close(self):
         if self.h:
             self.L.odba_job_destroy(self.h)
@@ -451,12 +462,11 @@ class Rounds:
     """Packed DBA rounds over prepared jobs of one value type and mode
     (odba_rounds_*): each step runs one round of every job held."""
 
-    def __init__(self, ctx, dtype, open_start=0, open_end=0, word_bytes=1 << 34):
+    def __init__(self, ctx, dtype, open_start=0, open_end=0):
         self.L = ctx.L
         self.ctx = ctx
         self.h = C.c_void_p()
-        ctx._check(self.L.odba_rounds_create(ctx.h, CODE[np.dtype(dtype)], open_start, open_end, word_bytes,
-                                             C.byref(self.h)))
+        ctx._check(self.L.odba_rounds_create(ctx.h, CODE[np.dtype(dtype)], open_start, open_end, C.byref(self.h)))
 
     def add(self, job):
         self.ctx._check(self.L.odba_rounds_add(self.h, job.h))

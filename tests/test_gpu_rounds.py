@@ -2,7 +2,7 @@
 the server): every job's result - centroid bits, medoid, rounds - equals
 the plain per-job path (odba_perform_dba) and, for small jobs, the oracle,
 whatever jobs it is packed with, at whichever step it joins and however the
-step words are split into batches."""
+step words were allocated (by the preparing side, or on joining)."""
 import numpy as np
 import pytest
 
@@ -41,15 +41,19 @@ def _same(a, b):
     assert np.array_equal(_bits(a["centroid"]), _bits(b["centroid"]))
 
 
-def _run_packed(ctx, ectx, jobs, os_, oe, max_rounds, word_bytes, join_every=2, norm=1):
+def _run_packed(ctx, ectx, jobs, os_, oe, max_rounds, prealloc=False, join_every=2, norm=1):
     """Prepares every job on ctx, then adds them to one engine on ectx two
-    steps apart (continuous batching) and steps until all have ended."""
+    steps apart (continuous batching) and steps until all have ended. With
+    prealloc, every other job's step words are allocated before it joins."""
     dtype = jobs[0][0].dtype
     prepared = [N.Job(ctx, s, os_, oe, norm, max_rounds) for s in jobs]
+    if prealloc:
+        for p in prepared[::2]:
+            assert p.words(allocate=True) > 0
     by_handle = {p.h.value: k for k, p in enumerate(prepared)}
     res = [None] * len(jobs)
     try:
-        with N.Rounds(ectx, dtype, os_, oe, word_bytes) as R:
+        with N.Rounds(ectx, dtype, os_, oe) as R:
             waiting = list(range(len(jobs)))
             step, held = 0, 0
             while waiting or held:
@@ -69,13 +73,13 @@ def _run_packed(ctx, ectx, jobs, os_, oe, max_rounds, word_bytes, join_every=2, 
 
 
 @pytest.mark.parametrize("oe", [0, 1])
-@pytest.mark.parametrize("word_bytes", [1 << 34, 1 << 16])
-def test_packed_rounds_equal_plain_path(oe, word_bytes):
+@pytest.mark.parametrize("prealloc", [False, True])
+def test_packed_rounds_equal_plain_path(oe, prealloc):
     jobs = _jobs(seed=7 + oe, n_jobs=6)
-    print(f"packed rounds: oe={oe} word_bytes={word_bytes}", flush=True)
+    print(f"packed rounds: oe={oe} prealloc={prealloc}", flush=True)
     with N.Context(0) as ctx, N.Context(0) as ectx:
         plain = [ctx.perform_dba(s, 0, oe, norm=1, max_rounds=25) for s in jobs]
-        packed = _run_packed(ctx, ectx, jobs, 0, oe, 25, word_bytes)
+        packed = _run_packed(ctx, ectx, jobs, 0, oe, 25, prealloc)
     for p, q in zip(plain, packed):
         _same(p, q)
 
@@ -86,7 +90,7 @@ def test_packed_rounds_open_start_modes(os_, oe):
     print(f"packed rounds: os={os_} oe={oe}", flush=True)
     with N.Context(0) as ctx, N.Context(0) as ectx:
         plain = [ctx.perform_dba(s, os_, oe, norm=1, max_rounds=20) for s in jobs]
-        packed = _run_packed(ctx, ectx, jobs, os_, oe, 20, 1 << 34, join_every=1)
+        packed = _run_packed(ctx, ectx, jobs, os_, oe, 20, False, join_every=1)
     for p, q in zip(plain, packed):
         _same(p, q)
 
@@ -98,7 +102,7 @@ def test_packed_rounds_other_value_types(dtype):
     norm = 0 if np.dtype(dtype).kind == "i" else 1
     with N.Context(0) as ctx, N.Context(0) as ectx:
         plain = [ctx.perform_dba(s, 0, 0, norm=norm, max_rounds=15) for s in jobs]
-        packed = _run_packed(ctx, ectx, jobs, 0, 0, 15, 1 << 34, norm=norm)
+        packed = _run_packed(ctx, ectx, jobs, 0, 0, 15, False, norm=norm)
     for p, q in zip(plain, packed):
         _same(p, q)
 
@@ -108,7 +112,7 @@ def test_packed_rounds_equal_oracle(oe):
     jobs = _jobs(seed=41 + oe, n_jobs=3, lmin=100, lmax=900, nmax=6)
     print(f"packed rounds vs oracle: oe={oe}", flush=True)
     with N.Context(0) as ctx, N.Context(0) as ectx:
-        packed = _run_packed(ctx, ectx, jobs, 0, oe, 12, 1 << 34, join_every=1)
+        packed = _run_packed(ctx, ectx, jobs, 0, oe, 12, False, join_every=1)
     for s, q in zip(jobs, packed):
         exp = O.perform_dba(s, 0, oe, norm=1, max_rounds=12)
         _same(dict(medoid=exp["medoid"], rounds=exp["rounds"], centroid=exp["centroid"]), q)
@@ -151,6 +155,40 @@ def test_packed_rounds_long_jobs_take_swath_bands(oe):
     print(f"packed rounds with long jobs oe={oe}", flush=True)
     with N.Context(0) as ctx, N.Context(0) as ectx:
         plain = [ctx.perform_dba(s, 0, oe, norm=1, max_rounds=4) for s in jobs]
-        packed = _run_packed(ctx, ectx, jobs, 0, oe, 4, 1 << 34, join_every=1)
+        packed = _run_packed(ctx, ectx, jobs, 0, oe, 4, False, join_every=1)
     for p, q in zip(plain, packed):
         _same(p, q)
+
+
+def _swath_words(ly, os_, oe):
+    """Words of one 1024-column swath: 64 lanes x the rows of its skewed main
+    loop rounded up to a multiple of 64 (the last row of an open-end swath
+    and the first of an open-start one are not stored)."""
+    nr = (ly - 2 if (oe and ly >= 2) else ly - 1) - (1 if os_ else 0) + 1
+    return (nr + 63) * 64 if nr > 0 else 0
+
+
+@pytest.mark.parametrize("os_,oe", [(0, 0), (0, 1), (1, 1)])
+def test_job_words_size(os_, oe):
+    """odba_job_words: every member's path words (flipped onto the columns
+    in open end when the centroid is shorter) plus 64 top-row words per
+    swath, 4 bytes each; allocating keeps the size and a second call is a
+    no-op."""
+    jobs = _jobs(seed=71 + oe, n_jobs=2, lmin=100, lmax=5000)
+    with N.Context(0) as ctx:
+        for s in jobs:
+            lc = len(ctx.perform_dba(s, os_, oe, norm=1, max_rounds=1)["centroid"])
+            want = 0
+            for x in s:
+                L = len(x)
+                flip = oe and lc < L
+                ly, lx = (lc, L) if flip else (L, lc)
+                K = (lx + 1023) // 1024
+                want += K * (_swath_words(ly, os_, oe) + 64) * 4
+            p = N.Job(ctx, s, os_, oe, 1, 5)
+            try:
+                assert p.words() == want
+                assert p.words(allocate=True) == want
+                assert p.words(allocate=True) == want
+            finally:
+                p.close()
