@@ -1130,6 +1130,16 @@ template <typename T> struct DbaState {
   size_t ck_jobs_cap = 0;
   unsigned char* ck_states = nullptr;
   size_t ck_states_cap = 0;
+  // Checkpointed stripes in the open modes (float): whole edge columns,
+  // per-swath (top, column min), band-start minima, job records.
+  float* sck_slots = nullptr;
+  size_t sck_slots_cap = 0;
+  float2* sck_tc = nullptr;
+  size_t sck_tc_cap = 0;
+  float* sck_pm = nullptr;
+  size_t sck_pm_cap = 0;
+  SckJob* sck_jobs = nullptr;
+  size_t sck_jobs_cap = 0;
   // Traceback recording (the cells writeDTWPath prints): per sequence a
   // count slot then up to |seq| + |centroid| packed steps.
   bool rec = false;
@@ -1168,7 +1178,7 @@ template <typename T> struct DbaState {
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
                     (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf,
                     (void*)ck_rowck, (void*)ck_colck, (void*)ck_cnt, (void*)ck_mv, (void*)ck_slots, (void*)ck_rings,
-                    (void*)ck_jobs, (void*)ck_states})
+                    (void*)ck_jobs, (void*)ck_states, (void*)sck_slots, (void*)sck_tc, (void*)sck_pm, (void*)sck_jobs})
       if (p) hipFree(p);
   }
 
@@ -1369,6 +1379,99 @@ template <typename T> struct DbaState {
     }
   }
 
+  // In the open modes the stripe path depends on each stripe's truncated
+  // height (the open-end top-row rule at the recompute's last row), so float
+  // jobs there keep the reference's stripe semantics but recompute only the
+  // band of rows holding the walk's current cell, from cost rows stored by a
+  // forward pass over the whole device (k_sck_forward); ODBA_CKPT=0 keeps
+  // the recompute from row 0, ODBA_CKPT_RSHIFT sets the band height 2^k.
+  bool use_sck() const {
+    const char* e = std::getenv("ODBA_CKPT");
+    return std::is_same<T, float>::value && (os || oe) && !(e && !std::strcmp(e, "0"));
+  }
+  static int ck_rshift() { return std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 14))); }
+  // Device 4-byte units of one checkpointed open-mode stripe job.
+  static long long sck_need(int H, int LX, int rshift) {
+    const long long R = 1LL << rshift, K = (LX + ODBA_W - 1) / ODBA_W, nb = (H + R - 1) / R;
+    return K * ((long long)H + 256) + (nb - 1) * (long long)LX + 2 * K + K * nb + K + (R + 127) * 64 + 64;
+  }
+
+  int run_sck_batch(const std::vector<int>& idx, int b0) {
+    if constexpr (!std::is_same<T, float>::value) {
+      return fail(c, ODBA_E_UNSUPPORTED, "checkpointed stripe paths are float only");
+    } else {
+      const int nj = (int)idx.size();
+      if (!nj) return ODBA_OK;
+      const int rshift = ck_rshift();
+      const long long R = 1LL << rshift;
+      const long long job_words = (R + 127) * 64;  // a block of up to R + 1 rows
+      std::vector<SckJob> sj(nj);
+      std::vector<TbJob> tb(nj);
+      long long nslot = 0, nrow = 0, ntc = 0, npm = 0, ncnt = 0, nclaims = 0, words = 0;
+      int kmax = 1, maxit = 0;
+      for (int q = 0; q < nj; q++) {
+        const int b = idx[q];
+        const int ls = h_lens[b];
+        const int flip = (oe && lc < ls) ? 1 : 0;
+        const int H = flip ? lc : ls, LX = flip ? ls : lc;
+        const int K = (LX + ODBA_W - 1) / ODBA_W;
+        const long long nb = (H + R - 1) / R;
+        const T* sq = seqdata + h_offs[b];
+        sj[q] = SckJob{flip ? cen : sq, flip ? sq : cen, H, LX, nslot, nrow, ntc, npm, ncnt, nclaims, words};
+        tb[q].ly = H; tb[q].lx = LX; tb[q].flip = flip; tb[q].seq = b - b0; tb[q].mv_off = words;
+        tb[q].top_off = (long long)q * 64;
+        tb[q].path_off = rec ? h_poff[b] : -1;
+        tb[q].runs_off = (long long)(b - b0) * lc;
+        tb[q].rp = nullptr;
+        nslot += (long long)K * ((long long)H + 256);
+        nrow += (nb - 1) * (long long)LX;
+        ntc += K;
+        npm += (long long)K * nb;
+        ncnt += K;
+        nclaims += (K + kSckBandSwaths - 1) / kSckBandSwaths;
+        words += job_words;
+        kmax = std::max(kmax, K);
+        maxit = std::max<int>(maxit, K + (int)nb + 1);  // each step leaves a stripe or a band, or ends
+      }
+      int rc;
+      if ((rc = grow(c, sck_slots, sck_slots_cap, (size_t)std::max(1LL, nslot))) ||
+          (rc = grow(c, ck_rowck, ck_rowck_cap, (size_t)std::max(1LL, nrow))) ||
+          (rc = grow(c, sck_tc, sck_tc_cap, (size_t)ntc)) || (rc = grow(c, sck_pm, sck_pm_cap, (size_t)npm)) ||
+          (rc = grow(c, ck_cnt, ck_cnt_cap, (size_t)ncnt)) || (rc = grow(c, smv, smv_cap, (size_t)words)) ||
+          (rc = grow(c, stop, stop_cap, (size_t)nj * 64)) || (rc = grow(c, sstate, sstate_cap, (size_t)nj)) ||
+          (rc = grow(c, sck_jobs, sck_jobs_cap, (size_t)nj)) || (rc = grow(c, stbj, stbj_cap, (size_t)nj)))
+        return rc;
+      HIPC(c, hipMemcpyAsync(sck_jobs, sj.data(), nj * sizeof(SckJob), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(stbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
+      HIPC(c, hipMemsetAsync(ck_cnt, 0, (size_t)ncnt * sizeof(int), c->stream));
+      unsigned long long* counter = (unsigned long long*)c->small;
+      int* ferr = (int*)((char*)c->small + 192);
+      HIPC(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+      HIPC(c, hipMemsetAsync(ferr, 0, sizeof(int), c->stream));
+      int khz = 0;
+      if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
+      const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;
+      const int nblocks = (int)std::max<long long>(1, std::min<long long>(nclaims, c->cus));
+      HIPC(c, launch_sck_forward(os, oe, nblocks, c->stream, sck_jobs, nj, nclaims, counter, sck_slots, ck_rowck, sck_tc,
+                                 ck_cnt, rshift, ferr, limit));
+      HIPC(c, launch_sck_init(os, oe, c->stream, sck_jobs, nj, kmax, sck_slots, sck_tc, sck_pm, sstate, rshift));
+      for (int it = 0; it < maxit; it++) {
+        HIPC(c, launch_sck_step(os, oe, c->stream, sck_jobs, nj, sck_slots, ck_rowck, sck_pm, smv, stop, sstate, rshift));
+        HIPC(c, launch_sck_traceback(c->stream, stbj, nj, smv, stop, sstate, runs, os, oe, err, rec ? pbuf : nullptr));
+      }
+      std::vector<StripeState> sts(nj);
+      int herr = 0;
+      HIPC(c, hipMemcpyAsync(&herr, ferr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipMemcpyAsync(sts.data(), sstate, nj * sizeof(StripeState), hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      if (herr & 8) return fail(c, ODBA_E_HIP, "checkpointed stripe forward pass: edge hand-off timed out");
+      for (int q = 0; q < nj; q++)
+        if (!sts[q].done && !sts[q].err)
+          return fail(c, ODBA_E_TRACEBACK, "checkpointed stripe traceback of sequence %d did not finish", idx[q]);
+      return ODBA_OK;
+    }
+  }
+
   int run_stripe_batch(const std::vector<int>& idx, int b0) {
     const int nj = (int)idx.size();
     if (!nj) return ODBA_OK;
@@ -1479,6 +1582,7 @@ template <typename T> struct DbaState {
         const bool st = use_stripe(b1);
         long long need = st ? ((long long)((lx + ODBA_W - 1) / ODBA_W) * (ly + 256) * (long long)sizeof(T) + swath_words(ly, os, oe) * 4) / 4
                             : (long long)((lx + ODBA_W - 1) / ODBA_W) * swath_words(ly, os, oe);
+        if (st && use_sck()) need = sck_need(ly, lx, ck_rshift());
         if (st && use_ckpt()) {  // checkpoint rows and columns plus one block's words (4-byte units)
           const long long R = 1LL << std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 14)));
           const long long E = (long long)(env_int("ODBA_CKPT_EW", 16) == 2 ? 2 : 16) * ODBA_W;
@@ -1526,7 +1630,8 @@ template <typename T> struct DbaState {
         if (rc) return rc;
         HIPC(c, launch_traceback(c->stream, tbj, nfull, mv, mtop, abr, runs, lc, os, oe, err, rec ? pbuf : nullptr));
       }
-      rc = use_ckpt() ? run_ckpt_batch(stripe_idx, b0) : run_stripe_batch(stripe_idx, b0);
+      rc = use_ckpt() ? run_ckpt_batch(stripe_idx, b0)
+                      : (use_sck() ? run_sck_batch(stripe_idx, b0) : run_stripe_batch(stripe_idx, b0));
       if (rc) return rc;
       HIPC(c, launch_accumulate(dtype_of<T>(), c->stream, seqdata, offs + b0, runs, b1 - b0, lc, acc, counts, first ? 1 : 0));
       // The host vectors must outlive the async copies: a round of one batch

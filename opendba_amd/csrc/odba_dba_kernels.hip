@@ -593,7 +593,11 @@ hipError_t launch_ckpt_trace(hipStream_t st, const void* jobs, int njobs, void* 
 // with column_offset / stripe_rows, dba.hpp:374-423 launched at 733), one
 // wavefront per job: walk from the carried cell until the path leaves the
 // stripe on the left (the new height is the row reached + 1) or reaches the
-// (0,0) anchor.
+// (0,0) anchor. BLK (checkpointed stripes): the stripe is the one holding
+// the carried cell, its step words cover rows br .. of the block last
+// recomputed, and a walk that goes below row br stops there (the next step
+// recomputes the band below).
+template <bool BLK>
 __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restrict__ jobs, int njobs, int m,
                                                           const uint32_t* __restrict__ mv,
                                                           const uint32_t* __restrict__ mtop,
@@ -606,13 +610,16 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
   const TbJob J = jobs[jid];
   StripeState S = ss[jid];
   const int K = (J.lx + ODBA_W - 1) / ODBA_W;
-  const int k = K - 1 - m;
-  if (k < 0 || S.done || S.err) return;
+  if (S.done || S.err || (BLK && S.j < 0)) return;
+  const int k = BLK ? wave_uniform(S.j) / ODBA_W : K - 1 - m;
+  if (k < 0) return;
   const int left = k * ODBA_W;
   const int h = S.h;
   const int wl = J.lx - left;
-  const int C = (k == K - 1) ? (wl <= 64 ? 1 : wl <= 128 ? 2 : wl <= 256 ? 4 : wl <= 512 ? 8 : wl <= 768 ? 12 : 16) : 16;
-  const int r0 = os ? 1 : 0;
+  // BLK: every stripe's words have the whole-swath layout (16 columns per lane)
+  const int C = (k == K - 1 && !BLK) ? (wl <= 64 ? 1 : wl <= 128 ? 2 : wl <= 256 ? 4 : wl <= 512 ? 8 : wl <= 768 ? 12 : 16) : 16;
+  const int br = (BLK && !S.sab) ? wave_uniform(S.br) : 0;  // the step words' first row
+  const int r0 = (os && br == 0) ? 1 : 0;
   const uint32_t* M = mv + J.mv_off;
   const uint32_t* TP = mtop + J.top_off;
   int4* R = J.rp ? J.rp : runs + J.runs_off;
@@ -633,7 +640,7 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
     l = (j - left) / C;
     q = (j - left) - l * C;
   }
-  while (j >= left) {
+  while (j >= left && i >= br) {
     uint8_t code;
     if (i == 0 && j == 0) code = os ? ODBA_NIL_OPEN_RIGHT : ODBA_NIL;
     else if (S.sab && i == h - 1) code = ODBA_OPEN_RIGHT;
@@ -643,7 +650,7 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
         const uint32_t c = ((uint32_t)__builtin_amdgcn_readlane((int)topw, l) >> (2 * q)) & 3u;
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_OPEN_RIGHT);
       } else {
-        const uint32_t c = plane_code(tile.get(M, 0, r0, 0, i, l, lane), q);
+        const uint32_t c = plane_code(tile.get(M, 0, r0, 0, i - br, l, lane), q);
         code = c == 0 ? ODBA_DIAGONAL : (c == 2 ? ODBA_UP : ODBA_RIGHT);
       }
     }
@@ -674,7 +681,7 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
     if (lane == 0) R[S.curc] = rr.out();
     S.done = 1;
     if (P && lane == 0) P[0] = (unsigned long long)S.npath;
-  } else if (!S.err) {
+  } else if (!S.err && j < left) {  // (BLK: else below the block, still in the stripe)
     if (j != left - 1) { S.err = 1; e |= 1; }
     S.h = i + 1;
   }
@@ -1187,8 +1194,17 @@ hipError_t launch_stripe_traceback(hipStream_t st, const TbJob* jobs, int njobs,
                                    const uint32_t* mtop, StripeState* ss, int4* runs, int lc, int os, int oe,
                                    int* err, unsigned long long* pbuf) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stripe_traceback, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
+  hipLaunchKernelGGL(k_stripe_traceback<false>, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, m, mv, mtop, ss, runs,
                      lc, os, oe, err, pbuf);
+  return hipGetLastError();
+}
+
+hipError_t launch_sck_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv,
+                                const uint32_t* mtop, StripeState* ss, int4* runs, int os, int oe, int* err,
+                                unsigned long long* pbuf) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stripe_traceback<true>, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs, 0, mv, mtop, ss,
+                     runs, 0, os, oe, err, pbuf);
   return hipGetLastError();
 }
 

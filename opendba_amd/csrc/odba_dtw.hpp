@@ -103,6 +103,25 @@ struct StripeState {
   int fab;       // swath where the forward pass abandoned the search, -1 none
   int sab;       // 1: this stripe's top row is all OPEN_RIGHT
   int done, err;
+  int br;        // checkpointed stripes: first row of the block last recomputed
+};
+
+// Swaths per claim of the checkpointed open-mode stripe forward pass (one
+// wave each).
+constexpr int kSckBandSwaths = 16;
+
+// Checkpointed stripe paths in the open modes (float): one record per job.
+// slot_off: K-1 edge columns of H + 256 floats (every swath's right edge, as
+// the reference's dtwCostSoFar); rowck_off: nb-1 checkpoint rows of LX
+// floats (nb = row bands of 2^rshift); tc_off: K (top, column min) pairs of
+// the forward pass; pm_off: K*nb prefix minima of the edge columns at the
+// band starts; cnt_off / claim0: progress words and first claim of the
+// job's column blocks of 16 swaths; mv_off: words of one recomputed block.
+struct SckJob {
+  const float* y;
+  const float* x;
+  int H, LX;
+  long long slot_off, rowck_off, tc_off, pm_off, cnt_off, claim0, mv_off;
 };
 
 // A few very long pairs: bands of `waves` strips (64*C columns each) claimed
@@ -153,6 +172,21 @@ hipError_t launch_ckpt_trace(hipStream_t st, const void* jobs, int njobs, void* 
                              long long job_words, int4* runs, int rshift, int ew, unsigned long long* pbuf);
 size_t ckpt_job_size();
 size_t ckpt_state_size();
+
+// Checkpointed stripe paths in the open modes (float; see k_sck_forward):
+// the forward pass over the whole device, then per step the block of the
+// path's current cell and its traceback, until every path is done.
+hipError_t launch_sck_forward(int os, int oe, int nblocks, hipStream_t st, const SckJob* jobs, int njobs,
+                              long long nclaims, unsigned long long* counter, float* slots, float* rowck, float2* tc,
+                              int* cnt, int rshift, int* err, long long tick_limit);
+hipError_t launch_sck_init(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, int kmax, const float* slots,
+                           const float2* tc, float* pm, StripeState* ss, int rshift);
+hipError_t launch_sck_step(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, const float* slots,
+                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, StripeState* ss,
+                           int rshift);
+hipError_t launch_sck_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv,
+                                const uint32_t* mtop, StripeState* ss, int4* runs, int os, int oe, int* err,
+                                unsigned long long* pbuf);
 
 hipError_t launch_stripe(int dtype, bool minfirst, bool forward, int os, int oe, int nblocks, hipStream_t st,
                          const void* jobs, int njobs, int m, unsigned int* counter, void* slots, uint32_t* mv,

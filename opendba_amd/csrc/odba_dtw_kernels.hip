@@ -2098,7 +2098,7 @@ __global__ void __launch_bounds__(256) k_stripe_forward(const StripeJob<T>* __re
     if (lane == 0) {
       StripeState z;
       z.i = jb.H - 1; z.j = jb.LX - 1; z.h = jb.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0; z.npath = 0;
-      z.fab = ab; z.sab = 0; z.done = 0; z.err = 0;
+      z.fab = ab; z.sab = 0; z.done = 0; z.err = 0; z.br = 0;
       st[j] = z;
     }
   }
@@ -2152,6 +2152,219 @@ __global__ void __launch_bounds__(256) k_stripe_step(const StripeJob<T>* __restr
     (void)swath_dispatch<T, OS, OE, MINFIRST, true>(C, io, &win, lane);
   }
   if (lane == 0) st[j].sab = sab;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpointed stripe paths in the open modes (float). The reference's stripe
+// mode (dba.hpp:646-763) recomputes each 1024-column stripe from row 0 up to
+// the current traceback row h-1, the open-end top-row rule applied at row
+// h-1 (the recompute's last row), and walks the stripe from its top-right
+// cell until the path leaves it on the left. The rows below h-1 of such a
+// recompute are the forward pass's rows (same recurrence, same left edge
+// column), so only the band of rows that holds the walk's current cell needs
+// recomputing, from the cost row stored below it:
+//   * k_sck_forward: the whole matrix once, as column blocks of 16 swaths
+//     claimed over the device, every swath's right edge column kept whole
+//     (the reference's dtwCostSoFar; each swath waits on its left
+//     neighbour's progress word), the cost row at the bottom of every band of
+//     R rows, and each swath's (top, column min) for the forward pass's
+//     open-end stop (dtw.hpp:74-129).
+//   * k_sck_init: the edge columns' minima over every band start, the stop
+//     swath of the forward pass and the walk states.
+//   * k_sck_step: per job, the stop test at the truncated height when the
+//     walk enters a stripe, else the block of rows bR .. i of the stripe
+//     (bR the band start at or below the current row i) from the stored row
+//     above it, the top-row rule only when i is the stripe's top row.
+//   * k_sck_traceback: the stripe walk through that block, until the path
+//     leaves the stripe (the next stripe's height is the row reached + 1) or
+//     the block (the next step recomputes the band below).
+// ---------------------------------------------------------------------------
+template <bool OS, bool OE>
+__global__ void __launch_bounds__(1024) k_sck_forward(const SckJob* __restrict__ jobs, int njobs, long long nclaims,
+                                                      unsigned long long* __restrict__ counter,
+                                                      float* __restrict__ slots, float* __restrict__ rowck,
+                                                      float2* __restrict__ tc, int* __restrict__ cnt, int rshift,
+                                                      int* __restrict__ err, long long limit) {
+  constexpr int EW = kSckBandSwaths;
+  static_assert(EW <= kMaxStripWaves, "one wave per swath of a claim");
+  __shared__ float s_y[EW][128];
+  __shared__ float s_e[EW][64];
+  __shared__ float s_o[EW][128];
+  __shared__ unsigned long long s_q;
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const WaveWin<float> win{s_y[wid], s_e[wid], s_o[wid]};
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous claim
+    if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const long long q = (long long)uniform_u64(s_q);
+    if (q >= nclaims) break;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].claim0 <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const SckJob J = jobs[uniform_i32(lo)];
+    const int m = uniform_i32((int)(q - J.claim0));
+    const int S = (J.LX + ODBA_W - 1) / ODBA_W;
+    const int s = m * EW + wid;
+    if (s >= S) continue;
+    const int off = s * ODBA_W;
+    const bool has_out = s < S - 1;
+    const long long sstride = (long long)J.H + 256;
+    // Every hand-off, inside the block of 16 too, goes through the swath's
+    // whole edge column and its global progress word: each wait targets the
+    // left neighbour, a lower wave of this workgroup or a lower claim.
+    BandPipe pp;
+    pp.in_global = s > 0;
+    pp.in_cnt = s > 0 ? &cnt[J.cnt_off + s - 1] : nullptr;
+    pp.in_used = nullptr;
+    pp.out_cnt = has_out ? &cnt[J.cnt_off + s] : nullptr;
+    pp.out_global = has_out;
+    pp.out_used = nullptr;
+    pp.ring = 0;
+    pp.main_end = (OE && J.H >= 2) ? J.H - 1 : J.H;
+    pp.lane = lane;
+    pp.err = err;
+    pp.limit = limit;
+    SwathIO<float> io;
+    io.y = J.y; io.ly = J.H; io.x = J.x; io.lx = J.LX; io.off = off;
+    io.ein = s == 0 ? nullptr : slots + J.slot_off + (long long)(s - 1) * sstride;
+    io.eout = has_out ? slots + J.slot_off + (long long)s * sstride : nullptr;
+    io.mv = nullptr; io.mtop = nullptr;
+    io.ref_left = 1;
+    io.rowck = rowck + J.rowck_off;
+    io.ck_stride = J.LX;
+    io.ck_rbase = 0;
+    io.ck_rshift = rshift;
+    const int nb = (J.H + (1 << rshift) - 1) >> rshift;
+    io.ck_rlimit = (nb - 1) << rshift;
+    // Every swath with 16 columns per lane, the job's last one too: its
+    // columns past LX read the sequences' padding (1,024 values at least)
+    // and influence no column before them; nothing reads them back (no
+    // edge column, checkpoint rows stop at LX). One form of the sweep keeps
+    // it inline.
+    const SwathRes<float> r = run_swath<float, 16, OS, OE, true, false, BandPipe, true>(io, &win, lane, pp);
+    if (has_out && lane == 0) tc[J.tc_off + s] = make_float2(r.top, r.colmin);
+  }
+}
+
+// Grid (column groups, jobs), a wave per edge column k < K-1 of job
+// blockIdx.y: pm[k][b] = min of its rows 0 .. b*R-1 (MAXV for b = 0). The
+// first wave of each job's first group then sets the job's stop swath and
+// walk state, as the reference's forward pass leaves them.
+template <bool OS, bool OE>
+__global__ void __launch_bounds__(256) k_sck_init(const SckJob* __restrict__ jobs, const float* __restrict__ slots,
+                                                  const float2* __restrict__ tc, float* __restrict__ pm,
+                                                  StripeState* __restrict__ st, int rshift) {
+  const float MAXV = Ar<float>::maxv();
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int t = blockIdx.y;
+  const SckJob J = jobs[t];
+  const int K = (J.LX + ODBA_W - 1) / ODBA_W;
+  const int R = 1 << rshift;
+  const int nb = (J.H + R - 1) >> rshift;
+  const int k = blockIdx.x * 4 + wid;
+  if (OE && !OS && k < K - 1) {
+    const float* col = slots + J.slot_off + (long long)k * ((long long)J.H + 256);
+    float* out = pm + J.pm_off + (long long)k * nb;
+    float run = MAXV;
+    if (lane == 0) out[0] = MAXV;
+    for (int b = 1; b < nb; b++) {
+      const int r1 = b * R;
+      for (int r = (b - 1) * R + lane; r < r1; r += 64) run = Ar<float>::mn(run, col[r]);
+      const float w = wave_min<float>(run);
+      if (lane == 0) out[b] = w;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int ab = -1;
+    if (OE && !OS) {
+      for (int s = 1; s < K - 1; s++)
+        if (s * ODBA_W > J.H && tc[J.tc_off + s - 1].x == tc[J.tc_off + s - 1].y) { ab = s; break; }
+      if (ab < 0 && K > 1 && (K - 1) * ODBA_W > J.H && tc[J.tc_off + K - 2].x == tc[J.tc_off + K - 2].y) ab = K - 1;
+    }
+    StripeState z;
+    z.i = J.H - 1; z.j = J.LX - 1; z.h = J.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0; z.npath = 0;
+    z.fab = ab; z.sab = 0; z.done = 0; z.err = 0; z.br = 0;
+    st[t] = z;
+  }
+}
+
+// The block's sweep: 16 columns per lane in every stripe, the last too (as
+// in the forward pass), so the step words of a stripe always have the
+// whole-swath layout.
+template <bool OS, bool OE>
+__device__ __forceinline__ void sck_block(const SwathIO<float>& io, const WaveWin<float>* W, int lane) {
+  (void)run_swath<float, 16, OS, OE, true, true>(io, W, lane);
+}
+
+// A wave per job (four per workgroup): see the comment of k_sck_forward.
+template <bool OS, bool OE>
+__global__ void __launch_bounds__(256) k_sck_step(const SckJob* __restrict__ jobs, int njobs,
+                                                  const float* __restrict__ slots, const float* __restrict__ rowck,
+                                                  const float* __restrict__ pm, uint32_t* __restrict__ mv,
+                                                  uint32_t* __restrict__ mtop, StripeState* __restrict__ st,
+                                                  int rshift) {
+  __shared__ float s_y[4][128];
+  __shared__ float s_e[4][64];
+  __shared__ float s_o[4][128];
+  const int lane = threadIdx.x & 63;
+  const int wid = uniform_i32((int)(threadIdx.x >> 6));
+  const int t = blockIdx.x * 4 + wid;
+  if (t >= njobs) return;
+  const WaveWin<float> win{s_y[wid], s_e[wid], s_o[wid]};
+  const SckJob jb = jobs[t];
+  const StripeState S = st[t];
+  if (S.done || S.err || S.j < 0) return;
+  const int k = uniform_i32(S.j) / ODBA_W;
+  const int h = uniform_i32(S.h), i = uniform_i32(S.i);
+  int sab = 0, br = 0;
+  if (S.fab >= 0 && k >= S.fab) {
+    sab = 1;  // abandoned in the forward pass: the stripe's top row is all OPEN_RIGHT
+  } else {
+    const int off = k * ODBA_W;
+    const bool entry = (i == h - 1);  // the walk is on the stripe's top row
+    const float* prev = k > 0 ? slots + jb.slot_off + (long long)(k - 1) * ((long long)jb.H + 256) : nullptr;
+    if (OE && !OS && entry && off > h) {  // dtw.hpp:74-129 at the truncated height
+      const float MAXV = Ar<float>::maxv();
+      if (prev[0] == MAXV) {
+        sab = 1;
+      } else {
+        const int nb = (jb.H + (1 << rshift) - 1) >> rshift;
+        const int bh = (h - 1) >> rshift;
+        float mn = pm[jb.pm_off + (long long)(k - 1) * nb + bh];  // rows below bh*R
+        for (int r = (bh << rshift) + lane; r < h; r += 64) mn = Ar<float>::mn(mn, prev[r]);
+        mn = wave_min<float>(mn);
+        if (prev[h - 1] == mn) sab = 1;
+      }
+    }
+    if (!sab) {
+      int b = i >> rshift;
+      // A top row alone in its band takes the band below too: the top-row
+      // rule reads the row under it.
+      if (OE && entry && b > 0 && i == (b << rshift)) b--;
+      const int bR = b << rshift;
+      br = bR;
+      SwathIO<float> io;
+      io.y = jb.y + bR; io.ly = i - bR + 1; io.x = jb.x; io.lx = jb.LX; io.off = off;
+      io.ein = prev ? prev + bR : nullptr; io.eout = nullptr;
+      io.mv = mv + jb.mv_off; io.mtop = mtop + (long long)t * 64; io.ref_left = 1;
+      io.top = b > 0 ? rowck + jb.rowck_off + (long long)(b - 1) * jb.LX : nullptr;
+      const bool os_b = OS && b == 0, oe_b = OE && entry;
+      if (os_b && oe_b) sck_block<true, true>(io, &win, lane);
+      else if (os_b) sck_block<true, false>(io, &win, lane);
+      else if (oe_b) sck_block<false, true>(io, &win, lane);
+      else sck_block<false, false>(io, &win, lane);
+    }
+  }
+  if (lane == 0) {
+    st[t].sab = sab;
+    st[t].br = br;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2325,6 +2538,38 @@ hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const 
   if (os && oe) ODBA_PBANDS(true, true); else if (os) ODBA_PBANDS(true, false); else if (oe) ODBA_PBANDS(false, true); else ODBA_PBANDS(false, false);
 #undef ODBA_PBANDS
   hipLaunchKernelGGL(k_abr_index, dim3((njobs + 255) / 256), dim3(256), 0, st, abr, njobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_sck_forward(int os, int oe, int nblocks, hipStream_t st, const SckJob* jobs, int njobs,
+                              long long nclaims, unsigned long long* counter, float* slots, float* rowck, float2* tc,
+                              int* cnt, int rshift, int* err, long long tick_limit) {
+  if (njobs < 1 || nblocks < 1) return hipErrorInvalidValue;
+  dim3 g(nblocks), b(64 * kSckBandSwaths);
+#define ODBA_SCKF(A, B) hipLaunchKernelGGL((k_sck_forward<A, B>), g, b, 0, st, jobs, njobs, nclaims, counter, slots, rowck, tc, cnt, rshift, err, tick_limit)
+  if (os && oe) ODBA_SCKF(true, true); else if (os) ODBA_SCKF(true, false); else if (oe) ODBA_SCKF(false, true); else ODBA_SCKF(false, false);
+#undef ODBA_SCKF
+  return hipGetLastError();
+}
+
+hipError_t launch_sck_init(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, int kmax, const float* slots,
+                           const float2* tc, float* pm, StripeState* ss, int rshift) {
+  if (njobs < 1) return hipSuccess;
+  dim3 g((unsigned)std::max(1, (kmax - 1 + 3) / 4), (unsigned)njobs), b(256);
+#define ODBA_SCKI(A, B) hipLaunchKernelGGL((k_sck_init<A, B>), g, b, 0, st, jobs, slots, tc, pm, ss, rshift)
+  if (os && oe) ODBA_SCKI(true, true); else if (os) ODBA_SCKI(true, false); else if (oe) ODBA_SCKI(false, true); else ODBA_SCKI(false, false);
+#undef ODBA_SCKI
+  return hipGetLastError();
+}
+
+hipError_t launch_sck_step(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, const float* slots,
+                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, StripeState* ss,
+                           int rshift) {
+  if (njobs < 1) return hipSuccess;
+  dim3 g((unsigned)((njobs + 3) / 4)), b(256);
+#define ODBA_SCKS(A, B) hipLaunchKernelGGL((k_sck_step<A, B>), g, b, 0, st, jobs, njobs, slots, rowck, pm, mv, mtop, ss, rshift)
+  if (os && oe) ODBA_SCKS(true, true); else if (os) ODBA_SCKS(true, false); else if (oe) ODBA_SCKS(false, true); else ODBA_SCKS(false, false);
+#undef ODBA_SCKS
   return hipGetLastError();
 }
 
