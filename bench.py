@@ -267,7 +267,9 @@ def run_stripes(args, N, WL):
     med = int(np.argmin((D.astype(np.float64) ** 2).sum(axis=1)))
     tb = {}
     if not args.no_traceback:
-        with N.Context(0) as ctx2, N.DbaSession(ctx2, seqs, 0, 0, path_policy=N.PATH_STRIPE) as s:
+        os_, oe = {"global": (0, 0), "open_end": (0, 1), "open_start": (1, 0), "open": (1, 1)}[args.stripe_mode]
+        lc = len(seqs[med])
+        with N.Context(0) as ctx2, N.DbaSession(ctx2, seqs, os_, oe, path_policy=N.PATH_STRIPE) as s:
             s.set_centroid(seqs[med])
             s.record_paths(True)
             t1 = time.perf_counter()
@@ -277,14 +279,22 @@ def run_stripes(args, N, WL):
             for k in range(n):
                 pi, pj, pc = s.path(k)
                 steps += len(pi)
-                ok = ok and pi[0] == len(seqs[k]) - 1 and pj[0] == len(seqs[med]) - 1 and pi[-1] == 0 and pj[-1] == 0
+                # from the matrix's far corner (the centroid on the rows when
+                # an open-end pair is flipped) to (0, 0), monotone
+                flip = oe and lc < len(seqs[k])
+                h, w = (lc, len(seqs[k])) if flip else (len(seqs[k]), lc)
+                ok = ok and pi[0] == h - 1 and pj[0] == w - 1 and pi[-1] == 0 and pj[-1] == 0
                 ok = ok and bool(np.all(np.diff(pi) <= 0) and np.all(np.diff(pj) <= 0))
                 ok = ok and bool(np.all((np.diff(pi) < 0) | (np.diff(pj) < 0)))
-            ok = ok and int(counts.astype(np.int64).sum()) == steps  # global mode: every path cell is counted
-        print(f"traceback round: {tb_s:.1f} s, {steps:,} path cells, checks {ok}", file=sys.stderr, flush=True)
-        fwd = float(sum(len(x) for x in seqs)) * len(seqs[med])
-        tb = {"dba_round_stripe_s": round(tb_s, 2), "medoid": med, "path_cells": steps, "forward_cells": fwd,
-              "paths_monotone_and_counts_match": ok}
+            counted = int(counts.astype(np.int64).sum())
+            # global mode counts every path cell; the open modes leave their
+            # open-right moves out
+            ok = ok and (counted == steps if (os_, oe) == (0, 0) else 0 < counted <= steps)
+        print(f"traceback round ({args.stripe_mode}): {tb_s:.1f} s, {steps:,} path cells, {counted:,} counted, "
+              f"checks {ok}", file=sys.stderr, flush=True)
+        fwd = float(sum(len(x) for x in seqs)) * lc
+        tb = {"mode": args.stripe_mode, "dba_round_stripe_s": round(tb_s, 2), "medoid": med, "path_cells": steps,
+              "counted_cells": counted, "forward_cells": fwd, "paths_monotone_and_counts_match": ok}
     kmed = float(np.median(kms))
     value = nominal / (kmed / 1000.0) / 1e9
     achieved = nominal * VALU_OPS_PER_CELL / (kmed / 1000.0) / 1e12
@@ -485,6 +495,8 @@ def main():
     ap.add_argument("--long-min", type=int, default=1_000_000, help="stripes: shortest length")
     ap.add_argument("--long-max", type=int, default=4_000_000, help="stripes: longest length")
     ap.add_argument("--no-traceback", action="store_true", help="stripes: skip the DBA round's tracebacks")
+    ap.add_argument("--stripe-mode", choices=["global", "open_end", "open_start", "open"], default="global",
+                    help="stripes: mode of the DBA round in stripe mode")
     ap.add_argument("--n-series", type=int, default=5000, help="clusters: squiggles")
     ap.add_argument("--clusters", type=int, default=250, help="clusters: K (clusters of 2+ members)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: jobs in the trace")

@@ -1140,6 +1140,8 @@ template <typename T> struct DbaState {
   size_t sck_pm_cap = 0;
   SckJob* sck_jobs = nullptr;
   size_t sck_jobs_cap = 0;
+  float* sck_q = nullptr;  // quarter strips' edges over a block, three per job
+  size_t sck_q_cap = 0;
   // Traceback recording (the cells writeDTWPath prints): per sequence a
   // count slot then up to |seq| + |centroid| packed steps.
   bool rec = false;
@@ -1178,7 +1180,8 @@ template <typename T> struct DbaState {
                     (void*)mv, (void*)mtop, (void*)abr, (void*)jobs, (void*)tbj, (void*)err, (void*)dbits,
                     (void*)slots, (void*)smv, (void*)stop, (void*)sstate, (void*)sjobs, (void*)stbj, (void*)pbuf,
                     (void*)ck_rowck, (void*)ck_colck, (void*)ck_cnt, (void*)ck_mv, (void*)ck_slots, (void*)ck_rings,
-                    (void*)ck_jobs, (void*)ck_states, (void*)sck_slots, (void*)sck_tc, (void*)sck_pm, (void*)sck_jobs})
+                    (void*)ck_jobs, (void*)ck_states, (void*)sck_slots, (void*)sck_tc, (void*)sck_pm, (void*)sck_jobs,
+                    (void*)sck_q})
       if (p) hipFree(p);
   }
 
@@ -1384,16 +1387,21 @@ template <typename T> struct DbaState {
   // jobs there keep the reference's stripe semantics but recompute only the
   // band of rows holding the walk's current cell, from cost rows stored by a
   // forward pass over the whole device (k_sck_forward); ODBA_CKPT=0 keeps
-  // the recompute from row 0, ODBA_CKPT_RSHIFT sets the band height 2^k.
+  // the recompute from row 0, ODBA_CKPT_RSHIFT sets the band height 2^k
+  // (default 4,096 rows: each step recomputes at most a band, so shorter
+  // bands pay for their checkpoint rows, a quarter of the edge columns'
+  // memory; 8 open-end pairs of 1M-4M samples: 44.3 s per round against
+  // 59.4 s with 16,384-row bands).
   bool use_sck() const {
     const char* e = std::getenv("ODBA_CKPT");
     return std::is_same<T, float>::value && (os || oe) && !(e && !std::strcmp(e, "0"));
   }
-  static int ck_rshift() { return std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 14))); }
+  static int ck_rshift() { return std::max(6, std::min(20, env_int("ODBA_CKPT_RSHIFT", 12))); }
   // Device 4-byte units of one checkpointed open-mode stripe job.
   static long long sck_need(int H, int LX, int rshift) {
     const long long R = 1LL << rshift, K = (LX + ODBA_W - 1) / ODBA_W, nb = (H + R - 1) / R;
-    return K * ((long long)H + 256) + (nb - 1) * (long long)LX + 2 * K + K * nb + K + (R + 127) * 64 + 64;
+    return K * ((long long)H + 256) + (nb - 1) * (long long)LX + 2 * K + K * nb + K + (R + 127) * 64 + 64 +
+           3 * (R + 1 + 256);
   }
 
   int run_sck_batch(const std::vector<int>& idx, int b0) {
@@ -1405,6 +1413,7 @@ template <typename T> struct DbaState {
       const int rshift = ck_rshift();
       const long long R = 1LL << rshift;
       const long long job_words = (R + 127) * 64;  // a block of up to R + 1 rows
+      const long long qstride = R + 1 + 256;       // a quarter strip's edge over a block
       std::vector<SckJob> sj(nj);
       std::vector<TbJob> tb(nj);
       long long nslot = 0, nrow = 0, ntc = 0, npm = 0, ncnt = 0, nclaims = 0, words = 0;
@@ -1439,7 +1448,8 @@ template <typename T> struct DbaState {
           (rc = grow(c, sck_tc, sck_tc_cap, (size_t)ntc)) || (rc = grow(c, sck_pm, sck_pm_cap, (size_t)npm)) ||
           (rc = grow(c, ck_cnt, ck_cnt_cap, (size_t)ncnt)) || (rc = grow(c, smv, smv_cap, (size_t)words)) ||
           (rc = grow(c, stop, stop_cap, (size_t)nj * 64)) || (rc = grow(c, sstate, sstate_cap, (size_t)nj)) ||
-          (rc = grow(c, sck_jobs, sck_jobs_cap, (size_t)nj)) || (rc = grow(c, stbj, stbj_cap, (size_t)nj)))
+          (rc = grow(c, sck_jobs, sck_jobs_cap, (size_t)nj)) || (rc = grow(c, stbj, stbj_cap, (size_t)nj)) ||
+          (rc = grow(c, sck_q, sck_q_cap, (size_t)nj * 3 * (size_t)qstride)))
         return rc;
       HIPC(c, hipMemcpyAsync(sck_jobs, sj.data(), nj * sizeof(SckJob), hipMemcpyHostToDevice, c->stream));
       HIPC(c, hipMemcpyAsync(stbj, tb.data(), nj * sizeof(TbJob), hipMemcpyHostToDevice, c->stream));
@@ -1456,7 +1466,8 @@ template <typename T> struct DbaState {
                                  ck_cnt, rshift, ferr, limit));
       HIPC(c, launch_sck_init(os, oe, c->stream, sck_jobs, nj, kmax, sck_slots, sck_tc, sck_pm, sstate, rshift));
       for (int it = 0; it < maxit; it++) {
-        HIPC(c, launch_sck_step(os, oe, c->stream, sck_jobs, nj, sck_slots, ck_rowck, sck_pm, smv, stop, sstate, rshift));
+        HIPC(c, launch_sck_step(os, oe, c->stream, sck_jobs, nj, sck_slots, ck_rowck, sck_pm, smv, stop, sck_q, qstride,
+                                sstate, rshift));
         HIPC(c, launch_sck_traceback(c->stream, stbj, nj, smv, stop, sstate, runs, os, oe, err, rec ? pbuf : nullptr));
       }
       std::vector<StripeState> sts(nj);

@@ -182,8 +182,8 @@ hipError_t launch_sck_forward(int os, int oe, int nblocks, hipStream_t st, const
 hipError_t launch_sck_init(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, int kmax, const float* slots,
                            const float2* tc, float* pm, StripeState* ss, int rshift);
 hipError_t launch_sck_step(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, const float* slots,
-                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, StripeState* ss,
-                           int rshift);
+                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, float* qslots,
+                           long long qstride, StripeState* ss, int rshift);
 hipError_t launch_sck_traceback(hipStream_t st, const TbJob* jobs, int njobs, const uint32_t* mv,
                                 const uint32_t* mtop, StripeState* ss, int4* runs, int os, int oe, int* err,
                                 unsigned long long* pbuf);

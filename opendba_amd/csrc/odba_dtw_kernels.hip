@@ -2294,29 +2294,39 @@ __global__ void __launch_bounds__(256) k_sck_init(const SckJob* __restrict__ job
   }
 }
 
-// The block's sweep: 16 columns per lane in every stripe, the last too (as
-// in the forward pass), so the step words of a stripe always have the
-// whole-swath layout.
+// The block's sweep as four quarter strips (a wave each, 4 columns per
+// lane), the last stripe too, so a stripe's step words always have the
+// whole-swath layout (the quarters join their fragments into it).
 template <bool OS, bool OE>
-__device__ __forceinline__ void sck_block(const SwathIO<float>& io, const WaveWin<float>* W, int lane) {
-  (void)run_swath<float, 16, OS, OE, true, true>(io, W, lane);
+__device__ __forceinline__ void sck_block(const SwathIO<float>& io, const WaveWin<float>* W, int lane,
+                                          const StripPipe& pp) {
+  (void)run_swath<float, 4, OS, OE, true, true, StripPipe, false, true>(io, W, lane, pp);
 }
 
-// A wave per job (four per workgroup): see the comment of k_sck_forward.
+// A workgroup of four waves per job: see the comment of k_sck_forward. Wave
+// q sweeps columns [256q, 256q + 256) of the stripe; wave 0 reads the
+// stored edge column of the left stripe (complete), waves 1-3 their left
+// neighbour's edge through qslots (rows of the block, LDS progress words).
 template <bool OS, bool OE>
 __global__ void __launch_bounds__(256) k_sck_step(const SckJob* __restrict__ jobs, int njobs,
                                                   const float* __restrict__ slots, const float* __restrict__ rowck,
                                                   const float* __restrict__ pm, uint32_t* __restrict__ mv,
-                                                  uint32_t* __restrict__ mtop, StripeState* __restrict__ st,
-                                                  int rshift) {
+                                                  uint32_t* __restrict__ mtop, float* __restrict__ qslots,
+                                                  long long qstride, StripeState* __restrict__ st, int rshift) {
   __shared__ float s_y[4][128];
   __shared__ float s_e[4][64];
   __shared__ float s_o[4][128];
+  __shared__ int s_owner[5];
+  __shared__ int s_cnt[5];
   const int lane = threadIdx.x & 63;
-  const int wid = uniform_i32((int)(threadIdx.x >> 6));
-  const int t = blockIdx.x * 4 + wid;
-  if (t >= njobs) return;
-  const WaveWin<float> win{s_y[wid], s_e[wid], s_o[wid]};
+  const int q = uniform_i32((int)(threadIdx.x >> 6));
+  const int t = blockIdx.x;
+  if (threadIdx.x < 5) {
+    s_owner[threadIdx.x] = threadIdx.x < 4 ? (int)threadIdx.x : -1;
+    s_cnt[threadIdx.x] = threadIdx.x < 4 ? 0 : INT_MAX;  // the last one: wave 0's input, always ready
+  }
+  __syncthreads();
+  const WaveWin<float> win{s_y[q], s_e[q], s_o[q]};
   const SckJob jb = jobs[t];
   const StripeState S = st[t];
   if (S.done || S.err || S.j < 0) return;
@@ -2329,7 +2339,7 @@ __global__ void __launch_bounds__(256) k_sck_step(const SckJob* __restrict__ job
     const int off = k * ODBA_W;
     const bool entry = (i == h - 1);  // the walk is on the stripe's top row
     const float* prev = k > 0 ? slots + jb.slot_off + (long long)(k - 1) * ((long long)jb.H + 256) : nullptr;
-    if (OE && !OS && entry && off > h) {  // dtw.hpp:74-129 at the truncated height
+    if (OE && !OS && entry && off > h) {  // dtw.hpp:74-129 at the truncated height (every wave alike)
       const float MAXV = Ar<float>::maxv();
       if (prev[0] == MAXV) {
         sab = 1;
@@ -2349,19 +2359,24 @@ __global__ void __launch_bounds__(256) k_sck_step(const SckJob* __restrict__ job
       if (OE && entry && b > 0 && i == (b << rshift)) b--;
       const int bR = b << rshift;
       br = bR;
+      StripPipe pp{&s_owner[q > 0 ? q - 1 : 4], &s_cnt[q > 0 ? q - 1 : 4], q > 0 ? q - 1 : -1,
+                   q < 3 ? &s_cnt[q] : nullptr, lane};
       SwathIO<float> io;
-      io.y = jb.y + bR; io.ly = i - bR + 1; io.x = jb.x; io.lx = jb.LX; io.off = off;
-      io.ein = prev ? prev + bR : nullptr; io.eout = nullptr;
-      io.mv = mv + jb.mv_off; io.mtop = mtop + (long long)t * 64; io.ref_left = 1;
+      io.y = jb.y + bR; io.ly = i - bR + 1; io.x = jb.x; io.lx = jb.LX; io.off = off + q * 256;
+      io.ein = q == 0 ? (prev ? prev + bR : nullptr) : qslots + ((long long)t * 3 + q - 1) * qstride;
+      io.eout = q < 3 ? qslots + ((long long)t * 3 + q) * qstride : nullptr;
+      io.mv = mv + jb.mv_off; io.mtop = mtop + (long long)t * 64;
+      io.ref_left = (q == 0);
+      io.lbase = q * 16;
       io.top = b > 0 ? rowck + jb.rowck_off + (long long)(b - 1) * jb.LX : nullptr;
       const bool os_b = OS && b == 0, oe_b = OE && entry;
-      if (os_b && oe_b) sck_block<true, true>(io, &win, lane);
-      else if (os_b) sck_block<true, false>(io, &win, lane);
-      else if (oe_b) sck_block<false, true>(io, &win, lane);
-      else sck_block<false, false>(io, &win, lane);
+      if (os_b && oe_b) sck_block<true, true>(io, &win, lane, pp);
+      else if (os_b) sck_block<true, false>(io, &win, lane, pp);
+      else if (oe_b) sck_block<false, true>(io, &win, lane, pp);
+      else sck_block<false, false>(io, &win, lane, pp);
     }
   }
-  if (lane == 0) {
+  if (q == 0 && lane == 0) {
     st[t].sab = sab;
     st[t].br = br;
   }
@@ -2563,11 +2578,11 @@ hipError_t launch_sck_init(int os, int oe, hipStream_t st, const SckJob* jobs, i
 }
 
 hipError_t launch_sck_step(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, const float* slots,
-                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, StripeState* ss,
-                           int rshift) {
+                           const float* rowck, const float* pm, uint32_t* mv, uint32_t* mtop, float* qslots,
+                           long long qstride, StripeState* ss, int rshift) {
   if (njobs < 1) return hipSuccess;
-  dim3 g((unsigned)((njobs + 3) / 4)), b(256);
-#define ODBA_SCKS(A, B) hipLaunchKernelGGL((k_sck_step<A, B>), g, b, 0, st, jobs, njobs, slots, rowck, pm, mv, mtop, ss, rshift)
+  dim3 g((unsigned)njobs), b(256);
+#define ODBA_SCKS(A, B) hipLaunchKernelGGL((k_sck_step<A, B>), g, b, 0, st, jobs, njobs, slots, rowck, pm, mv, mtop, qslots, qstride, ss, rshift)
   if (os && oe) ODBA_SCKS(true, true); else if (os) ODBA_SCKS(true, false); else if (oe) ODBA_SCKS(false, true); else ODBA_SCKS(false, false);
 #undef ODBA_SCKS
   return hipGetLastError();
