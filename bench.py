@@ -526,7 +526,7 @@ def main():
         # this when it starts; with fewer queues than streams, slots share
         # queues and their kernels run one after another). The default run
         # serves too, after its headline.
-        want = min(32, args.slots + 8)  # the slots' streams and two rounds engines' four streams each
+        want = min(32, args.slots + 20)  # the slots' streams and up to four rounds engines' five streams each
         if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
             os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     from opendba_amd import native as N

@@ -295,6 +295,8 @@ uint64_t odba_kseg_capacity(const uint64_t* lens, int n, int min_segment_length)
  * with allocate != 0, allocates it on the calling thread (a server does
  * this on its preparing thread once the bytes fit its memory budget);
  * odba_rounds_add allocates it when that has not been done.
+ * odba_job_shape gives a prepared job's centroid length (its medoid's) and
+ * the largest number of 1024-column swaths of its path matrices.
  */
 typedef struct odba_job odba_job;
 typedef struct odba_rounds odba_rounds;
@@ -303,6 +305,7 @@ int odba_job_prepare(odba_ctx* ctx, int dtype, const void* const* seqs, const ui
 int odba_job_result(const odba_job* job, void* centroid_out, uint64_t capacity, uint64_t* centroid_len,
                     int* medoid_sorted, int* rounds, int* job_rc);
 int odba_job_words(odba_job* job, int allocate, uint64_t* word_bytes);
+int odba_job_shape(const odba_job* job, uint64_t* centroid_len, int* max_swaths);
 int odba_job_destroy(odba_job* job);
 int odba_rounds_create(odba_ctx* ctx, int dtype, int use_open_start, int use_open_end, odba_rounds** out);
 int odba_rounds_add(odba_rounds* engine, odba_job* job);

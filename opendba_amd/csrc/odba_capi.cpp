@@ -2739,6 +2739,29 @@ int odba_job_words(odba_job* h, int allocate, uint64_t* word_bytes) {
   return ODBA_OK;
 }
 
+int odba_job_shape(const odba_job* h, uint64_t* centroid_len, int* max_swaths) {
+  if (!h || !h->prep) return ODBA_E_INVALID;
+  int lc = 0, kmax = 0;
+  auto shape = [&](const auto& P) {
+    lc = P.lc;
+    for (int q = 0; q < P.n; q++) {
+      int flip, ly, lx;
+      member_shape(P, q, flip, ly, lx);
+      kmax = std::max(kmax, (lx + ODBA_W - 1) / ODBA_W);
+    }
+  };
+  switch (h->dtype) {
+    case ODBA_I32: shape(*(const Prepared<int32_t>*)h->prep); break;
+    case ODBA_U32: shape(*(const Prepared<uint32_t>*)h->prep); break;
+    case ODBA_U64: shape(*(const Prepared<uint64_t>*)h->prep); break;
+    case ODBA_F32: shape(*(const Prepared<float>*)h->prep); break;
+    default: shape(*(const Prepared<double>*)h->prep); break;
+  }
+  if (centroid_len) *centroid_len = (uint64_t)lc;
+  if (max_swaths) *max_swaths = kmax;
+  return ODBA_OK;
+}
+
 int odba_job_destroy(odba_job* h) {
   if (!h) return ODBA_OK;
   hipSetDevice(h->device);

@@ -27,7 +27,7 @@ EXPORTS = [
     "odba_server_release", "odba_server_stats", "odba_dba_clusters", "odba_cluster", "odba_hclust_complete",
     "odba_cutree_k", "odba_prefix_chop", "odba_tune_all_pairs", "odba_get_schedule", "odba_set_schedule",
     "odba_ctx_set_graphs", "odba_dtw_all_pairs_noalloc", "odba_job_prepare", "odba_job_result", "odba_job_destroy",
-    "odba_job_words",
+    "odba_job_words", "odba_job_shape",
     "odba_rounds_create", "odba_rounds_add", "odba_rounds_step", "odba_rounds_destroy",
 ]
 SCHED_WAVE, SCHED_BANDS = 0, 1
@@ -107,6 +107,7 @@ def load():
     L.odba_job_result.argtypes = [vp, vp, u64, C.POINTER(u64), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.odba_job_destroy.argtypes = [vp]
     L.odba_job_words.argtypes = [vp, i32, C.POINTER(u64)]
+    L.odba_job_shape.argtypes = [vp, C.POINTER(u64), C.POINTER(i32)]
     L.odba_rounds_create.argtypes = [vp, i32, i32, i32, C.POINTER(vp)]
     L.odba_rounds_add.argtypes = [vp, vp]
     L.odba_rounds_step.argtypes = [vp, vp, i32, C.POINTER(i32), C.POINTER(i32)]
@@ -451,6 +452,14 @@ class Job:
         if rc:
             raise OdbaError(rc, "odba_job_words failed")
         return wb.value
+
+    def shape(self):
+        """(centroid length, largest swath count of its path matrices)."""
+        lc, k = C.c_uint64(), C.c_int()
+        rc = self.L.odba_job_shape(self.h, C.byref(lc), C.byref(k))
+        if rc:
+            raise OdbaError(rc, "odba_job_shape failed")
+        return lc.value, k.value
 
     def close(self):
         if self.h:

@@ -173,20 +173,23 @@ def test_job_words_size(os_, oe):
     """odba_job_words: every member's path words (flipped onto the columns
     in open end when the centroid is shorter) plus 64 top-row words per
     swath, 4 bytes each; allocating keeps the size and a second call is a
-    no-op."""
+    no-op. odba_job_shape: the centroid length and the widest member's
+    swath count."""
     jobs = _jobs(seed=71 + oe, n_jobs=2, lmin=100, lmax=5000)
     with N.Context(0) as ctx:
         for s in jobs:
             lc = len(ctx.perform_dba(s, os_, oe, norm=1, max_rounds=1)["centroid"])
-            want = 0
+            want, kmax = 0, 0
             for x in s:
                 L = len(x)
                 flip = oe and lc < L
                 ly, lx = (lc, L) if flip else (L, lc)
                 K = (lx + 1023) // 1024
+                kmax = max(kmax, K)
                 want += K * (_swath_words(ly, os_, oe) + 64) * 4
             p = N.Job(ctx, s, os_, oe, 1, 5)
             try:
+                assert p.shape() == (lc, kmax)
                 assert p.words() == want
                 assert p.words(allocate=True) == want
                 assert p.words(allocate=True) == want
