@@ -41,7 +41,30 @@ struct odba_ctx {
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   bool graphs = true;  // replay DBA rounds as hipGraphs (off for contexts that share a busy device)
+  // Grow-only buffers of the calls that would otherwise allocate and free
+  // per call (all-vs-all distances and flags, z-norm statistics, the pair
+  // order and small-pair records of a job's all-vs-all): freeing device
+  // memory waits for the device's queued work, so a serving slot that
+  // freed at the end of each preparation waited for every other slot's and
+  // engine's kernels.
+  void* gb[5] = {};
+  size_t gb_bytes[5] = {};
 };
+
+namespace {
+enum { kGbDist = 0, kGbFlags, kGbStats, kGbOrder, kGbSmallTab };
+// The context's grow-only buffer `k` with at least `bytes`, or null.
+void* ctx_buffer(odba_ctx* c, int k, size_t bytes) {
+  if (bytes <= c->gb_bytes[k]) return c->gb[k];
+  if (c->gb[k]) hipFree(c->gb[k]);
+  c->gb[k] = nullptr;
+  c->gb_bytes[k] = 0;
+  const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+  if (hipMalloc(&c->gb[k], want) != hipSuccess) return nullptr;
+  c->gb_bytes[k] = want;
+  return c->gb[k];
+}
+}  // namespace
 
 struct odba_seqset {
   odba_ctx* ctx = nullptr;
@@ -57,6 +80,7 @@ struct odba_seqset {
   void* order = nullptr;       // device pair order (int2 per pair), largest cell count first
   SmallPairRec* small_tab = nullptr;  // device pair records of the small-pair kernel
   int small_tab_order = -1;           // the order they were built in
+  bool borrowed = false;              // a view: order and small_tab live in the context's buffers
 };
 
 // Fixed device scratch of odba_dtw_all_pairs_noalloc: the pair counter in
@@ -221,6 +245,8 @@ int odba_ctx_destroy(odba_ctx* c) {
   hipStreamSynchronize(c->stream);
   if (c->esc) hipFree(c->esc);
   if (c->small) hipFree(c->small);
+  for (void* p : c->gb)
+    if (p) hipFree(p);
   if (c->pinned) hipHostFree(c->pinned);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -465,10 +491,9 @@ int ensure_pair_order(odba_ctx* c, odba_seqset* s, long long npairs) {
   std::stable_sort(ord.begin(), ord.end(), [&](const int2& a, const int2& b) {
     return (long long)s->h_lens[a.x] * s->h_lens[a.y] > (long long)s->h_lens[b.x] * s->h_lens[b.y];
   });
-  if (hipMalloc(&s->order, ord.size() * sizeof(int2)) != hipSuccess) {
-    s->order = nullptr;
-    return fail(c, ODBA_E_NOMEM, "cannot allocate the pair order");
-  }
+  if (s->borrowed) s->order = ctx_buffer(c, kGbOrder, ord.size() * sizeof(int2));
+  else if (hipMalloc(&s->order, ord.size() * sizeof(int2)) != hipSuccess) s->order = nullptr;
+  if (!s->order) return fail(c, ODBA_E_NOMEM, "cannot allocate the pair order");
   HIPC(c, hipMemcpyAsync(s->order, ord.data(), ord.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return ODBA_OK;
@@ -494,7 +519,10 @@ int ensure_small_tab(odba_ctx* c, odba_seqset* s, int order, long long npairs) {
     tab[i] = SmallPairRec{s->h_offs[r], s->h_offs[cc], (long long)r * (2 * n - r - 1) / 2 + (cc - r - 1),
                           s->h_lens[r], s->h_lens[cc]};
   }
-  if (!s->small_tab && hipMalloc(&s->small_tab, tab.size() * sizeof(SmallPairRec)) != hipSuccess) {
+  if (s->borrowed) {
+    s->small_tab = (SmallPairRec*)ctx_buffer(c, kGbSmallTab, tab.size() * sizeof(SmallPairRec));
+    if (!s->small_tab) return fail(c, ODBA_E_NOMEM, "cannot allocate the pair records");
+  } else if (!s->small_tab && hipMalloc(&s->small_tab, tab.size() * sizeof(SmallPairRec)) != hipSuccess) {
     s->small_tab = nullptr;
     return fail(c, ODBA_E_NOMEM, "cannot allocate the pair records");
   }
@@ -678,13 +706,11 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   const long long npairs = (long long)n * (n - 1) / 2;
   const size_t es = dtype_size(s->dtype);
   const PairSchedule ps = choose_schedule(c, s, os, oe, npairs);
-  void* dist = nullptr;
+  void* dist = ctx_buffer(c, kGbDist, (size_t)npairs * es);
+  if (!dist) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
   int* flags = nullptr;
-  if (hipMalloc(&dist, npairs * es) != hipSuccess) return fail(c, ODBA_E_NOMEM, "cannot allocate %lld distances", npairs);
-  if (flags_out && hipMalloc((void**)&flags, npairs * sizeof(int)) != hipSuccess) {
-    hipFree(dist);
+  if (flags_out && !(flags = (int*)ctx_buffer(c, kGbFlags, (size_t)npairs * sizeof(int))))
     return fail(c, ODBA_E_NOMEM, "cannot allocate flags");
-  }
   int rc = run_all_pairs(c, s, ps, os, oe, npairs, dist, flags, kernel_ms);
   // copies on the context's stream: a plain hipMemcpy runs on the null
   // stream and would wait for every other context's work on the device
@@ -693,8 +719,6 @@ int odba_dtw_all_pairs_resident(odba_ctx* c, odba_seqset* s, int os, int oe, voi
   if (!rc && flags_out && hipMemcpyAsync(flags_out, flags, npairs * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     rc = fail(c, ODBA_E_HIP, "copying flags");
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, ODBA_E_HIP, "stream synchronisation");
-  hipFree(dist);
-  if (flags) hipFree(flags);
   return rc;
 }
 
@@ -1928,7 +1952,6 @@ int prepare_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, i
   for (int i = 0; i < n; i++) std::memcpy(&hb[ho[i]], in[perm[i]], l[i] * sizeof(T));
   struct Stats {
     double* mu = nullptr; double* sg = nullptr;
-    ~Stats() { for (void* p : {(void*)mu, (void*)sg}) if (p) hipFree(p); }
   } S;
   HIPC(c, hipMalloc((void**)&P.d, hb.size() * sizeof(T)));
   HIPC(c, hipMalloc((void**)&P.o, n * sizeof(long long)));
@@ -1939,8 +1962,9 @@ int prepare_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, i
   P.mu.assign(n, 0.0);
   P.sg.assign(n, 1.0);
   if (norm) {
-    HIPC(c, hipMalloc((void**)&S.mu, n * sizeof(double)));
-    HIPC(c, hipMalloc((void**)&S.sg, n * sizeof(double)));
+    S.mu = (double*)ctx_buffer(c, kGbStats, 2 * (size_t)n * sizeof(double));
+    if (!S.mu) return fail(c, ODBA_E_NOMEM, "cannot allocate z-norm statistics");
+    S.sg = S.mu + n;
     HIPC(c, launch_znorm(dtype_of<T>(), c->stream, P.d, P.o, P.ln, n, S.mu, S.sg));
     HIPC(c, hipMemcpyAsync(P.mu.data(), S.mu, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipMemcpyAsync(P.sg.data(), S.sg, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1958,12 +1982,9 @@ int prepare_t(odba_ctx* c, const T* const* in, const uint64_t* in_lens, int n, i
   odba_seqset view;
   view.ctx = c; view.dtype = dtype_of<T>(); view.n = n; view.data = P.d; view.offs = P.o; view.lens = P.ln;
   view.h_offs = ho; view.h_lens = hl; view.maxlen = maxlen; view.int_no_wrap = P.no_wrap;
+  view.borrowed = true;  // its pair order and small-pair records go to the context's buffers
   std::vector<T> d((size_t)n * (n - 1) / 2);
-  int rc = odba_dtw_all_pairs_resident(c, &view, os, oe, d.data(), nullptr, nullptr);
-  if (view.order) {  // the view owns nothing else
-    hipFree(view.order);
-    view.order = nullptr;
-  }
+  const int rc = odba_dtw_all_pairs_resident(c, &view, os, oe, d.data(), nullptr, nullptr);
   if (rc) return rc;
   if (dists_out) std::memcpy(dists_out, d.data(), d.size() * sizeof(T));
   P.med = medoid_single<T>(d.data(), l, n);
