@@ -2251,46 +2251,48 @@ __global__ void __launch_bounds__(1024) k_sck_forward(const SckJob* __restrict__
   }
 }
 
-// Grid (column groups, jobs), a wave per edge column k < K-1 of job
-// blockIdx.y: pm[k][b] = min of its rows 0 .. b*R-1 (MAXV for b = 0). The
-// first wave of each job's first group then sets the job's stop swath and
-// walk state, as the reference's forward pass leaves them.
+// Grid (column groups, job groups), a wave per edge column k < K-1 of each
+// job t = blockIdx.y, blockIdx.y + gridDim.y, ...: pm[k][b] = min of its rows
+// 0 .. b*R-1 (MAXV for b = 0). The first wave of each job's first group
+// then sets the job's stop swath and walk state, as the reference's forward
+// pass leaves them.
 template <bool OS, bool OE>
-__global__ void __launch_bounds__(256) k_sck_init(const SckJob* __restrict__ jobs, const float* __restrict__ slots,
-                                                  const float2* __restrict__ tc, float* __restrict__ pm,
-                                                  StripeState* __restrict__ st, int rshift) {
+__global__ void __launch_bounds__(256) k_sck_init(const SckJob* __restrict__ jobs, int njobs,
+                                                  const float* __restrict__ slots, const float2* __restrict__ tc,
+                                                  float* __restrict__ pm, StripeState* __restrict__ st, int rshift) {
   const float MAXV = Ar<float>::maxv();
   const int lane = threadIdx.x & 63;
   const int wid = uniform_i32((int)(threadIdx.x >> 6));
-  const int t = blockIdx.y;
-  const SckJob J = jobs[t];
-  const int K = (J.LX + ODBA_W - 1) / ODBA_W;
-  const int R = 1 << rshift;
-  const int nb = (J.H + R - 1) >> rshift;
-  const int k = blockIdx.x * 4 + wid;
-  if (OE && !OS && k < K - 1) {
-    const float* col = slots + J.slot_off + (long long)k * ((long long)J.H + 256);
-    float* out = pm + J.pm_off + (long long)k * nb;
-    float run = MAXV;
-    if (lane == 0) out[0] = MAXV;
-    for (int b = 1; b < nb; b++) {
-      const int r1 = b * R;
-      for (int r = (b - 1) * R + lane; r < r1; r += 64) run = Ar<float>::mn(run, col[r]);
-      const float w = wave_min<float>(run);
-      if (lane == 0) out[b] = w;
+  for (int t = blockIdx.y; t < njobs; t += gridDim.y) {
+    const SckJob J = jobs[t];
+    const int K = (J.LX + ODBA_W - 1) / ODBA_W;
+    const int R = 1 << rshift;
+    const int nb = (J.H + R - 1) >> rshift;
+    const int k = blockIdx.x * 4 + wid;
+    if (OE && !OS && k < K - 1) {
+      const float* col = slots + J.slot_off + (long long)k * ((long long)J.H + 256);
+      float* out = pm + J.pm_off + (long long)k * nb;
+      float run = MAXV;
+      if (lane == 0) out[0] = MAXV;
+      for (int b = 1; b < nb; b++) {
+        const int r1 = b * R;
+        for (int r = (b - 1) * R + lane; r < r1; r += 64) run = Ar<float>::mn(run, col[r]);
+        const float w = wave_min<float>(run);
+        if (lane == 0) out[b] = w;
+      }
     }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int ab = -1;
-    if (OE && !OS) {
-      for (int s = 1; s < K - 1; s++)
-        if (s * ODBA_W > J.H && tc[J.tc_off + s - 1].x == tc[J.tc_off + s - 1].y) { ab = s; break; }
-      if (ab < 0 && K > 1 && (K - 1) * ODBA_W > J.H && tc[J.tc_off + K - 2].x == tc[J.tc_off + K - 2].y) ab = K - 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      int ab = -1;
+      if (OE && !OS) {
+        for (int s = 1; s < K - 1; s++)
+          if (s * ODBA_W > J.H && tc[J.tc_off + s - 1].x == tc[J.tc_off + s - 1].y) { ab = s; break; }
+        if (ab < 0 && K > 1 && (K - 1) * ODBA_W > J.H && tc[J.tc_off + K - 2].x == tc[J.tc_off + K - 2].y) ab = K - 1;
+      }
+      StripeState z;
+      z.i = J.H - 1; z.j = J.LX - 1; z.h = J.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0; z.npath = 0;
+      z.fab = ab; z.sab = 0; z.done = 0; z.err = 0; z.br = 0;
+      st[t] = z;
     }
-    StripeState z;
-    z.i = J.H - 1; z.j = J.LX - 1; z.h = J.H; z.curc = -1; z.lo = 1; z.hi = 0; z.lo2 = 1; z.hi2 = 0; z.gap = 0; z.npath = 0;
-    z.fab = ab; z.sab = 0; z.done = 0; z.err = 0; z.br = 0;
-    st[t] = z;
   }
 }
 
@@ -2570,8 +2572,8 @@ hipError_t launch_sck_forward(int os, int oe, int nblocks, hipStream_t st, const
 hipError_t launch_sck_init(int os, int oe, hipStream_t st, const SckJob* jobs, int njobs, int kmax, const float* slots,
                            const float2* tc, float* pm, StripeState* ss, int rshift) {
   if (njobs < 1) return hipSuccess;
-  dim3 g((unsigned)std::max(1, (kmax - 1 + 3) / 4), (unsigned)njobs), b(256);
-#define ODBA_SCKI(A, B) hipLaunchKernelGGL((k_sck_init<A, B>), g, b, 0, st, jobs, slots, tc, pm, ss, rshift)
+  dim3 g((unsigned)std::max(1, (kmax - 1 + 3) / 4), (unsigned)std::min(njobs, 65535)), b(256);
+#define ODBA_SCKI(A, B) hipLaunchKernelGGL((k_sck_init<A, B>), g, b, 0, st, jobs, njobs, slots, tc, pm, ss, rshift)
   if (os && oe) ODBA_SCKI(true, true); else if (os) ODBA_SCKI(true, false); else if (oe) ODBA_SCKI(false, true); else ODBA_SCKI(false, false);
 #undef ODBA_SCKI
   return hipGetLastError();
