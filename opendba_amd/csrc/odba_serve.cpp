@@ -62,7 +62,6 @@ struct Job {
   int os = 0, oe = 0, norm = 1, max_rounds = 250;
   uint64_t footprint = 0;
   bool worded = false;  // step words counted in the server's in_use
-  bool wide = false;    // a path matrix of more than 16 swaths
   int state = ODBA_JOB_QUEUED;
   int rc = 0;
   std::string err;
@@ -146,13 +145,7 @@ struct odba_server {
     return running == 0 || in_use + j->footprint <= budget;
   }
 
-  // Engines by value type and mode, and by width: jobs whose widest path
-  // matrix has more than 16 swaths (the swath-band schedule, whose launch
-  // spans the device for most of a step) get an engine of their own, so
-  // the rounds of narrower jobs do not wait for it every step. (A third
-  // class for one-swath jobs moved the last finish from the global-mode
-  // jobs to the open-end ones on the full-size trace: 43.9 s against 43.4 s.)
-  static int class_key(const Job& j) { return j.dtype * 8 + (j.os ? 4 : 0) + (j.oe ? 2 : 0) + (j.wide ? 1 : 0); }
+  static int class_key(const Job& j) { return j.dtype * 4 + (j.os ? 2 : 0) + (j.oe ? 1 : 0); }
 
   // Records a finished job's outcome (lock held).
   void complete(Job* j, int rc, const std::string& msg) {
@@ -234,9 +227,6 @@ struct odba_server {
       std::string msg = rc ? odba_last_error(c) : "";
       uint64_t wb = 0;
       if (!rc && (rc = odba_job_words(h, 0, &wb))) msg = "cannot size the job's step words";
-      int kmax = 0;
-      if (!rc && (rc = odba_job_shape(h, nullptr, &kmax))) msg = "cannot read the job's shape";
-      j->wide = kmax > 16;
       if (!rc) {
         std::unique_lock<std::mutex> lk(mu);
         in_use -= j->footprint;
