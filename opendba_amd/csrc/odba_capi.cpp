@@ -2087,6 +2087,9 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
     off += (uint64_t)lcs[k];
   }
   cen_offs[nc] = off;
+  // accumulation positions: each cluster starts at a multiple of 64
+  long long apos = 0;
+  for (int k = 0; k < nc; k++) apos += round64(lcs[k]);
   const bool no_wrap = int_cannot_wrap<T>(seqs, lens, n);
   // resident data, member offsets per cluster, centroid pools
   std::vector<T> hb((size_t)total, T(0));
@@ -2114,7 +2117,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
   void* jobs;
   int rc;
   if ((rc = B.get(c, data, hb.size())) || (rc = B.get(c, dmoffs, moffs.size())) || (rc = B.get(c, cen, coff)) ||
-      (rc = B.get(c, newc, coff)) || (rc = B.get(c, acc, off)) || (rc = B.get(c, counts, off)) ||
+      (rc = B.get(c, newc, coff)) || (rc = B.get(c, acc, apos)) || (rc = B.get(c, counts, apos)) ||
       (rc = B.get(c, runs, roff)) || (rc = B.get(c, dslots, nc)) || (rc = B.get(c, dbits, nc)) ||
       (rc = B.get(c, derr, 1)) || (rc = B.get(c, abr, n)) || (rc = B.get(c, tbj, n)))
     return rc;
@@ -2178,7 +2181,7 @@ int dba_clusters_t(odba_ctx* c, const T* const* seqs, const uint64_t* lens, int 
         sl.cen = cen + cen_off[k]; sl.newc = newc + cen_off[k];
         sl.lc = lcs[k]; sl.nmem = (int)mem[k].size(); sl.slot = k; sl.pad = 0;
         slots.push_back(sl);
-        pos += lcs[k];
+        pos += round64(lcs[k]);
         for (size_t q = 0; q < mem[k].size(); q++) {
           const int b = mem[k][q];
           const int flip = (oe && lcs[k] < hl[b]) ? 1 : 0;
@@ -2547,7 +2550,7 @@ template <typename T> struct RoundEngine {
         m.lc = J.P->lc; m.nmem = J.P->n; m.slot = (int)j; m.pad = 0;
         sl[v].push_back(m);
       }
-      pos += J.P->lc;
+      pos += round64(J.P->lc);
     }
     // The group that takes longest starts first: a group launched earlier
     // takes CUs that a later, longer group then waits for.

@@ -697,8 +697,8 @@ __global__ void __launch_bounds__(256) k_stripe_traceback(const TbJob* __restric
 // a += p[hi], p[hi-1], ..., p[lo], in that order. The values are loaded
 // eight at a time before they are added, so a long run costs one memory
 // latency per eight values instead of one per value.
-template <typename T>
-__device__ __forceinline__ T add_run_desc(T a, const T* __restrict__ p, int hi, int lo) {
+template <typename T, typename P = const T*>
+__device__ __forceinline__ T add_run_desc(T a, P __restrict__ p, int hi, int lo) {
   for (int k = hi; k >= lo; k -= 8) {
     T v[8];
 #pragma unroll
@@ -779,11 +779,31 @@ __device__ __forceinline__ void accumulate_runs(T& a, uint32_t& n, const int4* _
   }
 }
 
-__device__ __forceinline__ int wave_min_i(int v) {
+// Minimum / maximum over the 64 lanes as a wave-uniform value, by DPP
+// within the VALU: pairs, quads, half rows and rows combined in registers
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// then row 0 into row 1 and row 2 into row 3 (row_bcast:15), rows 0-1 into
+// rows 2-3 (row_bcast:31); lane 63 then holds the whole wave's result. A
+// lane that a masked row broadcast does not write keeps its own value.
+// Every lane of the wave must call it.
+template <bool MAX> __device__ __forceinline__ int wave_reduce_i(int v) {
+  auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_i<false>(v); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_i<true>(v); }
+// The same by cross-lane shuffles (LDS permutes), the result in every lane.
+__device__ __forceinline__ int wave_min_shfl(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
 }
-__device__ __forceinline__ int wave_max_i(int v) {
+__device__ __forceinline__ int wave_max_shfl(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
   return v;
 }
@@ -804,11 +824,23 @@ template <typename T> constexpr int stage_windows() { return sizeof(T) == 4 ? 32
 // together up to stage_windows<T>() of them (one memory latency per
 // batch); a batch whose ranges need more loads the rest one window at a
 // time, each one window ahead of the adds. buf holds (stage_windows<T>() + 1) * kStageW
-// values; every lane of the block's one wave must call this.
-template <typename T>
-__device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride, int nseq,
-                                  int nall, const T* __restrict__ data, const long long* __restrict__ offs,
-                                  T* __restrict__ buf) {
+// values; every lane of the block's one wave must call this. It is inlined
+// into its kernels so that buf is known to be LDS (ds_read / ds_write, not
+// flat accesses through a generic pointer).
+// Global-memory pointers for data read through pointers held in memory
+// (the multi-cluster slot table): without the address space the compiler
+// must use flat accesses with 64-bit lane addresses.
+template <typename T> using gptr = const T __attribute__((address_space(1)))*;
+
+// DPPRED: the batch's range reductions by DPP (wave-uniform results) or by
+// shuffles. Measured on gfx950, DPP took the single-cluster kernel from
+// 4.67 to 4.37 ms on a 2,000-sequence round, and shuffles kept the
+// multi-cluster kernel at 6.1 ms against 6.8 ms with DPP on a 19-job
+// round, so each kernel takes its faster form (the sums are the same).
+template <typename T, typename DP = const T*, typename OP = const long long*, bool DPPRED = true>
+__device__ __forceinline__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ runs, long long stride,
+                                                  int nseq, int nall, DP __restrict__ data, OP __restrict__ offs,
+                                                  T* __restrict__ buf) {
   constexpr int B = kStageB, SW = kStageW, NW = stage_windows<T>();
   const int lane = lane_id();
   int4 rn[B];
@@ -828,8 +860,8 @@ __device__ void accumulate_staged(T& a, uint32_t& n, const int4* __restrict__ ru
       r[u] = rn[u];
       o[u] = on[u];
       const bool ne = r[u].y >= r[u].x;
-      lo[u] = wave_min_i(ne ? r[u].x : INT32_MAX);
-      hi[u] = wave_max_i(ne ? r[u].y : INT32_MIN);
+      lo[u] = DPPRED ? wave_min_i(ne ? r[u].x : INT32_MAX) : wave_min_shfl(ne ? r[u].x : INT32_MAX);
+      hi[u] = DPPRED ? wave_max_i(ne ? r[u].y : INT32_MIN) : wave_max_shfl(ne ? r[u].y : INT32_MIN);
       nw[u] = hi[u] >= lo[u] ? (hi[u] - lo[u]) / SW + 1 : 0;
       wb0[u] = tot;
       tot += nw[u];
@@ -1012,7 +1044,9 @@ __global__ void k_mean_delta(const T* __restrict__ acc, const uint32_t* __restri
 // Several clusters' DBA rounds in one launch each: a table of the active
 // clusters (sorted by first position) maps a global centroid position to
 // its cluster; each position is accumulated and averaged exactly as in the
-// single-cluster kernels above.
+// single-cluster kernels above. Callers start each cluster at a multiple of
+// 64 positions, so no 64-lane block straddles two clusters; the positions
+// between a cluster's lc and the next cluster's start hold nothing.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ int slot_of(const MultiSlot* __restrict__ sl, int ns, long long g) {
@@ -1025,7 +1059,9 @@ __device__ __forceinline__ int slot_of(const MultiSlot* __restrict__ sl, int ns,
 }
 
 // One 64-lane block per 64 global positions: a block inside one cluster
-// takes the staged path, one that straddles clusters adds lane by lane.
+// takes the staged path (a long run costs one memory latency per 256-value
+// window), one that straddles clusters adds lane by lane (one latency per
+// eight values of a run).
 template <typename T>
 __global__ void __launch_bounds__(64) k_accumulate_multi(const MultiSlot* __restrict__ sl, int ns, long long total,
                                                          T* __restrict__ acc, uint32_t* __restrict__ counts) {
@@ -1035,14 +1071,24 @@ __global__ void __launch_bounds__(64) k_accumulate_multi(const MultiSlot* __rest
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = g < total;
   const int si = slot_of<T>(sl, ns, on ? g : total - 1);
-  const MultiSlot S = sl[si];
-  const int c = on ? (int)(g - S.pos0) : 0;
+  const int su = wave_min_i(si);  // wave-uniform
   T a = T(0);
   uint32_t n = 0;
-  if (wave_min_i(si) == wave_max_i(si))
-    accumulate_staged<T>(a, n, S.runs + c, S.lc, on ? S.nmem : 0, S.nmem, (const T*)S.data, S.moffs, buf);
-  else
-    accumulate_runs<T>(a, n, S.runs + c, S.lc, on ? S.nmem : 0, (const T*)S.data, S.moffs);
+  if (su == wave_max_i(si)) {
+    // One cluster for the whole wave: its slot is read once into scalar
+    // registers. Lanes in the cluster's padding (positions lc .. the next
+    // multiple of 64) hold no sequences.
+    const MultiSlot S = sl[su];
+    const int c = on ? (int)(g - S.pos0) : 0;
+    const bool live = on && c < S.lc;
+    accumulate_staged<T, gptr<T>, gptr<long long>, false>(a, n, S.runs + (live ? c : 0), S.lc, live ? S.nmem : 0,
+                                                          S.nmem, (gptr<T>)S.data, (gptr<long long>)S.moffs, buf);
+  } else {
+    const MultiSlot S = sl[si];
+    const int c = on ? (int)(g - S.pos0) : 0;
+    const bool live = on && c < S.lc;
+    accumulate_runs<T>(a, n, S.runs + (live ? c : 0), S.lc, live ? S.nmem : 0, (const T*)S.data, S.moffs);
+  }
   if (on) {
     acc[g] = a;
     counts[g] = n;
@@ -1059,8 +1105,8 @@ __global__ void k_mean_delta_multi(const T* __restrict__ acc, const uint32_t* __
   const bool on = g < total;
   const MultiSlot S = sl[slot_of<T>(sl, ns, on ? g : total - 1)];
   double d = -1.0;  // no delta
-  if (on) {
-    const int c = (int)(g - S.pos0);
+  const int c = on ? (int)(g - S.pos0) : 0;
+  if (on && c < S.lc) {  // not in the cluster's padding
     T m;
     if (!IsFloat<T>::v && counts[g] == 0) {
       atomicOr(err, 2);

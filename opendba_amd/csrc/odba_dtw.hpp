@@ -13,6 +13,9 @@ namespace odba {
 // Padded element count of a stored sequence: every swath may read one full
 // 1024-wide column block and the row window 128 entries past the end.
 inline long long padded_len(long long L) { return ((L + 63) / 64) * 64 + 1024; }
+// Positions of one cluster in a packed accumulation: lc rounded up to a
+// whole 64-lane block.
+inline long long round64(long long L) { return (L + 63) / 64 * 64; }
 
 // Waves of a path-strip or swath-band workgroup at most: 16 caps a path
 // kernel at 128 VGPRs, four waves per SIMD. With the step bits in two
