@@ -974,18 +974,21 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
     const int ring = std::max(128, env_int("ODBA_BAND_RING", 2048));
     const long long gstride = maxly + 256;
     const size_t ring_bytes = (size_t)nblocks * nw * ring * es, slot_bytes = (size_t)R * gstride * es;
+    // open end without open start: one boundary edge column per job
+    const size_t bnd_bytes = (oe && !os) ? (size_t)njobs * gstride * es : 0;
     const size_t sync_bytes = path_band_sync_bytes(R, max_claims, njobs);
-    int rc = ensure_scratch(c, ring_bytes + slot_bytes + sync_bytes + 256);
+    int rc = ensure_scratch(c, ring_bytes + slot_bytes + bnd_bytes + sync_bytes + 256);
     if (rc) return rc;
     char* base = (char*)c->esc;
-    void* sync = (void*)(((uintptr_t)(base + ring_bytes + slot_bytes) + 255) & ~(uintptr_t)255);
+    void* bslots = bnd_bytes ? (void*)(base + ring_bytes + slot_bytes) : nullptr;
+    void* sync = (void*)(((uintptr_t)(base + ring_bytes + slot_bytes + bnd_bytes) + 255) & ~(uintptr_t)255);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 100000;
     const long long limit = (long long)env_int("ODBA_BAND_WAIT_S", 60) * 1000LL * khz;
     unsigned long long* counter64 = (unsigned long long*)((char*)c->small + 64);
     HIPC(c, launch_fill_u32(c->stream, (unsigned int*)counter64, 0u, 2));
     e = launch_path_swath_bands(nw, os, oe, nblocks, c->stream, jobs_dev, njobs, max_claims, counter64, mv, mtop, abr,
-                                base, ring, base + ring_bytes, gstride, sync, R, err, limit);
+                                base, ring, base + ring_bytes, gstride, sync, R, err, limit, bslots);
   } else if (bands) {
     const long long max_claims = (long long)njobs * std::max(1, kmax);
     const int nblocks = (int)std::max(1LL, std::min(max_claims, 4LL * c->cus));

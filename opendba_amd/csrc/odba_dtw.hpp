@@ -152,11 +152,13 @@ hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const 
                              int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
                              int* err, long long tick_limit);
 // Path jobs as bands of nw whole swaths over the whole device (float);
-// njobs <= 1024, max_claims >= total bands; timeouts set err bit 8.
+// max_claims >= total bands; timeouts set err bit 8. In open end without
+// open start, bslots holds njobs boundary edge columns of gstride values
+// (null otherwise).
 hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                                    long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
                                    int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
-                                   int* err, long long tick_limit);
+                                   int* err, long long tick_limit, void* bslots);
 hipError_t launch_path_strips(int waves, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                               unsigned int* counter, uint32_t* mv, uint32_t* mtop, int* abr, void* esc,
                               long long stride, int quarters);

@@ -1361,9 +1361,11 @@ __global__ void __launch_bounds__(64 * kSmallMaxWaves) k_all_pairs_small(
 }
 
 __global__ void k_band_init(int* __restrict__ g_cnt, unsigned long long* __restrict__ g_free, int R,
-                            int* __restrict__ slot_of, long long nclaims, int* __restrict__ abr, int ntab) {
+                            int* __restrict__ slot_of, long long nclaims, int* __restrict__ abr, int ntab,
+                            int* __restrict__ bcnt = nullptr) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < R) g_cnt[i] = 0;
+  if (bcnt && i < ntab) bcnt[i] = 0;
   if (i < (R + 63) / 64) {
     const int rem = R - (int)i * 64;
     g_free[i] = rem >= 64 ? ~0ull : ((1ull << rem) - 1);
@@ -1556,34 +1558,64 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_strips(const Path
 // open-end stop (dtw.hpp:74-129) into abr[job] (INT_MAX: none).
 // band0[p] = first claim of the p-th job in claim order; band0[njobs] = all.
 // ---------------------------------------------------------------------------
+// Bands of a job claimed in the first phase. In open end without open
+// start the sweep may stop after any swath whose right edge passes the last
+// row ((s + 1) * W > ly, dtw.hpp:74-129); a job's bands after the one
+// holding the first such swath are claimed in a second phase, after every
+// job's first-phase bands, so that most of them find the stop already
+// decided and sweep nothing (claimed left to right right away, they ran a
+// few rows behind their left neighbours and were nearly done by the time
+// the stop was known).
+__host__ __device__ inline int first_phase_bands(int ly, int K, int per_band, bool stop_rule) {
+  const int nb = (K + per_band - 1) / per_band;
+  if (!stop_rule) return nb;
+  const int ks = ly / ODBA_W;  // first swath whose right edge passes the last row
+  return min(nb, ks / per_band + 1);
+}
+
+// band0[p] = first claim of the p-th job in claim order (its first-phase
+// bands), band0[njobs] = first-phase claims; with band1, band1[p] = first
+// claim of its second-phase bands (all after band0[njobs]) and
+// band1[njobs] = all claims.
 __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
-                                                     long long* __restrict__ band0, int per_band) {
+                                                     long long* __restrict__ band0, int per_band,
+                                                     long long* __restrict__ band1 = nullptr, int stop_rule = 0) {
   // One workgroup: thread t sums the bands of its chunk of claim-order jobs,
-  // the chunk sums are scanned, then each thread writes its chunk's prefixes.
+  // the chunk sums are scanned, then each thread writes its chunk's prefixes;
+  // once per phase.
   __shared__ long long s[1024];
   const int t = threadIdx.x;
   const int per = (njobs + 1023) / 1024;
   const int p0 = min(njobs, t * per), p1 = min(njobs, p0 + per);
-  auto bands = [&](int p) -> long long {
-    const int K = (jobs[njobs - 1 - p].lx + ODBA_W - 1) / ODBA_W;  // swaths of the job
-    return (long long)((K + per_band - 1) / per_band);
+  auto bands = [&](int p, int ph) -> long long {
+    const PathJob<float>& jb = jobs[njobs - 1 - p];
+    const int K = (jb.lx + ODBA_W - 1) / ODBA_W;  // swaths of the job
+    const int nb = (K + per_band - 1) / per_band;
+    const int n1 = band1 ? first_phase_bands(jb.ly, K, per_band, stop_rule != 0) : nb;
+    return (long long)(ph ? nb - n1 : n1);
   };
-  long long mine = 0;
-  for (int p = p0; p < p1; p++) mine += bands(p);
-  s[t] = mine;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const long long a = t >= o ? s[t - o] : 0;
+  long long base = 0;
+  for (int ph = 0; ph < (band1 ? 2 : 1); ph++) {
+    long long* out = ph ? band1 : band0;
+    long long mine = 0;
+    for (int p = p0; p < p1; p++) mine += bands(p, ph);
+    __syncthreads();  // every thread has read the previous phase's total
+    s[t] = mine;
     __syncthreads();
-    s[t] += a;
-    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const long long a = t >= o ? s[t - o] : 0;
+      __syncthreads();
+      s[t] += a;
+      __syncthreads();
+    }
+    long long run = base + s[t] - mine;
+    for (int p = p0; p < p1; p++) {
+      out[p] = run;
+      run += bands(p, ph);
+    }
+    if (t == 1023) out[njobs] = base + s[t];
+    base += s[1023];
   }
-  long long run = s[t] - mine;
-  for (int p = p0; p < p1; p++) {
-    band0[p] = run;
-    run += bands(p);
-  }
-  if (t == 1023) band0[njobs] = s[t];
 }
 
 __global__ void k_abr_index(int* __restrict__ abr, int n) {
@@ -1730,8 +1762,11 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
                                                            float* __restrict__ gslots, long long gstride,
                                                            int* __restrict__ g_cnt, unsigned long long* __restrict__ g_free,
                                                            int R, int* __restrict__ slot_of, int* __restrict__ abr,
-                                                           int* __restrict__ err, long long limit) {
+                                                           int* __restrict__ err, long long limit,
+                                                           const long long* __restrict__ band1,
+                                                           float* __restrict__ bslots, int* __restrict__ bcnt) {
   using T = float;
+  constexpr bool kStop = OE && !OS;  // the open-end stop rule applies
   __shared__ T s_y[kPathStripWaves][128];
   __shared__ T s_e[kPathStripWaves][64];
   __shared__ T s_o[kPathStripWaves][128];
@@ -1743,7 +1778,8 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
   const int nw = uniform_i32((int)(blockDim.x >> 6));
   T* rbase = rings + (long long)blockIdx.x * nw * ring;
   const WaveWin<T> win{s_y[wid], s_e[wid], s_o[wid]};
-  const long long nclaims = band0[njobs];
+  const long long nfirst = band0[njobs];                    // first-phase claims
+  const long long nclaims = kStop ? band1[njobs] : nfirst;  // all claims
   for (;;) {
     __syncthreads();  // every wave is done with the previous claim
     if (threadIdx.x == 0) s_q = atomicAdd(counter, 1ull);
@@ -1754,10 +1790,13 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
     __syncthreads();
     const long long q = (long long)uniform_u64(s_q);
     if (q >= nclaims) break;
-    int lo = 0, hi = njobs - 1;  // claim-order position: last with band0 <= q
+    // claim-order position: last with (first- or second-phase) start <= q
+    const bool second = q >= nfirst;
+    const long long* tabq = second ? band1 : band0;
+    int lo = 0, hi = njobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (band0[mid] <= q) lo = mid;
+      if (tabq[mid] <= q) lo = mid;
       else hi = mid - 1;
     }
     const int pos = uniform_i32(lo);
@@ -1765,7 +1804,15 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
     const PathJob<T> jb = jobs[j];
     const int ly = jb.ly, lx = jb.lx;
     const int K = (lx + ODBA_W - 1) / ODBA_W;
-    const int b = uniform_i32((int)(q - band0[pos]));  // band of the job
+    const int nb = (K + nw - 1) / nw;
+    const int n1 = first_phase_bands(ly, K, nw, kStop);                   // the job's first-phase bands
+    const int b = uniform_i32((int)(q - tabq[pos]) + (second ? n1 : 0));  // band of the job
+    // The hand-off between the job's last first-phase band and its first
+    // second-phase band goes through the job's own boundary slot (edge
+    // column and progress word), not the shared pool: the two claims are far
+    // apart, and pool slots held that long could leave the pool empty.
+    const bool bnd_out = kStop && b == n1 - 1 && n1 < nb;
+    const bool bnd_in = kStop && b == n1 && b > 0;
     const int s0 = b * nw;
     const int slast = min(s0 + nw, K) - 1;
     const int s = s0 + wid;  // this wave's swath
@@ -1789,8 +1836,10 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
     // below (one wave per band is both the consumer of the left band's slot
     // and the producer of its own, and may learn only after announcing that
     // its left neighbour was past the stop)
-    const bool took = out_global && !skip;
-    if (out_global) {  // take a slot, reset its progress word, then announce it
+    const bool pool_out = out_global && !bnd_out;
+    const bool pool_in = in_global && !bnd_in;
+    const bool took = pool_out && !skip;
+    if (pool_out) {  // take a slot, reset its progress word, then announce it
       if (took) gout = band_pop_slot(g_free, (R + 63) / 64, err, limit, lane);
       if (lane == 0) {
         if (took) __hip_atomic_store(&g_cnt[gout], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1798,7 +1847,7 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
       }
     }
     bool had_slot = false;
-    if (in_global && !band_failed(err)) {  // the left band's slot, or -2: none
+    if (pool_in && !band_failed(err)) {  // the left band's slot, or -2: none
       const long long t0 = wall_clock64();
       for (;;) {
         gin = uniform_i32(__hip_atomic_load(&slot_of[q - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
@@ -1812,10 +1861,13 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
         gin = 0;
       }
     }
+    // A boundary consumer waits on the boundary progress word and leaves
+    // that wait when its producer is past the stop: the stop word then
+    // reaches this strip's swath too.
     pp.in_global = in_global;
-    pp.in_cnt = in_global ? &g_cnt[gin] : (s > 0 ? &s_cnt[wid - 1] : nullptr);
+    pp.in_cnt = in_global ? (bnd_in ? &bcnt[pos] : &g_cnt[gin]) : (s > 0 ? &s_cnt[wid - 1] : nullptr);
     pp.in_used = (s > 0 && !in_global) ? &s_used[wid - 1] : nullptr;
-    pp.out_cnt = has_out ? (out_global ? &g_cnt[gout] : &s_cnt[wid]) : nullptr;
+    pp.out_cnt = has_out ? (out_global ? (bnd_out ? &bcnt[pos] : &g_cnt[gout]) : &s_cnt[wid]) : nullptr;
     pp.out_global = out_global;
     pp.out_used = (has_out && !out_global) ? &s_used[wid] : nullptr;
     pp.ring = ring;
@@ -1827,9 +1879,14 @@ __global__ void __launch_bounds__(64 * kPathStripWaves) k_path_swath_bands(const
     const size_t swath_words = (size_t)(nrows_main > 0 ? nrows_main + 63 : 0) * 64;
     SwathIO<T> io;
     io.y = jb.y; io.ly = ly; io.x = jb.x; io.lx = lx; io.off = s * ODBA_W;
-    io.ein = s == 0 ? nullptr : (in_global ? gslots + (long long)gin * gstride : rbase + (long long)(wid - 1) * ring);
+    float* const bslot = kStop ? bslots + (long long)pos * gstride : nullptr;
+    io.ein = s == 0 ? nullptr
+                    : (in_global ? (bnd_in ? bslot : gslots + (long long)gin * gstride)
+                                 : rbase + (long long)(wid - 1) * ring);
     io.imask = in_global ? -1 : ring - 1;
-    io.eout = has_out ? (out_global ? gslots + (long long)gout * gstride : rbase + (long long)wid * ring) : nullptr;
+    io.eout = has_out ? (out_global ? (bnd_out ? bslot : gslots + (long long)gout * gstride)
+                                    : rbase + (long long)wid * ring)
+                      : nullptr;
     io.omask = out_global ? -1 : ring - 1;
     io.mv = mv + jb.mv_off + (size_t)s * swath_words;
     io.mtop = mtop + jb.top_off + (size_t)s * 64;
@@ -2530,7 +2587,9 @@ hipError_t launch_all_pairs_bands(int dtype, int C, int waves, int os, int oe, i
 size_t band_pair_size() { return sizeof(BandPair); }
 
 size_t path_band_sync_bytes(int R, long long max_claims, int njobs) {
-  return (size_t)((R + 63) / 64) * 8 + (size_t)R * 4 + (size_t)max_claims * 4 + 64 + (size_t)(njobs + 1) * 8;
+  // g_free, g_cnt, slot_of, then (64-aligned) band0 and band1, then bcnt
+  return (size_t)((R + 63) / 64) * 8 + (size_t)R * 4 + (size_t)max_claims * 4 + 64 + (size_t)(njobs + 1) * 16 +
+         (size_t)njobs * 4;
 }
 
 hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
@@ -2593,24 +2652,30 @@ hipError_t launch_sck_step(int os, int oe, hipStream_t st, const SckJob* jobs, i
 hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStream_t st, const void* jobs, int njobs,
                                    long long max_claims, unsigned long long* counter, uint32_t* mv, uint32_t* mtop,
                                    int* abr, void* rings, int ring, void* gslots, long long gstride, void* sync, int R,
-                                   int* err, long long tick_limit) {
+                                   int* err, long long tick_limit, void* bslots) {
+  const bool stop_rule = oe && !os;
   if (njobs < 1 || nw < 1 || nw > kPathStripWaves || ring < 128 || (ring & (ring - 1)) ||
-      R < 2 * nblocks + 2)
+      R < 2 * nblocks + 2 || (stop_rule && !bslots))
     return hipErrorInvalidValue;
-  // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1] (8-aligned)
+  // sync: g_free (u64 words), g_cnt[R], slot_of[max_claims], band0[njobs + 1]
+  // (64-aligned), band1[njobs + 1], bcnt[njobs]; bslots: njobs boundary
+  // edge columns of gstride values (open end without open start only)
   unsigned long long* g_free = (unsigned long long*)sync;
   int* g_cnt = (int*)(g_free + (R + 63) / 64);
   int* slot_of = g_cnt + R;
   long long* band0 = (long long*)(((uintptr_t)(slot_of + max_claims) + 63) & ~(uintptr_t)63);
+  long long* band1 = band0 + njobs + 1;
+  int* bcnt = (int*)(band1 + njobs + 1);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
-  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, nw);
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, nw, band1, stop_rule ? 1 : 0);
   const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
   hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
-                     max_claims, abr, njobs);
+                     max_claims, abr, njobs, bcnt);
   dim3 g(nblocks), b(64 * nw);
   float* rg = (float*)rings;
   float* gs = (float*)gslots;
-#define ODBA_PSB(A, B) hipLaunchKernelGGL((k_path_swath_bands<A, B>), g, b, 0, st, J, njobs, band0, counter, mv, mtop, rg, ring, gs, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit)
+  float* bs = (float*)bslots;
+#define ODBA_PSB(A, B) hipLaunchKernelGGL((k_path_swath_bands<A, B>), g, b, 0, st, J, njobs, band0, counter, mv, mtop, rg, ring, gs, gstride, g_cnt, g_free, R, slot_of, abr, err, tick_limit, band1, bs, bcnt)
   if (os && oe) ODBA_PSB(true, true); else if (os) ODBA_PSB(true, false); else if (oe) ODBA_PSB(false, true); else ODBA_PSB(false, false);
 #undef ODBA_PSB
   hipLaunchKernelGGL(k_abr_index, dim3((njobs + 255) / 256), dim3(256), 0, st, abr, njobs);
