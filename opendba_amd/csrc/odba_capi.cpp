@@ -953,6 +953,13 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   // strip workgroup has waves (16): its swaths would run one after another
   // on one CU. Bands of whole swaths spread them over several workgroups.
   bool swath_bands = njobs < 4 * c->cus && njobs <= 1024 && kmax > 16 && minly >= 64;
+  // Open end without open start, any number of jobs wider than 16 swaths:
+  // the second-phase claims leave most bands past each job's stop with
+  // nothing to sweep, which strips (pipelined inside one workgroup) cannot
+  // do. 2,000 squiggles against an 18-swath centroid: 43.4 ms per round at
+  // four bands per CU, 45.9 ms as strips.
+  const bool many_open_end = !swath_bands && oe && !os && kmax > 16 && minly >= 64;
+  if (many_open_end) swath_bands = true;
   if (bands_per_cu > 0) swath_bands = maxlx > 0 && minly >= 64;
   if (env && std::strcmp(env, "swathband")) swath_bands = false;
   if (env && !std::strcmp(env, "swathband")) swath_bands = maxlx > 0;
@@ -967,7 +974,8 @@ int run_path_jobs(odba_ctx* c, int dtype, bool minfirst, int os, int oe, const v
   hipError_t e;
   if (swath_bands) {
     const int nw = std::max(1, std::min(kPathStripWaves, env_int("ODBA_PATH_BAND_WAVES", 4)));
-    const int per_cu = std::max(1, bands_per_cu > 0 ? bands_per_cu : env_int("ODBA_PATH_BANDS_PER_CU", 2));
+    const int per_cu =
+        std::max(1, bands_per_cu > 0 ? bands_per_cu : env_int("ODBA_PATH_BANDS_PER_CU", many_open_end ? 4 : 2));
     const long long max_claims = (long long)njobs * ((kmax + nw - 1) / nw);
     const int nblocks = (int)std::max(1LL, std::min(max_claims, (long long)per_cu * c->cus));
     const int R = 2 * nblocks + 2;
