@@ -416,8 +416,14 @@ __global__ void __launch_bounds__(256) k_traceback(const TbJob* __restrict__ job
         }
         const int4 one = cnt ? make_int4(0, 0, 1, 0) : make_int4(1, 0, 1, 0);
         const int4 first = rr.out();
+        // per component: a select of whole int4 values by lane compiles to a
+        // two-entry private array and a scratch load per store
         for (int b = 0; b < j; b += 64)
-          if (b + lane < j) R[j - b - lane] = (b + lane == 0) ? first : one;
+          if (b + lane < j) {
+            const bool f = b + lane == 0;
+            R[j - b - lane] = make_int4(f ? first.x : one.x, f ? first.y : one.y, f ? first.z : one.z,
+                                        f ? first.w : one.w);
+          }
         curc = 0;
         rr.reset();
       }
