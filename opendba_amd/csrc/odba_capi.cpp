@@ -2356,7 +2356,12 @@ template <typename T> struct RoundEngine {
   // CUs instead of setting the step's length on one workgroup.
   static constexpr int kGroups = 4;
   static int group_of(int K) { return K <= 1 ? 0 : (K <= 8 ? 1 : (K <= 16 ? 2 : 3)); }
-  static int bands_per_cu(int g) { return g == 3 ? 4 : 0; }
+  // Swath bands (four per CU) for the group wider than 16 swaths and, in
+  // open end without open start, for the group of 9-16 swaths too: their
+  // second-phase claims skip bands past each job's stop, which strips
+  // inside one workgroup cannot (13 open-end serving jobs: 82-83 ms per
+  // step alone against 85-87 ms with that group as strips).
+  int bands_per_cu(int g) const { return (g == 3 || (g == 2 && oe && !os)) ? 4 : 0; }
 
   odba_ctx* c = nullptr;
   odba_ctx* sub[kGroups] = {};
