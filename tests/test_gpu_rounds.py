@@ -160,6 +160,25 @@ def test_packed_rounds_long_jobs_take_swath_bands(oe):
         _same(p, q)
 
 
+@pytest.mark.parametrize("oe", [0, 1])
+def test_packed_rounds_nine_to_sixteen_swaths(oe):
+    """Jobs whose widest path matrices hold 9-16 swaths (sequences of
+    9k-16k samples): in open end that group runs as swath bands with
+    second-phase claims inside the packed step, in global mode as strips;
+    results equal the plain path (strips), with a short job beside them."""
+    rng = np.random.default_rng(71 + oe)
+    mid_job = [np.cumsum(rng.normal(size=int(L))).astype(np.float32) for L in rng.integers(9000, 16000, size=4)]
+    short_job = [np.cumsum(rng.normal(size=int(L))).astype(np.float32) for L in rng.integers(300, 2000, size=5)]
+    mixed_job = mid_job[:2] + short_job[:3]
+    jobs = [mid_job, short_job, mixed_job]
+    print(f"packed rounds with 9-16 swath jobs oe={oe}", flush=True)
+    with N.Context(0) as ctx, N.Context(0) as ectx:
+        plain = [ctx.perform_dba(s, 0, oe, norm=1, max_rounds=4) for s in jobs]
+        packed = _run_packed(ctx, ectx, jobs, 0, oe, 4, False, join_every=1)
+    for p, q in zip(plain, packed):
+        _same(p, q)
+
+
 def _swath_words(ly, os_, oe):
     """Words of one 1024-column swath: 64 lanes x the rows of its skewed main
     loop rounded up to a multiple of 64 (the last row of an open-end swath
