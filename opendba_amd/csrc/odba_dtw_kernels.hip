@@ -1577,15 +1577,16 @@ __host__ __device__ inline int first_phase_bands(int ly, int K, int per_band, bo
 // bands), band0[njobs] = first-phase claims; with band1, band1[p] = first
 // claim of its second-phase bands (all after band0[njobs]) and
 // band1[njobs] = all claims.
-__global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(64) k_path_band0(const PathJob<float>* __restrict__ jobs, int njobs,
                                                      long long* __restrict__ band0, int per_band,
                                                      long long* __restrict__ band1 = nullptr, int stop_rule = 0) {
-  // One workgroup: thread t sums the bands of its chunk of claim-order jobs,
-  // the chunk sums are scanned, then each thread writes its chunk's prefixes;
-  // once per phase.
-  __shared__ long long s[1024];
+  // One wave (it needs one free wave slot, not a whole CU, so it starts as
+  // soon as any wave of another stream's kernel retires): thread t sums the
+  // bands of its chunk of claim-order jobs, the chunk sums are scanned, then
+  // each thread writes its chunk's prefixes; once per phase.
+  __shared__ long long s[64];
   const int t = threadIdx.x;
-  const int per = (njobs + 1023) / 1024;
+  const int per = (njobs + 63) / 64;
   const int p0 = min(njobs, t * per), p1 = min(njobs, p0 + per);
   auto bands = [&](int p, int ph) -> long long {
     const PathJob<float>& jb = jobs[njobs - 1 - p];
@@ -1602,7 +1603,7 @@ __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __res
     __syncthreads();  // every thread has read the previous phase's total
     s[t] = mine;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
+    for (int o = 1; o < 64; o <<= 1) {
       const long long a = t >= o ? s[t - o] : 0;
       __syncthreads();
       s[t] += a;
@@ -1613,8 +1614,8 @@ __global__ void __launch_bounds__(1024) k_path_band0(const PathJob<float>* __res
       out[p] = run;
       run += bands(p, ph);
     }
-    if (t == 1023) out[njobs] = base + s[t];
-    base += s[1023];
+    if (t == 63) out[njobs] = base + s[t];
+    base += s[63];
   }
 }
 
@@ -2603,7 +2604,7 @@ hipError_t launch_path_bands(int os, int oe, int nblocks, hipStream_t st, const 
   int* slot_of = g_cnt + R;
   long long* band0 = (long long*)(((uintptr_t)(slot_of + max_claims) + 63) & ~(uintptr_t)63);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
-  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, 1);
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(64), 0, st, J, njobs, band0, 1);
   const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
   hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
                      max_claims, abr, njobs);
@@ -2667,7 +2668,7 @@ hipError_t launch_path_swath_bands(int nw, int os, int oe, int nblocks, hipStrea
   long long* band1 = band0 + njobs + 1;
   int* bcnt = (int*)(band1 + njobs + 1);
   const PathJob<float>* J = (const PathJob<float>*)jobs;
-  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(1024), 0, st, J, njobs, band0, nw, band1, stop_rule ? 1 : 0);
+  hipLaunchKernelGGL(k_path_band0, dim3(1), dim3(64), 0, st, J, njobs, band0, nw, band1, stop_rule ? 1 : 0);
   const long long ninit = std::max<long long>(std::max<long long>(R, max_claims), njobs);
   hipLaunchKernelGGL(k_band_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, g_cnt, g_free, R, slot_of,
                      max_claims, abr, njobs, bcnt);
